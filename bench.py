@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the delay-and-sum hot path on one MI355X.
+
+Workload (one step = one beamforming pass over one acquisition): C2, the
+compounded CPWC IQ configuration -- 128-element linear array, 75 plane waves
+(-16..16 deg), 1024 IQ samples at 10.4 MHz, 512 x 1024 pixels, cubic
+interpolation, phase rotation, tx/rx Tukey-25 apodization at f# 1.7,
+dimension both. Delays/apodization are computed on the device (geometry plan);
+channel data is resident in HBM before the timed region.
+
+Prints one JSON line: Gsamples/s (pixel x channel x wave per second), the
+kernel's roofline position (HIP events on the launch stream), the error
+against the das_c restatement and an FP64 restatement on a pixel sample, and
+the das_c restatement timed on the host cores on a bounded pixel sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+VALU_PEAK_TFLOPS = 157.3     # MI355X published FP32 vector peak
+HBM_PEAK_GBS = 8000.0        # MI355X published HBM3E peak
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--phase", default="separable", choices=["separable", "dasc"])
+    ap.add_argument("--wave-chunk", type=int, default=0)
+    ap.add_argument("--split", type=int, default=0)
+    return ap.parse_args()
+
+
+def flops_per_unit(interp: int, iq: bool) -> int:
+    """Algorithmic FP32 flops of one active (pixel, channel, wave) term:
+    fractional index (c = tx + rx, (c - t0) * fs: 3), interpolation of a
+    complex sample (linear: 2 complex-by-real FMAs = 8; cubic: Keys weights 16
+    + 4 complex-by-real FMAs 16 = 32), phase rotation and accumulation
+    (one complex multiply-add: 8; RF: a real weight, 4)."""
+    return 3 + (32 if interp == 1 else 8) + (8 if iq else 4)
+
+
+def active_units(plan, torch) -> int:
+    """Number of (pixel, channel, wave) terms with nonzero apodization: the
+    terms das_c evaluates (das_c.cpp:366,371)."""
+    ta, ra, _, _ = plan.materialize()
+    nt = (ta != 0).sum(0, dtype=torch.int64)
+    nr = (ra != 0).sum(0, dtype=torch.int64)
+    return int((nt * nr).sum().item()) * plan.F
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus %d is not supported (a frame/"
+                                   "acquisition-parallel multi-GPU layout is described in DESIGN.md)" % args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from ustb_amd import configs
+    from ustb_amd.geometry import make_plan
+
+    dev = torch.device("cuda", 0)
+    t_gen = time.time()
+    acq = configs.BUILDERS[args.config]()
+    print(f"[bench] {acq.name}: data {acq.data.shape} generated in {time.time() - t_gen:.1f}s", file=sys.stderr,
+          flush=True)
+    ch = torch.from_numpy(acq.data).to(dev)
+    plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=(args.phase == "separable"),
+                     **acq.plan_kwargs())
+    if args.wave_chunk or args.split:
+        plan.set_tiles(args.wave_chunk, args.split)
+    out = torch.empty(plan.out_shape, dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(args.warmup):
+        plan.run(ch, out)
+    torch.cuda.synchronize()
+
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        plan.run(ch, out)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    units = acq.units
+    value = units / (ms_per_step * 1e-3) / 1e9
+    print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step (kernel {kernel_ms:.3f} ms)", file=sys.stderr,
+          flush=True)
+
+    iq = acq.fd != 0.0
+    act = active_units(plan, torch)
+    fl = flops_per_unit(int(acq.interpolation), iq) * act
+    achieved_tflops = fl / (kernel_ms * 1e-3) / 1e12
+    in_bytes = acq.data.nbytes
+    out_bytes = int(np.prod(plan.out_shape)) * 8
+    alg_bytes = in_bytes + out_bytes
+    achieved_gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"das_geom_{args.config.lower()}_traffic.txt")
+    if os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith("traffic_bytes_per_launch"):
+                traffic = float(line.split("=")[1])
+
+    result = {
+        "metric": "DAS Gsamples/s (pixel x channel x wave)",
+        "value": round(value, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": acq.name, "pixels": acq.P, "channels": acq.probe.N_elements,
+                   "waves": len(acq.sequence), "frames": acq.n_frames, "samples": acq.n_samples,
+                   "interp": "cubic" if acq.interpolation == 1 else "linear",
+                   "dimension": int(acq.dimension), "phase": args.phase,
+                   "geometry": "on-device (delays + apodization per term)"},
+        "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": VALU_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved_tflops / VALU_PEAK_TFLOPS, 4),
+                     "traffic": traffic,
+                     "flops_per_active_unit": flops_per_unit(int(acq.interpolation), iq),
+                     "active_units": act, "kernel_ms": round(kernel_ms, 4)},
+        "hbm": {"algorithmic_bytes": alg_bytes, "achieved_GBs": round(achieved_gbs, 1),
+                "peak_GBs": HBM_PEAK_GBS, "frac": round(achieved_gbs / HBM_PEAK_GBS, 5)},
+    }
+
+    if not args.no_check:
+        result["errors"] = check_errors(acq, plan, out, ch, torch)
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(acq, plan, torch)
+    plan.close()
+    print(json.dumps(result))
+
+
+def _sample_pixels(P, k, seed=0):
+    return np.sort(np.random.default_rng(seed).choice(P, size=min(k, P), replace=False))
+
+
+def check_errors(acq, plan, out, ch, torch):
+    """das_c parity (linear mode, das_c has no cubic) and FP64 error of the
+    benchmarked configuration, on a pixel sample, relative to the peak image
+    magnitude."""
+    from oracle import oracle as O
+    from ustb_amd.geometry import make_plan
+
+    t = time.time()
+    ta, ra, td, rd = [x.cpu().numpy() for x in plan.materialize()]
+    pix = _sample_pixels(acq.P, 1024, seed=5)
+    img = out.cpu().numpy()
+    peak = float(np.nanmax(np.abs(img)))
+    res = {"pixels_sampled": int(pix.size)}
+    wd = O.dasc_wd(acq.fd)
+    ref64 = O.beamform_fp64(acq.data, acq.fs, acq.t0, ta, ra, td, rd, acq.fd, int(acq.dimension),
+                            interp="cubic" if acq.interpolation == 1 else "linear", pixels=pix, w0=wd)
+    res["vs_fp64"] = float(np.nanmax(np.abs(img[..., pix] - ref64)) / peak)
+    # linear mode of the same plan vs das_c
+    lin = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=True,
+                    **acq.plan_kwargs(interpolation=0))
+    img_lin = lin.run(ch).cpu().numpy()
+    lin.close()
+    peak_lin = float(np.nanmax(np.abs(img_lin)))
+    ref_c = O.das_c(acq.data, acq.fs, acq.t0, ta[:, pix], ra[:, pix], td[:, pix], rd[:, pix], acq.fd,
+                    int(acq.dimension))
+    res["vs_dasc_linear"] = float(np.nanmax(np.abs(img_lin[..., pix] - ref_c)) / peak_lin)
+    ref64l = O.beamform_fp64(acq.data, acq.fs, acq.t0, ta, ra, td, rd, acq.fd, int(acq.dimension),
+                             pixels=pix)
+    res["dasc_linear_vs_fp64_exact_w0"] = float(np.nanmax(np.abs(ref_c - ref64l)) / peak_lin)
+    res["check_s"] = round(time.time() - t, 1)
+    print(f"[bench] errors {res}", file=sys.stderr, flush=True)
+    return res
+
+
+def cpu_baseline(acq, plan, torch):
+    """The das_c restatement (C, pthreads over pixels like das_c's
+    tbb::parallel_for) timed on the host on a pixel sample of the same
+    workload, linear interpolation (das_c has no cubic)."""
+    from oracle import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    ta, ra, td, rd = [x.cpu().numpy() for x in plan.materialize()]
+    # bounded sample: every k-th pixel column so that one run is ~2-5 s
+    per_unit_s = 1.0 / (60e6 * threads)                   # rough prior
+    target_s = 3.0
+    frac = min(1.0, target_s / max(acq.units * per_unit_s, 1e-9))
+    k = max(1, int(round(acq.P * frac)))
+    pix = _sample_pixels(acq.P, k, seed=7)
+    args = (acq.data, acq.fs, acq.t0, np.ascontiguousarray(ta[:, pix]), np.ascontiguousarray(ra[:, pix]),
+            np.ascontiguousarray(td[:, pix]), np.ascontiguousarray(rd[:, pix]), acq.fd, int(acq.dimension))
+    O.das_c(*args, nthreads=threads)                      # warm-up
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        O.das_c(*args, nthreads=threads)
+        times.append(time.perf_counter() - t)
+    med = float(np.median(times))
+    units = pix.size * acq.probe.N_elements * len(acq.sequence) * acq.n_frames
+    return {"value": round(units / med / 1e9, 4), "unit": "Gsamples/s", "cores": threads, "kind": "port",
+            "sample": f"{pix.size} of {acq.P} pixels (all channels, waves, frames), linear interpolation, "
+                      f"median of 3 runs after 1 warm-up, {med:.2f} s per run"}
+
+
+if __name__ == "__main__":
+    main()

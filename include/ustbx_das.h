@@ -1,0 +1,256 @@
+/*
+ * ustbx_das.h — C-ABI of the MI355X (gfx950) delay-and-sum beamformer.
+ *
+ * Plain C types only: sizes, scalars and pointers. No torch, no HIP types in
+ * the signatures (streams travel as `void*`, a hipStream_t; NULL = default).
+ *
+ * Array layouts follow the reference MEX ABI (MATLAB column-major, first index
+ * fastest), i.e. in row-major terms:
+ *   channel data  ch [F][W][N][L]   complex64 (interleaved re, im)
+ *   tx_apo/tx_delay [W][P]          float32
+ *   rx_apo/rx_delay [N][P]          float32
+ *   output        out[F][W'][N'][P] complex64, N' = N for NONE/TX else 1,
+ *                                   W' = W for NONE/RX else 1
+ *
+ * Every entry point returns 0 on success or a negative USTBX_E* code; when
+ * `st` is not NULL it also receives the code and a message. Nothing longjmps
+ * and nothing aborts: a MEX gateway maps the codes to the reference's
+ * "Toolbox:SRP_SRC:*" error identifiers (ustbx_status_id()).
+ */
+#ifndef USTBX_DAS_H
+#define USTBX_DAS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define USTBX_ABI_VERSION 1
+
+/* error codes (mapped to the reference's mexErrMsgIdAndTxt identifiers,
+ * +mex/source/das_c.cpp:53-252, +mex/source/das_cuda.cu:117-124) */
+#define USTBX_OK 0
+#define USTBX_E_NRHS (-1)       /* Toolbox:SRP_SRC:nrhs        */
+#define USTBX_E_NLHS (-2)       /* Toolbox:SRP_SRC:nlhs        */
+#define USTBX_E_DIMENSIONS (-3) /* Toolbox:SRP_SRC:Dimensions  */
+#define USTBX_E_FLOAT (-4)      /* Toolbox:SRP_SRC:Float       */
+#define USTBX_E_SCALAR (-5)     /* Toolbox:SRP_SRC:Scalar      */
+#define USTBX_E_GPU (-6)        /* Toolbox:SRP_SRC:GPU         */
+#define USTBX_E_ARG (-7)        /* invalid argument (sum mode, interp, NULL) */
+#define USTBX_E_CAPACITY (-8)   /* request exceeds a stated capacity */
+#define USTBX_E_UNSUPPORTED (-9)/* feature not available on the device path */
+
+typedef struct ustbx_status {
+  int32_t code;
+  char message[256];
+} ustbx_status;
+
+/* sum modes: the reference `dimension` enum (dimension.m:27-32) */
+#define USTBX_DIM_NONE 0
+#define USTBX_DIM_RX 1
+#define USTBX_DIM_TX 2
+#define USTBX_DIM_BOTH 3
+
+/* interpolation */
+#define USTBX_INTERP_LINEAR 0 /* das_c / das_cuda / interp1 'linear' */
+#define USTBX_INTERP_CUBIC 1  /* Keys cubic convolution, a = -1/2 (no reference counterpart) */
+
+/* sample-window rule at the trace edges */
+#define USTBX_EDGE_DASC 0   /* 0 < trunc(d) < L-1           (das_c.cpp:377,381) */
+#define USTBX_EDGE_PYTHON 1 /* 0 <= floor(d) < L-1          (python das.py:233-236) */
+
+/* flags */
+#define USTBX_FLAG_DASC_ORDER 1u /* one thread per pixel, das_c summation order
+                                    (w outer, n inner): bit-identical to das_c
+                                    whenever fd == 0 */
+#define USTBX_FLAG_SEPARABLE_PHASE 2u /* geometry path: exp(i wd (tx+rx)) taken as
+                                         exp(i wd tx) * exp(i wd rx) */
+#define USTBX_FLAG_REF_PHASE 4u  /* geometry path: multiply by
+                                    exp(-1j*2*w0*r_ref/c) (das.m:306-309) */
+
+/* ---------------------------------------------------------------------------
+ * 1. Matrix ABI: the das_c / das_cuda MEX contract.
+ * Replaces: mex.das_c(ch, fs, t0, tx_apo, rx_apo, tx_delay, rx_delay, fd, dim)
+ *   (+mex/source/das_c.cpp:46-418, called from +midprocess/das.m:251-259) and
+ *   mex.das_cuda(..., dim, gpu_id) (+mex/source/das_cuda.cu:128-306,
+ *   das.m:262-271), which only supports dim == BOTH (das_cuda.cu:321-324);
+ *   this entry supports all four modes.
+ * ------------------------------------------------------------------------- */
+typedef struct ustbx_das_desc {
+  int32_t L, N, W, F; /* samples, channels, waves, frames */
+  int32_t P;          /* pixels */
+  float fs;           /* sampling frequency [Hz]   (das_c.cpp:212) */
+  float t0;           /* initial time [s]          (das_c.cpp:225) */
+  float fd;           /* modulation frequency [Hz] (das_c.cpp:238) */
+  int32_t dim;        /* USTBX_DIM_*               (das_c.cpp:78)  */
+  int32_t interp;     /* USTBX_INTERP_*            */
+  int32_t edge;       /* USTBX_EDGE_*              */
+  uint32_t flags;     /* USTBX_FLAG_*              */
+  int32_t split;      /* channel split per pixel tile (0 = auto)   */
+} ustbx_das_desc;
+
+/* All pointers are device (HBM) pointers; `out` is fully written (no need to
+ * clear it). Asynchronous on `stream`. */
+int ustbx_das_run(const ustbx_das_desc* d, const void* ch, const float* tx_apo,
+                  const float* rx_apo, const float* tx_delay,
+                  const float* rx_delay, void* out, void* stream,
+                  ustbx_status* st);
+
+/* Same contract on host buffers (what a MEX gateway holds): stages to the
+ * device, runs and copies the result back; synchronous. `device` selects the
+ * GPU like das_cuda's gpu_id (das_cuda.cu:161-162). */
+int ustbx_das_run_host(const ustbx_das_desc* d, const void* ch,
+                       const float* tx_apo, const float* rx_apo,
+                       const float* tx_delay, const float* rx_delay, void* out,
+                       int32_t device, ustbx_status* st);
+
+/* Validation of the das_c MEX argument contract on plain dimensions (what
+ * das_c.cpp:53-252 checks on the mxArrays): ndim/dims of each argument, the
+ * scalar sizes and the class flags. Returns USTBX_OK or the code das_c would
+ * raise; fills P/L/N/W/F of `d`. */
+int ustbx_das_check_mex_args(int32_t nrhs, int32_t nlhs, int32_t ch_ndim,
+                             const int64_t* ch_dims, int32_t ch_is_double,
+                             const int32_t* mat_ndim, const int64_t* mat_dims,
+                             const int32_t* mat_is_double,
+                             const int64_t* scalar_numel,
+                             const int32_t* scalar_is_double, int32_t sum_mode,
+                             int32_t ch_is_complex, float fd,
+                             ustbx_das_desc* d, ustbx_status* st);
+
+/* Reference error identifier for a code ("Toolbox:SRP_SRC:Dimensions", ...). */
+const char* ustbx_status_id(int32_t code);
+
+/* ---------------------------------------------------------------------------
+ * 2. Geometry plans: delays and apodization computed on the device.
+ * Replaces the host-side half of +midprocess/das.m:86-206 (tx/rx apodization,
+ * receive delay, transmit delay per wavefront) that feeds the MEX, and fuses
+ * the reference phase of das.m:306-309. A plan is device-resident state for
+ * one probe/scan/sequence configuration, reused across frames.
+ * ------------------------------------------------------------------------- */
+
+/* scan kinds */
+#define USTBX_SCAN_LINEAR 0   /* ndgrid(z, x, y), z fastest (linear_scan.m:71)    */
+#define USTBX_SCAN_SECTOR 1   /* ndgrid(depth, az, el) + sph2cart (sector_scan.m:90-112) */
+#define USTBX_SCAN_XYZ 2      /* explicit pixel list */
+
+/* wavefronts (uff.wavefront) */
+#define USTBX_WAVE_PLANE 0
+#define USTBX_WAVE_SPHERICAL 1
+#define USTBX_WAVE_PHOTOACOUSTIC 2
+
+/* spherical_transmit_delay_model (spherical_transmit_delay_model.m:32-37) */
+#define USTBX_TXMODEL_SPHERICAL 1
+#define USTBX_TXMODEL_UNIFIED 2 /* not on the device: pass tx_delay_matrix */
+#define USTBX_TXMODEL_HYBRID 3
+#define USTBX_TXMODEL_BLENDED 4
+
+/* windows (uff.window, window.m:24-37) */
+#define USTBX_WIN_NONE 0
+#define USTBX_WIN_BOXCAR 1
+#define USTBX_WIN_HANNING 2
+#define USTBX_WIN_HAMMING 3
+#define USTBX_WIN_TUKEY25 4
+#define USTBX_WIN_TUKEY50 5
+#define USTBX_WIN_TUKEY75 6
+#define USTBX_WIN_STA 7      /* host: reduces to a per-element vector */
+#define USTBX_WIN_SCANLINE 8 /* host: pass a matrix */
+#define USTBX_WIN_TRIANGLE 9
+
+typedef struct ustbx_scan {
+  int32_t kind;
+  int32_t n0, n1, n2;   /* axis lengths, fastest first; P = n0*n1*n2 */
+  const double* a0;     /* LINEAR: z_axis[n0]; SECTOR: depth_axis[n0]; XYZ: x[P] */
+  const double* a1;     /* LINEAR: x_axis[n1]; SECTOR: {sin az, cos az}[n1]; XYZ: y[P] */
+  const double* a2;     /* LINEAR: y_axis[n2]; SECTOR: {sin el, cos el}[n2]; XYZ: z[P] */
+  double origin[3];     /* SECTOR: scan origin (sector_scan.m:101-104) */
+  int32_t ref_kind;     /* reference distance: 0 = z (linear_scan.m:128-130),
+                           1 = depth rho (sector_scan.m:167-169) */
+} ustbx_scan;
+
+typedef struct ustbx_wave {
+  int32_t wavefront;    /* USTBX_WAVE_* */
+  int32_t _pad;
+  double src[3];        /* source x, y, z [m] (point.m:121-144) */
+  double src_distance;  /* [m], +inf for plane */
+  double azimuth, elevation;             /* source angles [rad] (point.m:28-30) */
+  double cos_az, sin_az, cos_el, sin_el; /* cos/sin of those angles */
+  double sound_speed;   /* wave.sound_speed (wave.m:55) */
+  double delay;         /* wave.delay [s]  (wave.m:54) */
+  double origin[3];     /* wave.origin (wave.m:46) */
+} ustbx_wave;
+
+typedef struct ustbx_apod {
+  int32_t window;       /* USTBX_WIN_*; NONE -> ones */
+  int32_t _pad;
+  double f_number[2];
+  double minimum_aperture[2];
+  double maximum_aperture[2];
+  double tilt[2];
+  double grating_lobe_angle[2];
+  int32_t has_origin;   /* rx: apodization.origin set (incidence_aperture) */
+  int32_t _pad2;
+  double origin[3];
+  const float* vector;  /* optional per-element (rx) / per-wave (tx) vector
+                           (apodization_vector, window.sta); host pointer */
+  const float* matrix;  /* optional explicit matrix ([N][P] rx, [W][P] tx);
+                           DEVICE pointer, used as is */
+} ustbx_apod;
+
+typedef struct ustbx_geometry {
+  int32_t N;                  /* elements */
+  int32_t W;                  /* waves    */
+  int32_t L;                  /* samples  */
+  int32_t F;                  /* frames   */
+  const double* elem_x;       /* [N] element positions (probe.m:42) */
+  const double* elem_y;
+  const double* elem_z;
+  ustbx_scan scan;
+  const ustbx_wave* waves;    /* [W] host array */
+  ustbx_apod rx_apod;
+  ustbx_apod tx_apod;
+  double sound_speed;         /* channel_data.sound_speed */
+  double lens_delay;          /* das.lens_delay (das.m:57) */
+  double pw_margin;           /* das.pw_margin (das.m:55) */
+  double blending_power;      /* das.blending_power (das.m:58) */
+  int32_t tx_model;           /* USTBX_TXMODEL_* for sources with z >= 0 */
+  int32_t _pad;
+  const float* tx_delay_matrix; /* optional [W][P] DEVICE pointer (unified model) */
+  float fs, t0, fd;
+  int32_t dim, interp, edge;
+  uint32_t flags;
+} ustbx_geometry;
+
+typedef struct ustbx_plan ustbx_plan;
+
+/* Uploads everything the plan needs into device memory on the current device. */
+int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan,
+                      ustbx_status* st);
+/* ch: device [F][W][N][L] complex64; out: device [F][W'][N'][P] complex64. */
+int ustbx_plan_run(const ustbx_plan* plan, const void* ch, void* out,
+                   void* stream, ustbx_status* st);
+/* Tile shape override (0 = keep): waves per register chunk (1, 8, 16) and
+ * channel groups per pixel tile (1 or 4; dimension both/transmit). */
+int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
+                         ustbx_status* st);
+/* Writes the plan's FP32 delay/apodization matrices (what das.m hands to the
+ * MEX) into device buffers: tx [W][P], rx [N][P]; any may be NULL. */
+int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
+                           float* tx_delay, float* rx_delay, void* stream,
+                           ustbx_status* st);
+int64_t ustbx_plan_pixels(const ustbx_plan* plan);
+int64_t ustbx_plan_device_bytes(const ustbx_plan* plan);
+void ustbx_plan_destroy(ustbx_plan* plan);
+
+/* ---------------------------------------------------------------------------
+ * 3. Library information.
+ * ------------------------------------------------------------------------- */
+int32_t ustbx_abi_version(void);
+/* Number of HIP devices visible (0 when none); never fails. */
+int32_t ustbx_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* USTBX_DAS_H */

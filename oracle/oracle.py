@@ -1,0 +1,486 @@
+"""CPU oracle for the delay-and-sum path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py's CPU-baseline leg. The product never imports it.
+
+Contents
+- ``das_c``: ctypes front of ``das_c_oracle.c``, the C restatement of the
+  reference das_c MEX (+mex/source/das_c.cpp:46-418), FP32.
+- ``beamform_fp64``: FP64 restatement of tools.matlab_beamformer
+  (+tools/matlab_beamformer.m:1-51) with selectable sample-window rule and a
+  cubic option, evaluated on all pixels or a pixel subset.
+- ``python_beamform``: restatement of the Python reference's
+  DAS._beamform (python/src/ustb/midprocess/das.py:203-261) in NumPy FP32.
+- geometry in FP64 rounded to FP32 where das.m applies single():
+  ``rx_delay`` (das.m:103-106), ``tx_delay`` (das.m:110-206),
+  ``window`` (apodization.m:172-186), ``rx_apod`` (apodization.m:323-468),
+  ``tx_apod`` (apodization.m:249-320, 472-555), pixel grids
+  (linear_scan.m:64-82, sector_scan.m:73-121), element positions
+  (linear_array.m:56-61, matrix_array.m:60-70), ``reference_phase``
+  (das.m:306-309).
+
+Array layout: MATLAB column-major [L,N,W,F] is held as NumPy C-order
+(F, W, N, L); [P,N] as (N, P); [P,W] as (W, P); outputs (F, W', N', P).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+DIM_NONE, DIM_RX, DIM_TX, DIM_BOTH = 0, 1, 2, 3
+WIN_NONE, WIN_BOXCAR, WIN_HANNING, WIN_HAMMING = 0, 1, 2, 3
+WIN_TUKEY25, WIN_TUKEY50, WIN_TUKEY75, WIN_STA, WIN_SCANLINE, WIN_TRIANGLE = 4, 5, 6, 7, 8, 9
+WAVE_PLANE, WAVE_SPHERICAL, WAVE_PHOTOACOUSTIC = 0, 1, 2
+MODEL_SPHERICAL, MODEL_UNIFIED, MODEL_HYBRID, MODEL_BLENDED = 1, 2, 3, 4
+EPS = np.finfo(np.float64).eps
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        from . import build as _b  # noqa: WPS433 (package-relative recipe)
+
+        path = _b.build(force=False)
+        lib = ctypes.CDLL(path)
+        fp = ctypes.POINTER(ctypes.c_float)
+        lib.oracle_das_c.argtypes = [fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_int, fp, fp, fp, fp, fp, ctypes.c_int]
+        lib.oracle_das_c.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def out_shape(dim, P, N, W, F):
+    return {DIM_NONE: (F, W, N, P), DIM_RX: (F, W, 1, P), DIM_TX: (F, 1, N, P),
+            DIM_BOTH: (F, 1, 1, P)}[dim]
+
+
+# ---------------------------------------------------------------------------
+# das_c restatement (C)
+# ---------------------------------------------------------------------------
+def das_c(ch, fs, t0, tx_apo, rx_apo, tx_delay, rx_delay, fd, dim, nthreads=None):
+    """Run the C restatement of das_c. ch: complex64 (F, W, N, L)."""
+    lib = _load()
+    ch = np.ascontiguousarray(ch, dtype=np.complex64)
+    F, W, N, L = ch.shape
+    tx_apo = np.ascontiguousarray(tx_apo, dtype=np.float32)
+    rx_apo = np.ascontiguousarray(rx_apo, dtype=np.float32)
+    tx_delay = np.ascontiguousarray(tx_delay, dtype=np.float32)
+    rx_delay = np.ascontiguousarray(rx_delay, dtype=np.float32)
+    P = tx_delay.shape[-1]
+    assert tx_apo.shape == (W, P) and tx_delay.shape == (W, P)
+    assert rx_apo.shape == (N, P) and rx_delay.shape == (N, P)
+    out = np.zeros(out_shape(dim, P, N, W, F), dtype=np.complex64)
+    if nthreads is None:
+        nthreads = os.cpu_count() or 1
+    rc = lib.oracle_das_c(_fptr(ch.view(np.float32)), L, N, W, F, P, np.float32(fs), np.float32(t0),
+                          np.float32(fd), int(dim), _fptr(tx_apo), _fptr(rx_apo), _fptr(tx_delay),
+                          _fptr(rx_delay), _fptr(out.view(np.float32)), int(nthreads))
+    if rc != 0:
+        raise ValueError("Unknown sum mode. Available: 0 -> NONE, 1->RX, 2->TX, 3->BOTH")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# FP64 matlab_beamformer restatement
+# ---------------------------------------------------------------------------
+def _keys(b):
+    b2 = b * b
+    b3 = b2 * b
+    return (-0.5 * b3 + b2 - 0.5 * b, 1.5 * b3 - 2.5 * b2 + 1.0,
+            -1.5 * b3 + 2.0 * b2 + 0.5 * b, 0.5 * b3 - 0.5 * b2)
+
+
+def _sample(trace, d, interp, edge):
+    """Interpolate trace (F, L) at fractional indices d (P,) -> (F, P)."""
+    L = trace.shape[-1]
+    i0 = np.floor(d)
+    if interp == "linear":
+        if edge == "dasc":
+            valid = (d >= 1.0) & (d < L - 1)
+        elif edge == "python":
+            valid = (d >= 0.0) & (d < L - 1)
+        else:  # interp1 closed interval [t_0, t_{L-1}]
+            valid = (d >= 0.0) & (d <= L - 1)
+        i0c = np.clip(np.where(valid, i0, 0), 0, L - 2).astype(np.int64)
+        b = np.where(valid, d - i0c, 0.0)
+        v = trace[:, i0c] * (1.0 - b) + trace[:, i0c + 1] * b
+    else:
+        valid = (d >= 1.0) & (d < L - 2)
+        i0c = np.where(valid, i0, 1).astype(np.int64)
+        b = np.where(valid, d - i0c, 0.0)
+        c0, c1, c2, c3 = _keys(b)
+        v = (trace[:, i0c - 1] * c0 + trace[:, i0c] * c1 + trace[:, i0c + 1] * c2
+             + trace[:, i0c + 2] * c3)
+    return v, valid
+
+
+def dasc_wd(fd):
+    """das_c's angular frequency: 2*(float)PI*fd in FP32 (das_c.cpp:33,240-244)."""
+    fd32 = np.float32(fd)
+    if abs(float(fd32)) <= 1e-6:
+        return 0.0
+    return float(np.float32(np.float32(2) * np.float32(3.14159265359)) * fd32)
+
+
+def beamform_fp64(ch, fs, t0, tx_apo, rx_apo, tx_delay, rx_delay, fd, dim,
+                  interp="linear", edge="dasc", pixels=None, w0=None, index="fp64"):
+    """FP64 generalized DAS (matlab_beamformer.m:19-51) on FP32 inputs.
+
+    w0 = 2*pi*fd unless given (e.g. dasc_wd(fd)); the phase exp(1i*w0*delay)
+    is applied when |w0| > eps (matlab_beamformer.m:33-35). ``pixels`` selects
+    a subset (indices). ``index="fp32"`` forms the fractional sample index
+    with das_c's FP32 arithmetic, d = (float(tx + rx) - t0) * Fs
+    (das_c.cpp:372-378), everything else staying FP64. Returns complex128 of
+    shape out_shape(dim, len(pixels), N, W, F).
+    """
+    ch = np.asarray(ch)
+    F, W, N, L = ch.shape
+    tx_apo = np.asarray(tx_apo, dtype=np.float64)
+    rx_apo = np.asarray(rx_apo, dtype=np.float64)
+    tx_delay = np.asarray(tx_delay, dtype=np.float64)
+    rx_delay = np.asarray(rx_delay, dtype=np.float64)
+    if pixels is not None:
+        pixels = np.asarray(pixels)
+        tx_apo, rx_apo = tx_apo[:, pixels], rx_apo[:, pixels]
+        tx_delay, rx_delay = tx_delay[:, pixels], rx_delay[:, pixels]
+    P = tx_delay.shape[1]
+    if w0 is None:
+        w0 = 2.0 * math.pi * float(fd) if abs(float(fd)) > 1e-6 else 0.0
+    out = np.zeros(out_shape(dim, P, N, W, F), dtype=np.complex128)
+    ch64 = ch.astype(np.complex128)
+    for w in range(W):
+        if not np.any(tx_apo[w]):
+            continue
+        for n in range(N):
+            if not np.any(rx_apo[n]):
+                continue
+            apo = rx_apo[n] * tx_apo[w]
+            delay = rx_delay[n] + tx_delay[w]
+            if index == "fp32":
+                c32 = rx_delay[n].astype(np.float32) + tx_delay[w].astype(np.float32)
+                d = ((c32 - np.float32(t0)) * np.float32(fs)).astype(np.float64)
+            else:
+                d = (delay - float(t0)) * float(fs)
+            v, valid = _sample(ch64[:, w, n, :], d, interp, edge)
+            # out-of-window terms and terms with a zero tx or rx weight are
+            # skipped, not multiplied (das_c.cpp:366,371,381); only there does
+            # das_c differ from matlab_beamformer (which gives 0*NaN = NaN)
+            valid = valid & (tx_apo[w] != 0) & (rx_apo[n] != 0)
+            temp = np.where(valid[None, :], v * apo[None, :], 0.0)
+            if abs(w0) > EPS:
+                temp = np.where(valid[None, :], temp * np.exp(1j * w0 * delay)[None, :], 0.0)
+            if dim == DIM_NONE:
+                out[:, w, n, :] = temp
+            elif dim == DIM_RX:
+                out[:, w, 0, :] += temp
+            elif dim == DIM_TX:
+                out[:, 0, n, :] += temp
+            else:
+                out[:, 0, 0, :] += temp
+    return out
+
+
+def python_beamform(ch_data, ch_data_time, tx_apo, rx_apo, tx_delay, rx_delay, w0, dim):
+    """Restatement of DAS._beamform (python das.py:203-261), NumPy FP32.
+
+    Reference shapes: ch_data (L, N, W, F) complex64; apodization/delays
+    (P, N) / (P, W). Returns (P, N', W', F) complex64.
+    """
+    L, N, W, F = ch_data.shape
+    P = tx_apo.shape[0]
+    shape = {DIM_NONE: (P, N, W, F), DIM_RX: (P, 1, W, F), DIM_TX: (P, N, 1, F),
+             DIM_BOTH: (P, 1, 1, F)}[int(dim)]
+    bf = np.zeros(shape, dtype=np.complex64)
+    t0 = float(ch_data_time[0])
+    dt = float(ch_data_time[1] - ch_data_time[0]) if len(ch_data_time) > 1 else 1.0
+    for w in range(W):
+        if not np.any(tx_apo[:, w]):
+            continue
+        for n in range(N):
+            if not np.any(rx_apo[:, n]):
+                continue
+            apo = rx_apo[:, n] * tx_apo[:, w]
+            delay = rx_delay[:, n] + tx_delay[:, w]
+            sample_idx = (delay - t0) / dt
+            idx = np.floor(sample_idx).astype(np.int32)
+            frac = sample_idx - idx
+            valid = (idx >= 0) & (idx < L - 1)
+            for f in range(F):
+                trace = ch_data[:, n, w, f]
+                iv = np.zeros(P, dtype=trace.dtype)
+                iv[valid] = trace[idx[valid]] * (1.0 - frac[valid]) + trace[idx[valid] + 1] * frac[valid]
+                temp = apo * iv
+                if abs(w0) > np.finfo(np.float32).eps:
+                    temp = np.exp(1j * w0 * delay) * temp
+                if dim == DIM_NONE:
+                    bf[:, n, w, f] = temp
+                elif dim == DIM_RX:
+                    bf[:, 0, w, f] += temp
+                elif dim == DIM_TX:
+                    bf[:, n, 0, f] += temp
+                else:
+                    bf[:, 0, 0, f] += temp
+    return bf
+
+
+# ---------------------------------------------------------------------------
+# Geometry restatement (FP64 -> FP32 where das.m applies single())
+# ---------------------------------------------------------------------------
+def linear_scan_xyz(x_axis, z_axis, y_axis=(0.0,)):
+    """[Z, X, Y] = ndgrid(z_axis, x_axis, y_axis) (linear_scan.m:71-81)."""
+    z_axis = np.asarray(z_axis, dtype=np.float64)
+    x_axis = np.asarray(x_axis, dtype=np.float64)
+    y_axis = np.asarray(y_axis, dtype=np.float64)
+    Z, X, Y = np.meshgrid(z_axis, x_axis, y_axis, indexing="ij")
+    return (X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F"))
+
+
+def sector_scan_xyz(azimuth_axis, depth_axis, elevation_axis=(0.0,), origin=(0.0, 0.0, 0.0)):
+    """ndgrid(depth, az, el) + sph2cart(phi, theta, rho) + origin (sector_scan.m:90-112).
+
+    sph2cart(az=phi, elev=theta, r=rho): z = r*sin(theta) -> X;
+    rcoselev = r*cos(theta); x = rcoselev*cos(phi) -> Z; y = rcoselev*sin(phi) -> Y.
+    Returns x, y, z, rho.
+    """
+    rho, th, ph = np.meshgrid(np.asarray(depth_axis, float), np.asarray(azimuth_axis, float),
+                              np.asarray(elevation_axis, float), indexing="ij")
+    rho = rho.ravel(order="F")
+    th = th.ravel(order="F")
+    ph = ph.ravel(order="F")
+    X = rho * np.sin(th)
+    rc = rho * np.cos(th)
+    Z = rc * np.cos(ph)
+    Y = rc * np.sin(ph)
+    return X + origin[0], Y + origin[1], Z + origin[2], rho
+
+
+def sector_tables(azimuth_axis, elevation_axis=(0.0,)):
+    """{sin, cos} per angle, the device tables of a sector scan."""
+    az = np.asarray(azimuth_axis, float)
+    el = np.asarray(elevation_axis, float)
+    return (np.stack([np.sin(az), np.cos(az)], 1).ravel(), np.stack([np.sin(el), np.cos(el)], 1).ravel())
+
+
+def sector_scan_xyz_tables(azimuth_axis, depth_axis, elevation_axis=(0.0,), origin=(0.0, 0.0, 0.0)):
+    """Same grid from precomputed sin/cos tables (the device's arithmetic)."""
+    rho, th, ph = np.meshgrid(np.asarray(depth_axis, float), np.arange(len(azimuth_axis)),
+                              np.arange(len(elevation_axis)), indexing="ij")
+    rho = rho.ravel(order="F")
+    ia = th.ravel(order="F")
+    ie = ph.ravel(order="F")
+    az = np.asarray(azimuth_axis, float)
+    el = np.asarray(elevation_axis, float)
+    st, ct = np.sin(az)[ia], np.cos(az)[ia]
+    sp, cp = np.sin(el)[ie], np.cos(el)[ie]
+    rc = rho * ct
+    return rho * st + origin[0], rc * sp + origin[1], rc * cp + origin[2], rho
+
+
+def linear_array_x(N, pitch):
+    """x0 = (1:N)*pitch; x0 = x0 - mean(x0) (linear_array.m:56-58)."""
+    x0 = np.arange(1, N + 1, dtype=np.float64) * pitch
+    return x0 - np.sum(x0) / N
+
+
+def matrix_array_xy(Nx, Ny, pitch_x, pitch_y):
+    """[X, Y] = ndgrid(x0, y0), x fastest (matrix_array.m:60-70)."""
+    x0 = np.arange(1, Nx + 1, dtype=np.float64) * pitch_x
+    y0 = np.arange(1, Ny + 1, dtype=np.float64) * pitch_y
+    x0 = x0 - np.sum(x0) / Nx
+    y0 = y0 - np.sum(y0) / Ny
+    X, Y = np.meshgrid(x0, y0, indexing="ij")
+    return X.ravel(order="F"), Y.ravel(order="F")
+
+
+def rx_delay(px, py, pz, ex, ey, ez, c, lens_delay=0.0):
+    """single(sqrt(xm.^2+ym.^2+zm.^2)/c + lens_delay), xm = probe.x.' - scan.x
+    (das.m:103-106). Returns (N, P) float32."""
+    xm = ex[None, :] - px[:, None]
+    ym = ey[None, :] - py[:, None]
+    zm = ez[None, :] - pz[:, None]
+    d = np.sqrt((xm * xm + ym * ym) + zm * zm) / c + lens_delay
+    return np.ascontiguousarray(d.astype(np.float32).T)
+
+
+def plane_delay(px, py, pz, az, el):
+    """z*cos(az)*cos(el) + x*sin(az)*cos(el) + y*sin(el) (das.m:193)."""
+    ca, sa, ce, se = math.cos(az), math.sin(az), math.cos(el), math.sin(el)
+    return ((pz * ca) * ce + (px * sa) * ce) + py * se
+
+
+def tx_delay(px, py, pz, waves, scan_kind="linear", model=MODEL_HYBRID, pw_margin=1e-3,
+             blending_power=0.5):
+    """Transmit delay (das.m:110-206). waves: dicts with wavefront, src (xyz),
+    distance, azimuth, elevation, sound_speed, delay. Returns (W, P) float32."""
+    out = np.zeros((len(waves), px.size), dtype=np.float32)
+    for w, wv in enumerate(waves):
+        sx, sy, sz = wv["src"]
+        dist = wv["distance"]
+        if wv["wavefront"] == WAVE_PLANE or (wv["wavefront"] == WAVE_SPHERICAL and math.isinf(dist)):
+            v = plane_delay(px, py, pz, wv["azimuth"], wv["elevation"])
+        elif wv["wavefront"] == WAVE_PHOTOACOUSTIC:
+            v = np.zeros_like(px)
+        else:
+            dx, dy, dz = sx - px, sy - py, sz - pz
+            sgn = np.where(pz < sz, -1.0, 1.0)
+            d = sgn * np.sqrt((dx * dx + dy * dy) + dz * dz)
+            if sz < 0:
+                v = d - abs(dist)
+            elif model == MODEL_SPHERICAL:
+                v = d + dist
+            elif model in (MODEL_HYBRID, MODEL_BLENDED):
+                if scan_kind == "linear":
+                    pd = sgn * np.sqrt(dz * dz) + dist
+                elif scan_kind == "sector":
+                    pd = plane_delay(px, py, pz, wv["azimuth"], wv["elevation"])
+                else:
+                    raise ValueError("Only linear scan and sector scan in 2D is supported")
+                if model == MODEL_HYBRID:
+                    mask = (pz < (sz + pw_margin)) & (pz > (sz - pw_margin))
+                    v = np.where(mask, pd, d + dist)
+                else:
+                    rr = np.sqrt((px * px + py * py) + pz * pz)
+                    nd = np.minimum(np.abs(dist - rr) / dist, 1.0)
+                    npw = nd ** blending_power
+                    v = (d + dist) * npw + pd * (1.0 - npw)
+            else:
+                raise ValueError("unified model is not restated")
+        out[w] = (v / wv["sound_speed"] - wv["delay"]).astype(np.float32)
+    return out
+
+
+def window(win, r):
+    """apodization.m:172-186, FP64 (a logical mask times NaN stays NaN)."""
+    r = np.asarray(r, dtype=np.float64)
+    with np.errstate(invalid="ignore"):
+        m = (r <= 0.5).astype(np.float64)
+        if win == WIN_BOXCAR:
+            return m
+        if win == WIN_HANNING:
+            return m * (0.5 + 0.5 * np.cos(2 * np.pi * r))
+        if win == WIN_HAMMING:
+            return m * (0.53836 + 0.46164 * np.cos(2 * np.pi * r))
+        if win in (WIN_TUKEY25, WIN_TUKEY50, WIN_TUKEY75):
+            roll = {WIN_TUKEY25: 0.25, WIN_TUKEY50: 0.50, WIN_TUKEY75: 0.75}[win]
+            th = 0.5 * (1.0 - roll)
+            return ((r <= th).astype(float)
+                    + (((r > th).astype(float) * (r < 0.5).astype(float)) * 0.5)
+                    * (1.0 + np.cos(((2 * np.pi) / roll) * ((r - roll / 2) - 0.5))))
+        if win == WIN_TRIANGLE:
+            return m * (1.0 - 2.0 * r)
+    raise ValueError("Unknown apodization type!")
+
+
+def _clamp(z, min_af, max_af):
+    zd = z.copy()
+    m = np.abs(z) <= min_af
+    zd[m] = np.sign(zd[m]) * min_af
+    m = np.abs(z) >= max_af
+    zd[m] = np.sign(zd[m]) * max_af
+    return zd
+
+
+def rx_apod(px, py, pz, ex, ey, ez, window_kind, f_number=(1.0, 1.0), minimum_aperture=(1e-3, 1e-3),
+            maximum_aperture=(10.0, 10.0), tilt=(0.0, 0.0), origin=None):
+    """Receive apodization (apodization.m:323-468) for flat probes.
+    origin=None: aperture centred on each pixel (linear scan); else centred on
+    the origin (sector scan or a user origin). Returns (N, P) float32."""
+    N, P = ex.size, px.size
+    if window_kind == WIN_NONE:
+        return np.ones((N, P), dtype=np.float32)
+    if origin is None:
+        xd = px[:, None] - ex[None, :]
+        yd = py[:, None] - ey[None, :]
+        zd = pz[:, None] - ez[None, :]
+    else:
+        x0, y0, z0 = origin
+        dx, dy, dz = px - x0, py - y0, pz - z0
+        pdist = np.sqrt((dx * dx + dy * dy) + dz * dz)
+        xd = np.broadcast_to(ex[None, :] - x0, (P, N)).copy()
+        yd = np.broadcast_to(ey[None, :] - y0, (P, N)).copy()
+        zd = np.broadcast_to(pdist[:, None], (P, N)).copy()
+    if abs(tilt[0]) > 0:
+        ct, st = math.cos(tilt[0]), math.sin(tilt[0])
+        xd, zd = xd * ct - zd * st, xd * st + zd * ct
+    if abs(tilt[1]) > 0:
+        ct, st = math.cos(tilt[1]), math.sin(tilt[1])
+        yd, zd = yd * ct - zd * st, yd * st + zd * ct
+    zx = _clamp(zd, minimum_aperture[0] * f_number[0], maximum_aperture[0] * f_number[0])
+    zy = _clamp(zd, minimum_aperture[1] * f_number[1], maximum_aperture[1] * f_number[1])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rt = np.abs(f_number[0] * (xd / zx))
+        rp = np.abs(f_number[1] * (yd / zy))
+        a = window(window_kind, rt) * window(window_kind, rp)
+    return np.ascontiguousarray(a.astype(np.float32).T)
+
+
+def tx_apod(px, py, pz, waves, window_kind, f_number=(1.0, 1.0), minimum_aperture=(1e-3, 1e-3),
+            maximum_aperture=(10.0, 10.0), tilt=(0.0, 0.0), grating_lobe_angle=(math.pi, math.pi)):
+    """Transmit apodization (apodization.m:249-320, 472-555). waves: dicts with
+    wavefront, src, origin, distance, azimuth, elevation. Returns (W, P) float32."""
+    W, P = len(waves), px.size
+    if window_kind == WIN_NONE:
+        return np.ones((W, P), dtype=np.float32)
+    out = np.zeros((W, P), dtype=np.float32)
+    for w, wv in enumerate(waves):
+        if wv["wavefront"] == WAVE_PLANE or math.isinf(wv["distance"]):
+            rt = abs(f_number[0] * math.tan(wv["azimuth"] - tilt[0]))
+            rp = abs(f_number[1] * math.tan(wv["elevation"] - tilt[1]))
+            out[w] = np.float32(window(window_kind, rt) * window(window_kind, rp))
+            continue
+        s = np.asarray(wv["src"], float)
+        o = np.asarray(wv["origin"], float)
+        SP = (px - s[0], py - s[1], pz - s[2])
+        OP = (px - o[0], py - o[1], pz - o[2])
+        OS = s - o
+        if math.sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2]) <= EPS:
+            OS = np.array([0.0, 0.0, 1.0])
+        ns = math.sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2])
+        zu = OS / ns
+        e1 = np.array([1.0, 0.0, 0.0])
+        yu = np.array([zu[1] * e1[2] - zu[2] * e1[1], zu[2] * e1[0] - zu[0] * e1[2],
+                       zu[0] * e1[1] - zu[1] * e1[0]])
+        xu = np.array([zu[1] * yu[2] - zu[2] * yu[1], zu[2] * yu[0] - zu[0] * yu[2],
+                       zu[0] * yu[1] - zu[1] * yu[0]])
+        zd = (SP[0] * zu[0] + SP[1] * zu[1]) + SP[2] * zu[2]
+        xd = (SP[0] * xu[0] + SP[1] * xu[1]) + SP[2] * xu[2]
+        yd = (SP[0] * yu[0] + SP[1] * yu[1]) + SP[2] * yu[2]
+        zx = _clamp(zd, minimum_aperture[0] * f_number[0], maximum_aperture[0] * f_number[0])
+        zy = _clamp(zd, minimum_aperture[1] * f_number[1], maximum_aperture[1] * f_number[1])
+        zx[np.abs(np.arctan2(OP[0], OP[2]) - math.atan2(OS[0], OS[2])) > 0.5 * grating_lobe_angle[0]] = EPS
+        zy[np.abs(np.arctan2(OP[1], OP[2]) - math.atan2(OS[1], OS[2])) > 0.5 * grating_lobe_angle[1]] = EPS
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rt = np.abs(f_number[0] * (xd / zx))
+            rp = np.abs(f_number[1] * (yd / zy))
+            out[w] = (window(window_kind, rt) * window(window_kind, rp)).astype(np.float32)
+    return out
+
+
+def reference_phase(r_ref, fd, c):
+    """exp(-1j*2*w0*r_ref/c) (das.m:306-309; das.py:164-173), complex128.
+
+    w0 = 2*pi*single(fd) is a single-precision value in both references
+    (das.m:88-89; das.py:54-55 under NumPy's scalar promotion), equal to
+    das_c's wd."""
+    w0 = dasc_wd(fd)
+    return np.exp(-1j * 2.0 * w0 * np.asarray(r_ref, float) / c)
+
+
+def hilbert(x, axis=-1):
+    """Analytic signal (das.m:211-214 / das.py:144-145) via scipy.signal.hilbert."""
+    from scipy.signal import hilbert as _h
+
+    return _h(x, axis=axis)

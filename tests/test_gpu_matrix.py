@@ -1,0 +1,114 @@
+"""GPU parity of the matrix (das_c MEX) contract against the das_c oracle.
+
+Bar: bit-identical for RF (fd = 0: no sin/cos) when the das_c summation order
+is requested; otherwise |gpu - oracle| <= 1e-5 * max|oracle| (FP32 tolerance
+stated by the north-star contract). Cubic interpolation has no reference and
+is checked against the FP64 oracle with the same bar.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.cases import matrix_problem
+from ustb_amd import ops
+from ustb_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _dev(pb, cuda):
+    return {k: (torch.from_numpy(v).to(cuda) if isinstance(v, np.ndarray) else v) for k, v in pb.items()}
+
+
+def _run(pb, cuda, fd, dim, **kw):
+    d = _dev(pb, cuda)
+    out = ops.das(d["ch"], d["fs"], d["t0"], d["tx_apo"], d["rx_apo"], d["tx_delay"], d["rx_delay"], fd, dim, **kw)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a.astype(np.complex128) - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+@pytest.mark.parametrize("dim", [0, 1, 2, 3])
+def test_rf_bit_exact_dasc_order(cuda, dim):
+    pb = matrix_problem(seed=10 + dim)
+    ref = O.das_c(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"], pb["rx_delay"],
+                  0.0, dim)
+    print(f"case rf dasc_order dim={dim}", flush=True)
+    got = _run(pb, cuda, 0.0, dim, dasc_order=True)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref), f"max diff {np.max(np.abs(got - ref))}"
+
+
+@pytest.mark.parametrize("dim", [0, 1, 2, 3])
+@pytest.mark.parametrize("split", [1, 2, 4, 8])
+@pytest.mark.parametrize("fd", [0.0, 0.05])
+def test_split_tolerance(cuda, dim, split, fd):
+    pb = matrix_problem(seed=20 + dim)
+    ref = O.das_c(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"], pb["rx_delay"], fd,
+                  dim)
+    print(f"case split={split} fd={fd} dim={dim}", flush=True)
+    got = _run(pb, cuda, fd, dim, split=split)
+    assert got.shape == ref.shape
+    assert _rel(got, ref) <= TOL
+    if dim in (0, 2) and fd == 0.0:
+        # no cross-channel reduction: the das_c order is kept for any split
+        assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("dim", [0, 3])
+def test_iq_dasc_order_tolerance(cuda, dim):
+    pb = matrix_problem(seed=30 + dim, F=1)
+    ref = O.das_c(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"], pb["rx_delay"],
+                  0.05, dim)
+    got = _run(pb, cuda, 0.05, dim, dasc_order=True)
+    assert _rel(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("dim", [0, 3])
+def test_python_edge_rule(cuda, dim):
+    pb = matrix_problem(seed=40 + dim, F=1)
+    ref = O.beamform_fp64(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"],
+                          pb["rx_delay"], 0.0, dim, edge="python")
+    got = _run(pb, cuda, 0.0, dim, edge=L.EDGE_PYTHON)
+    assert _rel(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("dim", [0, 1, 2, 3])
+def test_cubic_vs_fp64(cuda, dim):
+    pb = matrix_problem(seed=50 + dim, F=2)
+    ref = O.beamform_fp64(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"],
+                          pb["rx_delay"], 0.05, dim, interp="cubic")
+    got = _run(pb, cuda, 0.05, dim, interp=L.INTERP_CUBIC)
+    assert _rel(got, ref) <= TOL
+
+
+def test_empty_and_degenerate(cuda):
+    pb = matrix_problem(seed=3, P=5, N=2, W=1, F=1, L=2)  # L = 2: no valid sample (0 < n0 < 1)
+    got = _run(pb, cuda, 0.0, 3)
+    assert np.all(got == 0)
+    pb = matrix_problem(seed=3, P=0, N=2, W=1, F=1, L=16)
+    got = _run(pb, cuda, 0.0, 3)
+    assert got.shape == (1, 1, 1, 0)
+
+
+def test_host_entry_matches_device(cuda):
+    import ctypes
+
+    pb = matrix_problem(seed=7, F=2)
+    ref = O.das_c(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"], pb["rx_delay"], 0.0, 1)
+    F, W, N, Ls = pb["ch"].shape
+    P = pb["tx_delay"].shape[1]
+    out = np.zeros(ref.shape, np.complex64)
+    d = L.DasDesc(L=Ls, N=N, W=W, F=F, P=P, fs=1.0, t0=0.0, fd=0.0, dim=1, interp=0, edge=0,
+                  flags=L.FLAG_DASC_ORDER, split=0)
+    st = L.Status()
+    rc = L.lib().ustbx_das_run_host(ctypes.byref(d), pb["ch"].ctypes.data, pb["tx_apo"].ctypes.data,
+                                    pb["rx_apo"].ctypes.data, pb["tx_delay"].ctypes.data,
+                                    pb["rx_delay"].ctypes.data, out.ctypes.data, 0, ctypes.byref(st))
+    L.check(rc, st)
+    assert np.array_equal(out, ref)

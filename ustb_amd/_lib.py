@@ -1,0 +1,144 @@
+"""ctypes binding of libustbx.so (the C-ABI in include/ustbx_das.h).
+
+The library is loaded from the package directory. There is no fallback: if
+the library is missing or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libustbx.so")
+
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_uint32 = ctypes.c_uint32
+c_float = ctypes.c_float
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+P_double = ctypes.POINTER(ctypes.c_double)
+P_float = ctypes.POINTER(ctypes.c_float)
+
+# constants (mirror include/ustbx_das.h)
+OK = 0
+E_NRHS, E_NLHS, E_DIMENSIONS, E_FLOAT, E_SCALAR, E_GPU, E_ARG, E_CAPACITY, E_UNSUPPORTED = (
+    -1, -2, -3, -4, -5, -6, -7, -8, -9)
+DIM_NONE, DIM_RX, DIM_TX, DIM_BOTH = 0, 1, 2, 3
+INTERP_LINEAR, INTERP_CUBIC = 0, 1
+EDGE_DASC, EDGE_PYTHON = 0, 1
+FLAG_DASC_ORDER, FLAG_SEPARABLE_PHASE, FLAG_REF_PHASE = 1, 2, 4
+SCAN_LINEAR, SCAN_SECTOR, SCAN_XYZ = 0, 1, 2
+WAVE_PLANE, WAVE_SPHERICAL, WAVE_PHOTOACOUSTIC = 0, 1, 2
+TXMODEL_SPHERICAL, TXMODEL_UNIFIED, TXMODEL_HYBRID, TXMODEL_BLENDED = 1, 2, 3, 4
+
+
+class Status(ctypes.Structure):
+    _fields_ = [("code", c_int32), ("message", ctypes.c_char * 256)]
+
+
+class DasDesc(ctypes.Structure):
+    _fields_ = [("L", c_int32), ("N", c_int32), ("W", c_int32), ("F", c_int32), ("P", c_int32),
+                ("fs", c_float), ("t0", c_float), ("fd", c_float), ("dim", c_int32),
+                ("interp", c_int32), ("edge", c_int32), ("flags", c_uint32), ("split", c_int32)]
+
+
+class Scan(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("n0", c_int32), ("n1", c_int32), ("n2", c_int32),
+                ("a0", P_double), ("a1", P_double), ("a2", P_double),
+                ("origin", c_double * 3), ("ref_kind", c_int32)]
+
+
+class Wave(ctypes.Structure):
+    _fields_ = [("wavefront", c_int32), ("_pad", c_int32), ("src", c_double * 3),
+                ("src_distance", c_double), ("azimuth", c_double), ("elevation", c_double),
+                ("cos_az", c_double), ("sin_az", c_double), ("cos_el", c_double),
+                ("sin_el", c_double), ("sound_speed", c_double), ("delay", c_double),
+                ("origin", c_double * 3)]
+
+
+class Apod(ctypes.Structure):
+    _fields_ = [("window", c_int32), ("_pad", c_int32), ("f_number", c_double * 2),
+                ("minimum_aperture", c_double * 2), ("maximum_aperture", c_double * 2),
+                ("tilt", c_double * 2), ("grating_lobe_angle", c_double * 2),
+                ("has_origin", c_int32), ("_pad2", c_int32), ("origin", c_double * 3),
+                ("vector", P_float), ("matrix", P_float)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("N", c_int32), ("W", c_int32), ("L", c_int32), ("F", c_int32),
+                ("elem_x", P_double), ("elem_y", P_double), ("elem_z", P_double),
+                ("scan", Scan), ("waves", ctypes.POINTER(Wave)), ("rx_apod", Apod),
+                ("tx_apod", Apod), ("sound_speed", c_double), ("lens_delay", c_double),
+                ("pw_margin", c_double), ("blending_power", c_double), ("tx_model", c_int32),
+                ("_pad", c_int32), ("tx_delay_matrix", P_float), ("fs", c_float), ("t0", c_float),
+                ("fd", c_float), ("dim", c_int32), ("interp", c_int32), ("edge", c_int32),
+                ("flags", c_uint32)]
+
+
+class SynthDesc(ctypes.Structure):
+    _fields_ = [("L", c_int32), ("N", c_int32), ("W", c_int32), ("F", c_int32), ("S", c_int32),
+                ("iq", c_int32), ("half", c_int32), ("threads", c_int32), ("fs", c_double),
+                ("t0", c_double), ("f0", c_double), ("bandwidth", c_double), ("c", c_double)]
+
+
+EXPORTS = {
+    "ustbx_abi_version": (c_int32, []),
+    "ustbx_device_count": (c_int32, []),
+    "ustbx_status_id": (ctypes.c_char_p, [c_int32]),
+    "ustbx_das_run": (ctypes.c_int, [ctypes.POINTER(DasDesc), c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_das_run_host": (ctypes.c_int, [ctypes.POINTER(DasDesc), c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, c_int32,
+                                          ctypes.POINTER(Status)]),
+    "ustbx_das_check_mex_args": (ctypes.c_int, [c_int32, c_int32, c_int32, ctypes.POINTER(c_int64),
+                                                c_int32, ctypes.POINTER(c_int32),
+                                                ctypes.POINTER(c_int64), ctypes.POINTER(c_int32),
+                                                ctypes.POINTER(c_int64), ctypes.POINTER(c_int32),
+                                                c_int32, c_int32, c_float, ctypes.POINTER(DasDesc),
+                                                ctypes.POINTER(Status)]),
+    "ustbx_plan_create": (ctypes.c_int, [ctypes.POINTER(Geometry), ctypes.POINTER(c_void_p),
+                                         ctypes.POINTER(Status)]),
+    "ustbx_plan_run": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_plan_set_tiles": (ctypes.c_int, [c_void_p, c_int32, c_int32, ctypes.POINTER(Status)]),
+    "ustbx_plan_materialize": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_plan_pixels": (c_int64, [c_void_p]),
+    "ustbx_plan_device_bytes": (c_int64, [c_void_p]),
+    "ustbx_plan_destroy": (None, [c_void_p]),
+    "ustbx_synth_echoes": (ctypes.c_int, [ctypes.POINTER(SynthDesc), c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p]),
+}
+
+_lib = None
+
+
+class USTBXError(RuntimeError):
+    """An error returned by the C-ABI, carrying the reference identifier."""
+
+    def __init__(self, code: int, identifier: str, message: str):
+        super().__init__(f"{identifier}: {message}")
+        self.code = code
+        self.identifier = identifier
+
+
+def lib():
+    """Load libustbx.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                               "or python -m ustb_amd.build_lib")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, st: Status):
+    if rc != OK:
+        ident = lib().ustbx_status_id(rc).decode()
+        raise USTBXError(rc, ident, st.message.decode(errors="replace"))

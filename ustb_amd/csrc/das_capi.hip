@@ -1,0 +1,674 @@
+// das_capi.hip — the C-ABI (include/ustbx_das.h): argument validation with the
+// reference's error contract, geometry plans (device-resident probe / scan /
+// sequence state), host staging for MEX-style callers.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "../../include/ustbx_das.h"
+#include "das_launch.h"
+
+using namespace ustbx;
+
+namespace {
+
+int set_status(ustbx_status* st, int code, const char* fmt, ...) {
+  if (st) {
+    st->code = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(st->message, sizeof(st->message), fmt, ap);
+    va_end(ap);
+  }
+  return code;
+}
+
+int ok(ustbx_status* st) {
+  if (st) {
+    st->code = USTBX_OK;
+    st->message[0] = 0;
+  }
+  return USTBX_OK;
+}
+
+int hip_fail(ustbx_status* st, hipError_t e, const char* what) {
+  return set_status(st, USTBX_E_GPU, "HIP error: %s in %s", hipGetErrorString(e), what);
+}
+
+// das_c: wd = 2*(float)PI*fd if |fd| > 1e-6 else 0 (das_c.cpp:32-33,240-244)
+float dasc_wd(float fd) {
+  volatile float two_pi = 2 * (float)3.14159265359;
+  return (fabsf(fd) > 1e-6) ? two_pi * fd : 0.0f;
+}
+
+int auto_split(int P) { return P >= (1 << 18) ? 1 : 4; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t ustbx_abi_version(void) { return USTBX_ABI_VERSION; }
+
+int32_t ustbx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+const char* ustbx_status_id(int32_t code) {
+  switch (code) {
+    case USTBX_OK: return "";
+    case USTBX_E_NRHS: return "Toolbox:SRP_SRC:nrhs";
+    case USTBX_E_NLHS: return "Toolbox:SRP_SRC:nlhs";
+    case USTBX_E_DIMENSIONS: return "Toolbox:SRP_SRC:Dimensions";
+    case USTBX_E_FLOAT: return "Toolbox:SRP_SRC:Float";
+    case USTBX_E_SCALAR: return "Toolbox:SRP_SRC:Scalar";
+    case USTBX_E_GPU: return "Toolbox:SRP_SRC:GPU";
+    case USTBX_E_CAPACITY: return "Toolbox:SRP_SRC:Capacity";
+    case USTBX_E_UNSUPPORTED: return "Toolbox:SRP_SRC:Unsupported";
+    default: return "Toolbox:SRP_SRC:Argument";
+  }
+}
+
+// The das_c argument checks, in das_c's order and with its messages
+// (das_c.cpp:53-244). mat_* index: 0 tx_apo, 1 rx_apo, 2 tx_delay, 3 rx_delay
+// (prhs[3..6]); scalar_*: 0 fs, 1 t0, 2 fd (prhs[1], prhs[2], prhs[7]).
+int ustbx_das_check_mex_args(int32_t nrhs, int32_t nlhs, int32_t ch_ndim,
+                             const int64_t* ch_dims, int32_t ch_is_double,
+                             const int32_t* mat_ndim, const int64_t* mat_dims,
+                             const int32_t* mat_is_double,
+                             const int64_t* scalar_numel,
+                             const int32_t* scalar_is_double, int32_t sum_mode,
+                             int32_t ch_is_complex, float fd, ustbx_das_desc* d,
+                             ustbx_status* st) {
+  if (nrhs < 9) return set_status(st, USTBX_E_NRHS, "Too few input arguments");
+  if (nrhs > 10) return set_status(st, USTBX_E_NRHS, "Too many input arguments");
+  if (nlhs > 1) return set_status(st, USTBX_E_NLHS, "Too many output arguments");
+  if (!ch_dims || !mat_ndim || !mat_dims || !mat_is_double || !scalar_numel ||
+      !scalar_is_double || !d)
+    return set_status(st, USTBX_E_ARG, "NULL argument");
+  if (sum_mode < 0 || sum_mode > 3)
+    return set_status(st, USTBX_E_ARG,
+                      "Unknown sum mode. Available: 0 -> NONE, 1->RX, 2->TX, 3->BOTH");
+  if (ch_ndim < 2 || ch_ndim > 4)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Unknown channel data format. Expected from 2 to 4 dimensions: "
+                      "[time, channel, wave, frame]");
+  const int64_t L = ch_dims[0], N = ch_dims[1];
+  const int64_t W = ch_ndim > 2 ? ch_dims[2] : 1;
+  const int64_t F = ch_ndim > 3 ? ch_dims[3] : 1;
+  if (ch_is_double) return set_status(st, USTBX_E_FLOAT, "The channel data should be single precision");
+  // tx delay (prhs[5])
+  if (mat_ndim[2] > 2)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Unknown transmit delay format. Expected 2 dimensions: [pixel, wave]");
+  const int64_t P = mat_dims[2 * 2 + 0];
+  int64_t Wc = mat_ndim[2] == 2 ? mat_dims[2 * 2 + 1] : 1;
+  if (W != Wc)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Number of waves in channel data & transmit delays do not match.");
+  if (mat_is_double[2]) return set_status(st, USTBX_E_FLOAT, "The transmiy delays should be single precision");
+  // rx delay (prhs[6])
+  if (mat_ndim[3] > 2)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Unknown receive delay format. Expected 2 dimensions: [pixel, channel]");
+  if (mat_dims[3 * 2 + 0] != P)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Number of pixels in transmit and receive delays do not match.");
+  int64_t Nc = mat_ndim[3] == 2 ? mat_dims[3 * 2 + 1] : 1;
+  if (N != Nc)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Number of channels in channel data & receive delays do not match.");
+  if (mat_is_double[3]) return set_status(st, USTBX_E_FLOAT, "The transmiy delays should be single precision");
+  // tx apodization (prhs[3])
+  if (mat_ndim[0] > 2)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Unknown transmit apodization format. Expected 2 dimensions: [pixel, wave]");
+  if (mat_dims[0] != P)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Number of pixels in transmit apodization and transmit delays do not match.");
+  Wc = mat_ndim[0] == 2 ? mat_dims[1] : 1;
+  if (W != Wc)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Number of waves in channel data & transmit apodization do not match.");
+  if (mat_is_double[0]) return set_status(st, USTBX_E_FLOAT, "The transmiy apodization should be single precision");
+  // rx apodization (prhs[4])
+  if (mat_ndim[1] > 2)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Unknown receive apodization format. Expected 2 dimensions: [pixel, channel]");
+  if (mat_dims[1 * 2 + 0] != P)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Number of pixels in receive apodization and receive delays do not match.");
+  Nc = mat_ndim[1] == 2 ? mat_dims[1 * 2 + 1] : 1;
+  if (N != Nc)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "Number of channels in channel data & receive apodization do not match.");
+  if (mat_is_double[1]) return set_status(st, USTBX_E_FLOAT, "The transmiy delays should be single precision");
+  // scalars
+  if (scalar_numel[0] != 1) return set_status(st, USTBX_E_SCALAR, "The sampling frequency should be an escalar");
+  if (scalar_is_double[0]) return set_status(st, USTBX_E_FLOAT, "The sampling frequency should be single precision");
+  if (scalar_numel[1] != 1) return set_status(st, USTBX_E_SCALAR, "The initial time should be an escalar");
+  if (scalar_is_double[1]) return set_status(st, USTBX_E_FLOAT, "The initial time should be single precision");
+  if (scalar_numel[2] != 1) return set_status(st, USTBX_E_SCALAR, "Modulation frequency 'fd' should be an escalar");
+  if (scalar_is_double[2]) return set_status(st, USTBX_E_FLOAT, "The modulation frequency should be single precision");
+  if (fabsf(fd) > 1e-6 && !ch_is_complex)
+    return set_status(st, USTBX_E_ARG,
+                      "The modulation frequency > 0 but the input data is real. Check inputs.");
+  if (L > INT32_MAX || N > INT32_MAX || W > INT32_MAX || F > INT32_MAX || P > INT32_MAX ||
+      L < 0 || N < 0 || W < 0 || F < 0 || P < 0)
+    return set_status(st, USTBX_E_CAPACITY, "array dimension exceeds 2^31-1");
+  d->L = (int32_t)L;
+  d->N = (int32_t)N;
+  d->W = (int32_t)W;
+  d->F = (int32_t)F;
+  d->P = (int32_t)P;
+  d->dim = sum_mode;
+  return ok(st);
+}
+
+static int check_desc(const ustbx_das_desc* d, ustbx_status* st) {
+  if (!d) return set_status(st, USTBX_E_ARG, "NULL descriptor");
+  if (d->dim < 0 || d->dim > 3)
+    return set_status(st, USTBX_E_ARG, "Unknown sum mode. Available: 0 -> NONE, 1->RX, 2->TX, 3->BOTH");
+  if (d->interp != USTBX_INTERP_LINEAR && d->interp != USTBX_INTERP_CUBIC)
+    return set_status(st, USTBX_E_ARG, "unknown interpolation %d", d->interp);
+  if (d->edge != USTBX_EDGE_DASC && d->edge != USTBX_EDGE_PYTHON)
+    return set_status(st, USTBX_E_ARG, "unknown edge rule %d", d->edge);
+  if (d->L < 0 || d->N < 0 || d->W < 0 || d->F < 0 || d->P < 0)
+    return set_status(st, USTBX_E_DIMENSIONS, "negative dimension");
+  if (d->split < 0) return set_status(st, USTBX_E_ARG, "negative split");
+  return USTBX_OK;
+}
+
+static bool empty_problem(const ustbx_das_desc* d) {
+  return d->P == 0 || d->F == 0;
+}
+
+static size_t out_elems(int dim, size_t P, size_t N, size_t W, size_t F) {
+  switch (dim) {
+    case 0: return P * N * W * F;
+    case 1: return P * W * F;
+    case 2: return P * N * F;
+    default: return P * F;
+  }
+}
+
+int ustbx_das_run(const ustbx_das_desc* d, const void* ch, const float* tx_apo,
+                  const float* rx_apo, const float* tx_delay, const float* rx_delay,
+                  void* out, void* stream, ustbx_status* st) {
+  int rc = check_desc(d, st);
+  if (rc) return rc;
+  if (empty_problem(d)) return ok(st);
+  if (!out) return set_status(st, USTBX_E_ARG, "NULL output");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const size_t ne = out_elems(d->dim, d->P, d->N, d->W, d->F);
+  if (d->N == 0 || d->W == 0 || d->L < 2) {
+    // nothing can be interpolated: das_c returns its zero-initialised array
+    hipError_t e = launch_zero_fill(out, ne, s);
+    if (e != hipSuccess) return hip_fail(st, e, "zero_fill_kernel launch");
+    return ok(st);
+  }
+  if (!ch || !tx_apo || !rx_apo || !tx_delay || !rx_delay)
+    return set_status(st, USTBX_E_ARG, "NULL input array");
+  MatLaunch m;
+  m.ch = ch;
+  m.tx_apo = tx_apo;
+  m.rx_apo = rx_apo;
+  m.tx_delay = tx_delay;
+  m.rx_delay = rx_delay;
+  m.out = out;
+  m.L = d->L; m.N = d->N; m.W = d->W; m.F = d->F; m.P = d->P;
+  m.fs = d->fs;
+  m.t0 = d->t0;
+  m.wd = dasc_wd(d->fd);
+  m.dim = d->dim;
+  m.interp = d->interp;
+  m.edge = d->edge;
+  m.split = (d->flags & USTBX_FLAG_DASC_ORDER) ? 1 : (d->split ? d->split : auto_split(d->P));
+  if (m.split != 1 && m.split != 2 && m.split != 4 && m.split != 8)
+    return set_status(st, USTBX_E_ARG, "split must be 1, 2, 4 or 8");
+  hipError_t e = launch_das_matrix(m, s);
+  if (e != hipSuccess) return hip_fail(st, e, "das_matrix_kernel launch");
+  return ok(st);
+}
+
+int ustbx_das_run_host(const ustbx_das_desc* d, const void* ch, const float* tx_apo,
+                       const float* rx_apo, const float* tx_delay, const float* rx_delay,
+                       void* out, int32_t device, ustbx_status* st) {
+  int rc = check_desc(d, st);
+  if (rc) return rc;
+  if (!out) return set_status(st, USTBX_E_ARG, "NULL output");
+  const size_t P = d->P, N = d->N, W = d->W, F = d->F, L = d->L;
+  const size_t ne = out_elems(d->dim, P, N, W, F);
+  if (empty_problem(d)) return ok(st);
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return set_status(st, USTBX_E_GPU, "no HIP device available");
+  if (device < 0 || device >= ndev)
+    return set_status(st, USTBX_E_GPU, "invalid gpu id %d (%d devices)", device, ndev);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_fail(st, e, "hipSetDevice");
+  const size_t bytes_ch = L * N * W * F * 8, bytes_tx = P * W * 4, bytes_rx = P * N * 4;
+  const size_t bytes_out = ne * 8;
+  char* buf = nullptr;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t total = al(bytes_ch) + 2 * al(bytes_tx) + 2 * al(bytes_rx) + al(bytes_out);
+  e = hipMalloc(&buf, total);
+  if (e != hipSuccess) return hip_fail(st, e, "hipMalloc");
+  char* p = buf;
+  void* dch = p; p += al(bytes_ch);
+  float* dta = (float*)p; p += al(bytes_tx);
+  float* dtd = (float*)p; p += al(bytes_tx);
+  float* dra = (float*)p; p += al(bytes_rx);
+  float* drd = (float*)p; p += al(bytes_rx);
+  void* dout = p;
+  int result = USTBX_OK;
+  do {
+    if (bytes_ch && (e = hipMemcpy(dch, ch, bytes_ch, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if (bytes_tx && (e = hipMemcpy(dta, tx_apo, bytes_tx, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if (bytes_tx && (e = hipMemcpy(dtd, tx_delay, bytes_tx, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if (bytes_rx && (e = hipMemcpy(dra, rx_apo, bytes_rx, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if (bytes_rx && (e = hipMemcpy(drd, rx_delay, bytes_rx, hipMemcpyHostToDevice)) != hipSuccess) break;
+    result = ustbx_das_run(d, dch, dta, dra, dtd, drd, dout, nullptr, st);
+    if (result) break;
+    if ((e = hipDeviceSynchronize()) != hipSuccess) break;
+    if ((e = hipMemcpy(out, dout, bytes_out, hipMemcpyDeviceToHost)) != hipSuccess) break;
+  } while (0);
+  (void)hipFree(buf);
+  if (result) return result;
+  if (e != hipSuccess) return hip_fail(st, e, "ustbx_das_run_host");
+  return ok(st);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Geometry plans
+// ---------------------------------------------------------------------------
+struct ustbx_plan {
+  int device;
+  GeomLaunch G;
+  char* dev;
+  size_t dev_bytes;
+  float4* tx_table;      // resident [W][P] transmit table
+  size_t tx_table_bytes;
+};
+
+namespace {
+
+struct Arena {
+  std::vector<char> host;
+  size_t add(const void* src, size_t bytes) {
+    const size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + bytes);
+    if (bytes) memcpy(host.data() + off, src, bytes);
+    return off;
+  }
+};
+
+int check_apod(const ustbx_apod& a, const char* which, ustbx_status* st) {
+  if (a.window < 0 || a.window > 9)
+    return set_status(st, USTBX_E_ARG, "%s apodization: unknown window %d", which, a.window);
+  return USTBX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* st) {
+  if (!g || !plan) return set_status(st, USTBX_E_ARG, "NULL argument");
+  *plan = nullptr;
+  if (g->N <= 0 || g->W <= 0 || g->L < 2 || g->F <= 0)
+    return set_status(st, USTBX_E_DIMENSIONS, "N, W, F must be >= 1 and L >= 2");
+  if (!g->elem_x || !g->elem_y || !g->elem_z || !g->waves)
+    return set_status(st, USTBX_E_ARG, "NULL element or wave array");
+  if (g->dim < 0 || g->dim > 3) return set_status(st, USTBX_E_ARG, "Unknown sum mode");
+  if (g->interp != USTBX_INTERP_LINEAR && g->interp != USTBX_INTERP_CUBIC)
+    return set_status(st, USTBX_E_ARG, "unknown interpolation");
+  if (g->edge != USTBX_EDGE_DASC && g->edge != USTBX_EDGE_PYTHON)
+    return set_status(st, USTBX_E_ARG, "unknown edge rule");
+  int rc = check_apod(g->rx_apod, "receive", st);
+  if (rc) return rc;
+  rc = check_apod(g->tx_apod, "transmit", st);
+  if (rc) return rc;
+  const ustbx_scan& sc = g->scan;
+  if (sc.kind < 0 || sc.kind > 2) return set_status(st, USTBX_E_ARG, "unknown scan kind");
+  if (sc.n0 <= 0 || sc.n1 <= 0 || sc.n2 <= 0) return set_status(st, USTBX_E_DIMENSIONS, "empty scan");
+  const int64_t P64 = (int64_t)sc.n0 * sc.n1 * sc.n2;
+  if (P64 > INT32_MAX) return set_status(st, USTBX_E_CAPACITY, "more than 2^31-1 pixels");
+  if (!sc.a0 || !sc.a1 || !sc.a2) return set_status(st, USTBX_E_ARG, "NULL scan axis");
+  if ((int64_t)g->L * g->N * g->W * 8 > INT32_MAX)
+    return set_status(st, USTBX_E_CAPACITY, "one frame of channel data must be below 2 GiB");
+  const int P = (int)P64, N = g->N, W = g->W;
+
+  Arena ar;
+  size_t off_a0, off_a1, off_a2;
+  if (sc.kind == USTBX_SCAN_LINEAR) {
+    off_a0 = ar.add(sc.a0, sizeof(double) * sc.n0);
+    off_a1 = ar.add(sc.a1, sizeof(double) * sc.n1);
+    off_a2 = ar.add(sc.a2, sizeof(double) * sc.n2);
+  } else if (sc.kind == USTBX_SCAN_SECTOR) {
+    off_a0 = ar.add(sc.a0, sizeof(double) * sc.n0);
+    off_a1 = ar.add(sc.a1, sizeof(double) * 2 * sc.n1);
+    off_a2 = ar.add(sc.a2, sizeof(double) * 2 * sc.n2);
+  } else {
+    off_a0 = ar.add(sc.a0, sizeof(double) * P);
+    off_a1 = ar.add(sc.a1, sizeof(double) * P);
+    off_a2 = ar.add(sc.a2, sizeof(double) * P);
+  }
+  const size_t off_ex = ar.add(g->elem_x, sizeof(double) * N);
+  const size_t off_ey = ar.add(g->elem_y, sizeof(double) * N);
+  const size_t off_ez = ar.add(g->elem_z, sizeof(double) * N);
+
+  // ---- receive apodization (apodization.m:323-468) ----
+  DevRxApod rx;
+  memset(&rx, 0, sizeof(rx));
+  const ustbx_apod& ra = g->rx_apod;
+  size_t off_rvec = SIZE_MAX;
+  std::vector<float> rvec;
+  if (ra.matrix) {
+    rx.kind = RXAP_MATRIX;
+    rx.mat = ra.matrix;
+  } else if (ra.vector) {
+    rx.kind = RXAP_VECTOR;
+    rvec.assign(ra.vector, ra.vector + N);
+  } else if (ra.window == USTBX_WIN_NONE) {
+    rx.kind = RXAP_ONES;
+  } else if (ra.window == USTBX_WIN_STA) {
+    // element(s) closest to the origin (apodization.m:345-349)
+    if (!ra.has_origin) return set_status(st, USTBX_E_ARG, "origin must be set to use STA apodization");
+    std::vector<double> dist(N);
+    double mn = INFINITY;
+    for (int n = 0; n < N; ++n) {
+      const double dx = g->elem_x[n] - ra.origin[0], dy = g->elem_y[n] - ra.origin[1],
+                   dz = g->elem_z[n] - ra.origin[2];
+      dist[n] = sqrt((dx * dx + dy * dy) + dz * dz);
+      mn = fmin(mn, dist[n]);
+    }
+    rx.kind = RXAP_VECTOR;
+    rvec.resize(N);
+    for (int n = 0; n < N; ++n) rvec[n] = dist[n] == mn ? 1.0f : 0.0f;
+  } else if (ra.window == USTBX_WIN_SCANLINE) {
+    return set_status(st, USTBX_E_UNSUPPORTED, "Unknown apodization type!");
+  } else {
+    rx.kind = RXAP_WINDOW;
+    rx.window = ra.window;
+    if (sc.kind == USTBX_SCAN_SECTOR || ra.has_origin) {
+      rx.geo = RXGEO_ORIGIN;
+      const double* o = ra.has_origin ? ra.origin : sc.origin;
+      rx.ox = o[0]; rx.oy = o[1]; rx.oz = o[2];
+    } else {
+      rx.geo = RXGEO_FLAT;
+    }
+  }
+  if (!rvec.empty()) off_rvec = ar.add(rvec.data(), sizeof(float) * N);
+  rx.f0 = ra.f_number[0];
+  rx.f1 = ra.f_number[1];
+  rx.min0 = ra.minimum_aperture[0] * ra.f_number[0];
+  rx.min1 = ra.minimum_aperture[1] * ra.f_number[1];
+  rx.max0 = ra.maximum_aperture[0] * ra.f_number[0];
+  rx.max1 = ra.maximum_aperture[1] * ra.f_number[1];
+  rx.tilt = (fabs(ra.tilt[0]) > 0 ? 1 : 0) | (fabs(ra.tilt[1]) > 0 ? 2 : 0);
+  rx.ct0 = cos(ra.tilt[0]); rx.st0 = sin(ra.tilt[0]);
+  rx.ct1 = cos(ra.tilt[1]); rx.st1 = sin(ra.tilt[1]);
+  rx.win0 = win_eval(ra.window, 0.0);
+
+  // ---- transmit side (das.m:110-206, apodization.m:249-320, 472-555) ----
+  const ustbx_apod& ta = g->tx_apod;
+  DevTxApod tx;
+  memset(&tx, 0, sizeof(tx));
+  tx.window = ta.window;
+  tx.f0 = ta.f_number[0];
+  tx.f1 = ta.f_number[1];
+  tx.min0 = ta.minimum_aperture[0] * ta.f_number[0];
+  tx.min1 = ta.minimum_aperture[1] * ta.f_number[1];
+  tx.max0 = ta.maximum_aperture[0] * ta.f_number[0];
+  tx.max1 = ta.maximum_aperture[1] * ta.f_number[1];
+  tx.gl0 = 0.5 * ta.grating_lobe_angle[0];
+  tx.gl1 = 0.5 * ta.grating_lobe_angle[1];
+  tx.mat = ta.matrix;
+  if (!ta.matrix && ta.window == USTBX_WIN_SCANLINE)
+    return set_status(st, USTBX_E_UNSUPPORTED, "scanline transmit apodization needs an explicit matrix");
+  if (!ta.matrix && ta.window == USTBX_WIN_STA)
+    return set_status(st, USTBX_E_UNSUPPORTED, "Unknown apodization type!");
+
+  std::vector<DevWave> waves(W);
+  for (int w = 0; w < W; ++w) {
+    const ustbx_wave& s = g->waves[w];
+    DevWave& d = waves[w];
+    memset(&d, 0, sizeof(d));
+    d.sx = s.src[0]; d.sy = s.src[1]; d.sz = s.src[2];
+    d.sdist = s.src_distance;
+    d.ca = s.cos_az; d.sa = s.sin_az; d.ce = s.cos_el; d.se = s.sin_el;
+    d.c = s.sound_speed;
+    d.delay = s.delay;
+    d.ox = s.origin[0]; d.oy = s.origin[1]; d.oz = s.origin[2];
+    // delay kind
+    if (g->tx_delay_matrix) {
+      d.txk = TXK_MATRIX;
+    } else if (s.wavefront == USTBX_WAVE_PLANE) {
+      d.txk = TXK_PLANE;
+    } else if (s.wavefront == USTBX_WAVE_PHOTOACOUSTIC) {
+      d.txk = TXK_PHOTO;
+    } else if (s.wavefront == USTBX_WAVE_SPHERICAL) {
+      if (isinf(s.src_distance)) {
+        d.txk = TXK_PLANE;
+      } else if (s.src[2] < 0) {
+        d.txk = TXK_DIVERGING;
+      } else {
+        switch (g->tx_model) {
+          case USTBX_TXMODEL_SPHERICAL: d.txk = TXK_SPHERICAL; break;
+          case USTBX_TXMODEL_HYBRID:
+          case USTBX_TXMODEL_BLENDED:
+            if (sc.kind == USTBX_SCAN_XYZ)
+              return set_status(st, USTBX_E_UNSUPPORTED,
+                                "Only linear scan and sector scan in 2D is supported for the "
+                                "hybrid spherical transmit delay model.");
+            if (g->tx_model == USTBX_TXMODEL_HYBRID)
+              d.txk = sc.kind == USTBX_SCAN_LINEAR ? TXK_HYBRID_LIN : TXK_HYBRID_SEC;
+            else
+              d.txk = sc.kind == USTBX_SCAN_LINEAR ? TXK_BLENDED_LIN : TXK_BLENDED_SEC;
+            break;
+          default:
+            return set_status(st, USTBX_E_UNSUPPORTED,
+                              "unified transmit delay model: pass tx_delay_matrix");
+        }
+      }
+    } else {
+      return set_status(st, USTBX_E_ARG, "Unknown wavefront. Check available options at uff.wavefront");
+    }
+    // apodization kind
+    if (ta.matrix) {
+      d.apk = TXAP_MATRIX;
+    } else if (ta.vector) {
+      d.apk = TXAP_CONST;
+      d.apo_const = ta.vector[w];
+    } else if (ta.window == USTBX_WIN_NONE) {
+      d.apk = TXAP_CONST;
+      d.apo_const = 1.0f;
+    } else if (s.wavefront == USTBX_WAVE_PLANE || isinf(s.src_distance)) {
+      // plane wave: constant per wave (apodization.m:481-487)
+      const double tt = tan(s.azimuth - ta.tilt[0]);
+      const double tp = tan(s.elevation - ta.tilt[1]);
+      const double rt = fabs(ta.f_number[0] * tt), rp = fabs(ta.f_number[1] * tp);
+      d.apk = TXAP_CONST;
+      d.apo_const = (float)(win_eval(ta.window, rt) * win_eval(ta.window, rp));
+    } else {
+      d.apk = TXAP_INCIDENCE;
+      double OS[3] = {s.src[0] - s.origin[0], s.src[1] - s.origin[1], s.src[2] - s.origin[2]};
+      const double nrm = sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2]);
+      if (nrm <= 2.220446049250313e-16) {  // STA (apodization.m:506-518), flat probes
+        OS[0] = 0.0; OS[1] = 0.0; OS[2] = 1.0;
+      }
+      const double ns = sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2]);
+      const double zu[3] = {OS[0] / ns, OS[1] / ns, OS[2] / ns};
+      const double e1[3] = {1.0, 0.0, 0.0};
+      // cross(a, b) = [a2 b3 - a3 b2, a3 b1 - a1 b3, a1 b2 - a2 b1]
+      const double yu[3] = {zu[1] * e1[2] - zu[2] * e1[1], zu[2] * e1[0] - zu[0] * e1[2],
+                            zu[0] * e1[1] - zu[1] * e1[0]};
+      const double xu[3] = {zu[1] * yu[2] - zu[2] * yu[1], zu[2] * yu[0] - zu[0] * yu[2],
+                            zu[0] * yu[1] - zu[1] * yu[0]};
+      d.zu0 = zu[0]; d.zu1 = zu[1]; d.zu2 = zu[2];
+      d.xu0 = xu[0]; d.xu1 = xu[1]; d.xu2 = xu[2];
+      d.yu0 = yu[0]; d.yu1 = yu[1]; d.yu2 = yu[2];
+      d.os_az = atan2(OS[0], OS[2]);
+      d.os_el = atan2(OS[1], OS[2]);
+    }
+  }
+  const size_t off_waves = ar.add(waves.data(), sizeof(DevWave) * W);
+
+  // ---- upload ----
+  ustbx_plan* pl = new (std::nothrow) ustbx_plan;
+  if (!pl) return set_status(st, USTBX_E_CAPACITY, "host allocation failed");
+  memset(pl, 0, sizeof(*pl));
+  hipError_t e = hipGetDevice(&pl->device);
+  if (e != hipSuccess) {
+    delete pl;
+    return hip_fail(st, e, "hipGetDevice");
+  }
+  pl->dev_bytes = ar.host.size();
+  e = hipMalloc(&pl->dev, pl->dev_bytes);
+  if (e != hipSuccess) {
+    delete pl;
+    return hip_fail(st, e, "hipMalloc (plan)");
+  }
+  e = hipMemcpy(pl->dev, ar.host.data(), pl->dev_bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(pl->dev);
+    delete pl;
+    return hip_fail(st, e, "hipMemcpy (plan)");
+  }
+  char* base = pl->dev;
+  GeomLaunch& G = pl->G;
+  memset(&G, 0, sizeof(G));
+  G.L = g->L; G.N = N; G.W = W; G.F = g->F; G.P = P;
+  G.fs = g->fs;
+  G.t0 = g->t0;
+  G.wd = dasc_wd(g->fd);
+  G.dim = g->dim;
+  G.interp = g->interp;
+  G.edge = g->edge;
+  G.separable = (g->flags & USTBX_FLAG_SEPARABLE_PHASE) ? 1 : 0;
+  G.ref_phase = ((g->flags & USTBX_FLAG_REF_PHASE) && G.wd != 0.0f) ? 1 : 0;
+  // das.m:308: exp(-1j*2*w0*r_ref/c); w0 = 2*pi*single(fd) is the FP32 value
+  // das_c calls wd (das.m:88-89), the rest is evaluated in FP64 here
+  G.w0_ref = 2.0 * (double)G.wd / g->sound_speed;
+  G.scan.kind = sc.kind;
+  G.scan.n0 = sc.n0; G.scan.n1 = sc.n1; G.scan.n2 = sc.n2;
+  G.scan.a0 = (const double*)(base + off_a0);
+  G.scan.a1 = (const double*)(base + off_a1);
+  G.scan.a2 = (const double*)(base + off_a2);
+  G.scan.ox = sc.origin[0]; G.scan.oy = sc.origin[1]; G.scan.oz = sc.origin[2];
+  G.scan.ref_kind = sc.ref_kind;
+  G.ex = (const double*)(base + off_ex);
+  G.ey = (const double*)(base + off_ey);
+  G.ez = (const double*)(base + off_ez);
+  G.waves = (const DevWave*)(base + off_waves);
+  if (off_rvec != SIZE_MAX) rx.vec = (const float*)(base + off_rvec);
+  G.rx = rx;
+  G.tx = tx;
+  G.tx_delay_mat = g->tx_delay_matrix;
+  G.c = g->sound_speed;
+  G.lens = g->lens_delay;
+  G.pw_margin = g->pw_margin;
+  G.bpow = g->blending_power;
+  const bool dasc_order = (g->flags & USTBX_FLAG_DASC_ORDER) != 0;
+  G.wave_chunk = dasc_order ? 1 : (G.separable ? (W >= 16 ? 16 : 8) : 8);
+  G.split = dasc_order ? 1 : ((G.dim == 3 || G.dim == 2) ? auto_split(P) : 1);
+
+  // receive aperture culling: one line of elements sorted by x, a window that
+  // is zero above ratio 1/2, no tilt (see GeomLaunch::rx_cull)
+  bool cull = rx.kind == RXAP_WINDOW && rx.tilt == 0 && N >= 2 &&
+              (rx.window == WIN_BOXCAR || rx.window == WIN_HANNING || rx.window == WIN_HAMMING ||
+               rx.window == WIN_TUKEY25 || rx.window == WIN_TUKEY50 || rx.window == WIN_TUKEY75 ||
+               rx.window == WIN_TRIANGLE) &&
+              rx.f0 != 0.0;
+  for (int n = 1; cull && n < N; ++n)
+    cull = g->elem_x[n] > g->elem_x[n - 1] && g->elem_y[n] == g->elem_y[0] && g->elem_z[n] == g->elem_z[0];
+  G.rx_cull = cull ? 1 : 0;
+
+  // resident transmit table
+  pl->tx_table_bytes = (size_t)W * (size_t)P * sizeof(float4);
+  e = hipMalloc(&pl->tx_table, pl->tx_table_bytes);
+  if (e != hipSuccess) {
+    (void)hipFree(pl->dev);
+    delete pl;
+    return hip_fail(st, e, "hipMalloc (transmit table)");
+  }
+  G.tx_table = pl->tx_table;
+  e = launch_geom_tx_table(G, pl->tx_table, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(pl->tx_table);
+    (void)hipFree(pl->dev);
+    delete pl;
+    return hip_fail(st, e, "transmit table");
+  }
+  *plan = pl;
+  return ok(st);
+}
+
+int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split, ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (wave_chunk) {
+    if (wave_chunk != 1 && wave_chunk != 8 && wave_chunk != 16)
+      return set_status(st, USTBX_E_ARG, "wave_chunk must be 1, 8 or 16");
+    if (wave_chunk == 16 && !plan->G.separable)
+      return set_status(st, USTBX_E_ARG, "wave_chunk 16 needs the separable phase");
+    if (wave_chunk == 1 && plan->G.separable)
+      return set_status(st, USTBX_E_ARG, "wave_chunk 1 needs the das_c phase");
+    plan->G.wave_chunk = wave_chunk;
+  }
+  if (split) {
+    if (split != 1 && split != 4) return set_status(st, USTBX_E_ARG, "split must be 1 or 4");
+    if (split != 1 && !(plan->G.dim == 3 || plan->G.dim == 2))
+      return set_status(st, USTBX_E_ARG, "split applies to dimension both/transmit only");
+    plan->G.split = split;
+  }
+  if (plan->G.wave_chunk == 1) plan->G.split = 1;
+  return ok(st);
+}
+
+int ustbx_plan_run(const ustbx_plan* plan, const void* ch, void* out, void* stream,
+                   ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (!ch || !out) return set_status(st, USTBX_E_ARG, "NULL channel data or output");
+  GeomLaunch G = plan->G;
+  G.ch = ch;
+  G.out = out;
+  hipError_t e = launch_das_geom(G, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(st, e, "das_geom_kernel launch");
+  return ok(st);
+}
+
+int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
+                           float* tx_delay, float* rx_delay, void* stream, ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  hipError_t e = launch_geom_materialize(plan->G, tx_apo, rx_apo, tx_delay, rx_delay,
+                                         reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(st, e, "geom_materialize_kernel launch");
+  return ok(st);
+}
+
+int64_t ustbx_plan_pixels(const ustbx_plan* plan) { return plan ? plan->G.P : 0; }
+
+int64_t ustbx_plan_device_bytes(const ustbx_plan* plan) {
+  return plan ? (int64_t)(plan->dev_bytes + plan->tx_table_bytes) : 0;
+}
+
+void ustbx_plan_destroy(ustbx_plan* plan) {
+  if (!plan) return;
+  (void)hipFree(plan->tx_table);
+  (void)hipFree(plan->dev);
+  delete plan;
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// das_launch.h — host-side launch descriptors shared by the kernel files and
+// the C-ABI (das_capi.hip). Plain structs; device pointers only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "das_common.h"
+
+namespace ustbx {
+
+struct MatLaunch {
+  const void* ch;
+  const float* tx_apo;
+  const float* rx_apo;
+  const float* tx_delay;
+  const float* rx_delay;
+  void* out;
+  int L, N, W, F, P;
+  float fs, t0, wd;
+  int dim, interp, edge, split;
+};
+
+hipError_t launch_das_matrix(const MatLaunch& m, hipStream_t stream);
+
+// Fills n complex64 values with zeros (a kernel, so it can be graph-captured).
+hipError_t launch_zero_fill(void* p, size_t n_complex, hipStream_t stream);
+
+// Geometry (on-the-fly) path: everything resident in device memory.
+struct GeomLaunch {
+  const void* ch;
+  void* out;
+  int L, N, W, F, P;
+  float fs, t0, wd;
+  int dim, interp, edge;
+  int separable;       // exp(i wd tx) * exp(i wd rx) instead of exp(i wd (tx+rx))
+  int ref_phase;       // multiply by exp(-1j*2*w0*r_ref/c)
+  double w0_ref;       // 2*w0/c for the reference phase, in FP64
+  DevScan scan;
+  const double* ex;    // element positions [N]
+  const double* ey;
+  const double* ez;
+  const DevWave* waves;  // [W]
+  DevRxApod rx;
+  DevTxApod tx;
+  const float* tx_delay_mat;  // [W][P] or null
+  double c, lens, pw_margin, bpow;
+  int wave_chunk;      // waves per register chunk (autotuned)
+  int split;           // channel groups per pixel tile (BOTH/RX)
+  // Resident transmit table [W][P] of {tx delay, tx apodization,
+  // tx apodization * exp(i wd tx delay)}: constant for a plan, built once.
+  const float4* tx_table;
+  // Receive aperture culling: elements sorted by x on one line (y, z equal),
+  // a window with support ratio <= 1/2 and no tilt, so the channels a pixel
+  // can see form one contiguous index range.
+  int rx_cull;
+};
+
+// Builds the transmit table of a plan (one lane per (wave, pixel)).
+hipError_t launch_geom_tx_table(const GeomLaunch& g, float4* table, hipStream_t stream);
+
+hipError_t launch_das_geom(const GeomLaunch& g, hipStream_t stream);
+
+// Writes the geometry's FP32 matrices (das.m's inputs to the MEX).
+hipError_t launch_geom_materialize(const GeomLaunch& g, float* tx_apo,
+                                   float* rx_apo, float* tx_delay,
+                                   float* rx_delay, hipStream_t stream);
+
+}  // namespace ustbx

@@ -1,0 +1,138 @@
+"""Device operators over the C-ABI, on torch tensors (torch supplies device
+memory and the current HIP stream; the arithmetic is the library's).
+
+Layouts follow the reference MEX arrays (MATLAB column-major) held as torch
+row-major tensors with reversed dimensions:
+    ch        (F, W, N, L) complex64      <- channel_data [L, N, W, F]
+    tx_*      (W, P) float32              <- [P, W]
+    rx_*      (N, P) float32              <- [P, N]
+    output    (F, W', N', P) complex64    <- [P, N', W', F]
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib as L
+
+
+def _stream_ptr(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def out_shape(dim: int, P: int, N: int, W: int, F: int) -> tuple:
+    return {L.DIM_NONE: (F, W, N, P), L.DIM_RX: (F, W, 1, P), L.DIM_TX: (F, 1, N, P),
+            L.DIM_BOTH: (F, 1, 1, P)}[int(dim)]
+
+
+def _require_cuda(*ts):
+    for t in ts:
+        if not (isinstance(t, torch.Tensor) and t.is_cuda):
+            raise TypeError("device tensors are required (the HIP path has no CPU fallback)")
+        if not t.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+
+
+def das(ch: torch.Tensor, fs: float, t0: float, tx_apo: torch.Tensor, rx_apo: torch.Tensor,
+        tx_delay: torch.Tensor, rx_delay: torch.Tensor, fd: float, dim: int, *,
+        interp: int = L.INTERP_LINEAR, edge: int = L.EDGE_DASC, dasc_order: bool = False,
+        split: int = 0, out: torch.Tensor | None = None) -> torch.Tensor:
+    """mex.das_c / mex.das_cuda contract on device tensors (all four modes)."""
+    _require_cuda(ch, tx_apo, rx_apo, tx_delay, rx_delay)
+    if ch.dtype != torch.complex64:
+        raise TypeError("The channel data should be single precision complex")
+    for t in (tx_apo, rx_apo, tx_delay, rx_delay):
+        if t.dtype != torch.float32:
+            raise TypeError("delays and apodization should be single precision")
+    if ch.dim() != 4:
+        raise ValueError("channel data must be (F, W, N, L)")
+    F, W, N, Ls = ch.shape
+    P = tx_delay.shape[-1]
+    if tuple(tx_delay.shape) != (W, P) or tuple(tx_apo.shape) != (W, P):
+        raise ValueError("Number of waves in channel data & transmit delays/apodization do not match.")
+    if tuple(rx_delay.shape) != (N, P) or tuple(rx_apo.shape) != (N, P):
+        raise ValueError("Number of channels in channel data & receive delays/apodization do not match.")
+    shape = out_shape(dim, P, N, W, F)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.complex64, device=ch.device)
+    elif tuple(out.shape) != shape or out.dtype != torch.complex64 or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous complex64 tensor of shape {shape}")
+    d = L.DasDesc(L=Ls, N=N, W=W, F=F, P=P, fs=fs, t0=t0, fd=fd, dim=int(dim), interp=int(interp),
+                  edge=int(edge), flags=(L.FLAG_DASC_ORDER if dasc_order else 0), split=int(split))
+    st = L.Status()
+    rc = L.lib().ustbx_das_run(ctypes.byref(d), ch.data_ptr(), tx_apo.data_ptr(), rx_apo.data_ptr(),
+                               tx_delay.data_ptr(), rx_delay.data_ptr(), out.data_ptr(),
+                               _stream_ptr(ch.device), ctypes.byref(st))
+    L.check(rc, st)
+    return out
+
+
+class GeometryPlan:
+    """Device-resident geometry (probe, scan, sequence, apodization) for one
+    acquisition configuration; delays and apodization are computed on the fly."""
+
+    def __init__(self, geometry: L.Geometry, keepalive=(), device: torch.device | None = None):
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        self._keepalive = keepalive
+        self.handle = ctypes.c_void_p()
+        st = L.Status()
+        with torch.cuda.device(self.device):
+            rc = L.lib().ustbx_plan_create(ctypes.byref(geometry), ctypes.byref(self.handle),
+                                           ctypes.byref(st))
+        L.check(rc, st)
+        self.N, self.W, self.L, self.F = geometry.N, geometry.W, geometry.L, geometry.F
+        self.dim = geometry.dim
+        self.P = int(L.lib().ustbx_plan_pixels(self.handle))
+        self._keepalive = None  # host arrays were copied into the plan
+
+    @property
+    def out_shape(self):
+        return out_shape(self.dim, self.P, self.N, self.W, self.F)
+
+    @property
+    def device_bytes(self) -> int:
+        return int(L.lib().ustbx_plan_device_bytes(self.handle))
+
+    def set_tiles(self, wave_chunk: int = 0, split: int = 0):
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_set_tiles(self.handle, wave_chunk, split, ctypes.byref(st)), st)
+
+    def run(self, ch: torch.Tensor, out: torch.Tensor | None = None,
+            stream: torch.cuda.Stream | None = None) -> torch.Tensor:
+        _require_cuda(ch)
+        if ch.dtype != torch.complex64 or tuple(ch.shape) != (self.F, self.W, self.N, self.L):
+            raise ValueError(f"channel data must be complex64 {(self.F, self.W, self.N, self.L)}")
+        if out is None:
+            out = torch.empty(self.out_shape, dtype=torch.complex64, device=ch.device)
+        elif tuple(out.shape) != self.out_shape or out.dtype != torch.complex64 or not out.is_contiguous():
+            raise ValueError(f"out must be contiguous complex64 {self.out_shape}")
+        sp = stream.cuda_stream if stream is not None else _stream_ptr(ch.device)
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_run(self.handle, ch.data_ptr(), out.data_ptr(), sp,
+                                       ctypes.byref(st)), st)
+        return out
+
+    def materialize(self):
+        """FP32 (tx_apo, rx_apo, tx_delay, rx_delay) as das.m hands them to the MEX."""
+        kw = dict(dtype=torch.float32, device=self.device)
+        ta = torch.empty((self.W, self.P), **kw)
+        td = torch.empty((self.W, self.P), **kw)
+        ra = torch.empty((self.N, self.P), **kw)
+        rd = torch.empty((self.N, self.P), **kw)
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_materialize(self.handle, ta.data_ptr(), ra.data_ptr(), td.data_ptr(),
+                                               rd.data_ptr(), _stream_ptr(self.device),
+                                               ctypes.byref(st)), st)
+        return ta, ra, td, rd
+
+    def close(self):
+        if self.handle and self.handle.value:
+            L.lib().ustbx_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
