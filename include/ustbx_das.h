@@ -234,6 +234,11 @@ int ustbx_plan_run(const ustbx_plan* plan, const void* ch, void* out,
  * channel groups per pixel tile (1 or 4; dimension both/transmit). */
 int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
                          ustbx_status* st);
+/* Kernel variant: 0 = per-lane wave chunks (every mode), 1 = receive state
+ * shared in LDS by four wave-splits of a 64-pixel tile (dimension both, IQ,
+ * separable phase, one frame; the default where it applies). For variant 1
+ * set_tiles' wave_chunk is the number of waves per split (8..32). */
+int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st);
 /* Writes the plan's FP32 delay/apodization matrices (what das.m hands to the
  * MEX) into device buffers: tx [W][P], rx [N][P]; any may be NULL. */
 int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
@@ -242,6 +247,13 @@ int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
 int64_t ustbx_plan_pixels(const ustbx_plan* plan);
 int64_t ustbx_plan_device_bytes(const ustbx_plan* plan);
 void ustbx_plan_destroy(ustbx_plan* plan);
+
+/* Analytic signal of real RF traces along time (FFT method), device buffers:
+ * rf [traces][L] float32 -> analytic [traces][L] complex64. Replaces the RF
+ * branch of +midprocess/das.m:211-214 (MATLAB hilbert) and
+ * python/src/ustb/midprocess/das.py:144-145 (scipy.signal.hilbert). */
+int ustbx_hilbert(const float* rf, void* analytic, int32_t L, int64_t traces,
+                  void* stream, ustbx_status* st);
 
 /* ---------------------------------------------------------------------------
  * 3. Library information.

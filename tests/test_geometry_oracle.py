@@ -1,0 +1,202 @@
+"""Pins of the oracle's geometry restatement (CPU): closed forms of das.m's
+delays, textbook window values, aperture support, pixel grids, and the known
+answers of the reference's own tests (python/tests/test_das.py,
+tests/integration_*_test.m), run through the oracle's FP32 pipeline."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.cases import wave_dict
+from ustb_amd import synth
+from ustb_amd.uff import LinearArray, LinearScan, MatrixArray, Point, SectorScan, Wave
+
+C = 1540.0
+
+
+def test_rx_delay_on_axis_is_z_over_c():
+    # python/tests/test_das.py:144-157: pixel (0, 0, z), centre element
+    d = O.rx_delay(np.array([0.0]), np.array([0.0]), np.array([30e-3]), np.array([0.0]), np.array([0.0]),
+                   np.array([0.0]), C)
+    assert d[0, 0] == np.float32(30e-3 / C)
+
+
+def test_rx_delay_pythagoras():
+    prb = LinearArray(16, 0.3e-3)
+    px, py, pz = np.array([1e-3, -2e-3]), np.zeros(2), np.array([10e-3, 20e-3])
+    d = O.rx_delay(px, py, pz, prb.x, prb.y, prb.z, C, lens_delay=1e-7)
+    ref = np.hypot(prb.x[:, None] - px[None, :], pz[None, :]) / C + 1e-7
+    assert np.allclose(d, ref, rtol=1e-7)
+
+
+def test_plane_wave_delays():
+    # python/tests/test_das.py:174-187 (0 deg: z/c); steered: z cos + x sin
+    px, py, pz = np.array([0.0, 3e-3]), np.zeros(2), np.array([20e-3, 25e-3])
+    w0 = wave_dict(Wave.plane(0.0))
+    assert O.tx_delay(px, py, pz, [w0])[0, 0] == pytest.approx(20e-3 / C, rel=1e-6)
+    a = math.radians(12.0)
+    d = O.tx_delay(px, py, pz, [wave_dict(Wave.plane(a))])[0]
+    assert np.allclose(d, (pz * math.cos(a) + px * math.sin(a)) / C, rtol=1e-6)
+
+
+def test_photoacoustic_is_zero():
+    # python/tests/test_das.py:161-172
+    w = wave_dict(Wave(wavefront=2))
+    assert O.tx_delay(np.array([0.0]), np.array([0.0]), np.array([20e-3]), [w])[0, 0] == 0.0
+
+
+def test_diverging_wave_delay():
+    # das.m:119-124: |s - x| - |s.distance| for a source behind the array
+    w = Wave.spherical(-2e-3, 0.0, -10e-3)
+    px, py, pz = np.array([0.0, 5e-3]), np.zeros(2), np.array([20e-3, 30e-3])
+    d = O.tx_delay(px, py, pz, [wave_dict(w)])[0]
+    ref = (np.hypot(px + 2e-3, pz + 10e-3) - math.hypot(2e-3, 10e-3)) / C
+    assert np.allclose(d, ref, rtol=1e-6)
+
+
+def test_focused_models():
+    """spherical: |s-x| signed by (z < sz) + distance; hybrid replaces the
+    +-pw_margin band around the focal depth by the plane delay (das.m:125-188)."""
+    w = Wave.spherical(0.0, 0.0, 15e-3)
+    pz = np.array([10e-3, 15.5e-3, 25e-3])
+    px, py = np.zeros(3), np.zeros(3)
+    sph = O.tx_delay(px, py, pz, [wave_dict(w)], model=O.MODEL_SPHERICAL)[0] * C
+    assert np.allclose(sph, [-5e-3 + 15e-3, 0.5e-3 + 15e-3, 10e-3 + 15e-3], rtol=1e-6)
+    hyb = O.tx_delay(px, py, pz, [wave_dict(w)], model=O.MODEL_HYBRID)[0] * C
+    assert np.allclose(hyb, sph, rtol=1e-6)  # on axis the plane patch equals the spherical delay
+
+
+@pytest.mark.parametrize("win,r,v", [
+    (O.WIN_TUKEY25, 0.0, 1.0), (O.WIN_TUKEY25, 0.375, 1.0), (O.WIN_TUKEY25, 0.4375, 0.5),
+    (O.WIN_TUKEY25, 0.5, 0.0), (O.WIN_TUKEY50, 0.375, 0.5), (O.WIN_HANNING, 0.0, 1.0),
+    (O.WIN_HANNING, 0.25, 0.5), (O.WIN_HANNING, 0.5, 0.0), (O.WIN_HANNING, 0.51, 0.0),
+    (O.WIN_HAMMING, 0.5, 0.07672), (O.WIN_HAMMING, 0.6, 0.0), (O.WIN_TRIANGLE, 0.25, 0.5),
+    (O.WIN_BOXCAR, 0.5, 1.0), (O.WIN_BOXCAR, 0.5000001, 0.0)])
+def test_window_values(win, r, v):
+    assert O.window(win, np.array([r]))[0] == pytest.approx(v, abs=1e-12)
+
+
+def test_window_nan_semantics():
+    """A logical mask times NaN stays NaN in MATLAB (apodization.m:172-186)."""
+    assert np.isnan(O.window(O.WIN_HANNING, np.array([np.inf]))[0])
+    assert np.isnan(O.window(O.WIN_TUKEY25, np.array([np.nan]))[0])
+    assert O.window(O.WIN_BOXCAR, np.array([np.inf]))[0] == 0.0
+
+
+def test_rx_aperture_support_and_symmetry():
+    """Boxcar at f#: nonzero iff |x_pixel - x_element| <= z / (2 f#)."""
+    prb = LinearArray(64, 0.3e-3)
+    px, py, pz = np.array([0.0, 1.05e-3]), np.zeros(2), np.array([10e-3, 17e-3])
+    a = O.rx_apod(px, py, pz, prb.x, prb.y, prb.z, O.WIN_BOXCAR, (1.7, 1.7))
+    for j in range(2):
+        inside = np.abs(px[j] - prb.x) <= pz[j] / (2 * 1.7)
+        assert np.array_equal(a[:, j] != 0, inside)
+    h = O.rx_apod(np.array([0.0]), np.array([0.0]), np.array([12e-3]), prb.x, prb.y, prb.z, O.WIN_HANNING,
+                  (1.5, 1.5))[:, 0]
+    assert np.array_equal(h, h[::-1])
+
+
+def test_tx_apod_plane_is_constant_per_wave():
+    a = math.radians(10.0)
+    P = 50
+    t = O.tx_apod(np.linspace(-5e-3, 5e-3, P), np.zeros(P), np.linspace(5e-3, 30e-3, P),
+                  [wave_dict(Wave.plane(a))], O.WIN_TUKEY25, (1.7, 1.7))
+    assert np.all(t == t[0, 0])
+    assert t[0, 0] == np.float32(O.window(O.WIN_TUKEY25, np.array([1.7 * math.tan(a)]))[0])
+
+
+def test_pixel_grids():
+    s = LinearScan(np.array([-1e-3, 0.0, 2e-3]), np.array([5e-3, 6e-3]))
+    x, y, z = O.linear_scan_xyz(s.x_axis, s.z_axis)
+    assert np.array_equal(x, [-1e-3, -1e-3, 0, 0, 2e-3, 2e-3]) and np.array_equal(z, [5e-3, 6e-3] * 3)
+    assert np.array_equal(x, s.x) and np.array_equal(z, s.z)
+    az = np.deg2rad([-30.0, 0.0, 30.0])
+    sc = SectorScan(az, np.array([10e-3, 20e-3]))
+    x, y, z, rho = O.sector_scan_xyz(az, sc.depth_axis)
+    assert np.allclose(x, rho * np.sin(np.repeat(az, 2))) and np.allclose(z, rho * np.cos(np.repeat(az, 2)))
+    assert np.allclose(sc.x, x, atol=1e-18) and np.allclose(sc.z, z, atol=1e-18)
+
+
+def test_element_positions():
+    x = O.linear_array_x(4, 0.3e-3)
+    assert np.allclose(x, [-0.45e-3, -0.15e-3, 0.15e-3, 0.45e-3])
+    assert np.array_equal(x, LinearArray(4, 0.3e-3).x)
+    mx, my = O.matrix_array_xy(3, 2, 1e-3, 2e-3)
+    m = MatrixArray(3, 2, 1e-3, 2e-3)
+    assert np.array_equal(mx, m.x) and np.array_equal(my, m.y)
+    assert np.allclose(mx, [-1e-3, 0, 1e-3] * 2) and np.allclose(my, [-1e-3] * 3 + [1e-3] * 3)
+
+
+def test_point_getters():
+    # point.m:121-144
+    p = Point.from_xyz(3e-3, 0.0, 4e-3)
+    assert p.distance == pytest.approx(5e-3) and p.xyz == pytest.approx((3e-3, 0.0, 4e-3))
+    assert Point(1.0, math.pi / 2, 0.0).z == 0.0 and Point(2.0, 0.0, 0.0).x == 0.0
+
+
+def test_reference_phase_unit_modulus():
+    ph = O.reference_phase(np.array([0.0, 1e-3, 20e-3]), 5.2e6, C)
+    assert ph[0] == 1.0 and np.allclose(np.abs(ph), 1.0)
+
+
+# ---------------------------------------------------------------------------
+# known answers of the reference's own tests, through the oracle pipeline
+# ---------------------------------------------------------------------------
+def _oracle_image(prb, scan, seq, ch, fs, fd, rx_win, rx_f, tx_win=O.WIN_NONE, tx_f=1.0, dim=3,
+                  scan_kind="linear"):
+    waves = [wave_dict(w) for w in seq]
+    ra = O.rx_apod(scan.x, scan.y, scan.z, prb.x, prb.y, prb.z, rx_win, (rx_f, rx_f))
+    rd = O.rx_delay(scan.x, scan.y, scan.z, prb.x, prb.y, prb.z, C)
+    ta = O.tx_apod(scan.x, scan.y, scan.z, waves, tx_win, (tx_f, tx_f))
+    td = O.tx_delay(scan.x, scan.y, scan.z, waves, scan_kind)
+    return O.das_c(ch, fs, 0.0, ta, ra, td, rd, fd, dim)
+
+
+def test_focus_at_correct_depth():
+    """python/tests/test_das.py:263-303: point at 30 mm, 32 elements, plane wave,
+    Gaussian pulse; the envelope peak lies within 2 mm of 30 mm."""
+    prb = LinearArray(32, 0.3e-3)
+    fs, N, Ls = 40e6, 32, 4096
+    ch = np.zeros((1, 1, N, Ls), np.complex64)
+    t = np.arange(Ls)
+    for i in range(N):
+        tau = 30e-3 / C + math.hypot(prb.x[i], 30e-3) / C
+        ch[0, 0, i] = np.exp(-0.5 * ((t - tau * fs) / 2.0) ** 2)
+    z = np.linspace(10e-3, 50e-3, 200)
+    scan = LinearScan(np.array([0.0]), z)
+    img = _oracle_image(prb, scan, [Wave.plane(0.0)], ch, fs, 5e6, O.WIN_NONE, 1.0)
+    assert abs(z[np.argmax(np.abs(img[0, 0, 0]))] - 30e-3) < 2e-3
+
+
+@pytest.mark.parametrize("dim,shape", [(3, (1, 1, 1, 10)), (1, (1, 3, 1, 10)), (2, (1, 1, 8, 10)),
+                                       (0, (1, 3, 8, 10))])
+def test_output_shape_per_dimension(dim, shape):
+    """python/tests/test_das.py:190-229 (as (F, W', N', P))."""
+    rng = np.random.default_rng(0)
+    prb = LinearArray(8, 0.3e-3)
+    scan = LinearScan(np.array([0.0]), np.linspace(5e-3, 50e-3, 10))
+    ch = rng.standard_normal((1, 3, 8, 512)).astype(np.complex64)
+    img = _oracle_image(prb, scan, [Wave.plane(0.0)] * 3, ch, 40e6, 0.0, O.WIN_NONE, 1.0, dim=dim)
+    assert img.shape == shape and np.all(np.isfinite(img))
+
+
+@pytest.mark.parametrize("kind", ["cpwc", "dw", "sta"])
+def test_point_target_peak_within_2mm(kind):
+    """tests/integration_{cpwc,dw,sta}_linear_test.m: a simulated point
+    scatterer is imaged within 2 mm of its position."""
+    prb = LinearArray(64, 0.3e-3)
+    fs, f0 = 41.6e6, 5.2e6
+    if kind == "cpwc":
+        seq, tgt = [Wave.plane(a) for a in np.linspace(-0.3, 0.3, 11)], (0.0, 0.0, 20e-3)
+    elif kind == "dw":
+        seq, tgt = [Wave.spherical(x, 0.0, -20e-3) for x in np.linspace(-9.6e-3, 9.6e-3, 5)], (0.0, 0.0, 20e-3)
+    else:
+        seq = [Wave.spherical(x, 0.0, 0.0, delay=abs(x) / C) for x in prb.x[::4]]
+        tgt = (1e-3, 0.0, 15e-3)
+    rf = synth.echoes(prb, seq, np.array([tgt]), np.ones(1), fs=fs, n_samples=2400, f0=f0, bandwidth=0.6, iq=False)
+    ch = synth.analytic(rf)
+    scan = LinearScan(np.linspace(-5e-3, 5e-3, 41), np.linspace(10e-3, 30e-3, 81))
+    img = _oracle_image(prb, scan, seq, ch, fs, 0.0, O.WIN_HANNING, 1.7)[0, 0, 0]
+    k = np.argmax(np.abs(img))
+    assert abs(scan.x[k] - tgt[0]) < 2e-3 and abs(scan.z[k] - tgt[2]) < 2e-3

@@ -155,11 +155,13 @@ def test_geometry_reference_phase(cuda, name):
     plan.close()
 
 
-@pytest.mark.parametrize("wc,split", [(8, 1), (8, 4), (16, 1), (16, 4)])
-def test_tile_shapes_agree(cuda, wc, split):
+@pytest.mark.parametrize("variant,wc,split", [(0, 8, 1), (0, 8, 4), (0, 16, 1), (0, 16, 4), (1, 8, 4),
+                                              (1, 10, 4), (1, 12, 8), (1, 16, 8)])
+def test_tile_shapes_agree(cuda, variant, wc, split):
     case = CASES["cpwc_tukey"]
     ch = torch.from_numpy(channel_noise(case, seed=13)).to(cuda)
     plan = _plan(case, dim=3, separable_phase=True)
+    plan.set_variant(variant)
     plan.set_tiles(wave_chunk=wc, split=split)
     got = plan.run(ch).cpu().numpy()
     ta, ra, td, rd = oracle_matrices(case)

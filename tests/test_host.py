@@ -1,0 +1,139 @@
+"""Host-side tests (CPU): the C-ABI library loads and exports every symbol the
+headers declare; das_c's argument contract (+mex/source/das_c.cpp:53-252) as
+enforced by ustbx_das_check_mex_args; loud failure without the library;
+enumerations; the synthetic generator's determinism."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ustb_amd import _lib as L
+from ustb_amd import synth
+from ustb_amd.enums import Code, Dimension, SphericalTransmitDelayModel, Wavefront, Window
+from ustb_amd.uff import LinearArray, Wave
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(ustbx_\w+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    declared = _declared_functions()
+    assert len(declared) >= 14
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(L.EXPORTS) == declared
+
+
+def test_abi_version_and_device_count():
+    assert L.lib().ustbx_abi_version() == 1
+    assert L.lib().ustbx_device_count() >= 0
+
+
+def _check(nrhs=9, nlhs=1, ch_ndim=4, ch_dims=(64, 8, 3, 2), ch_double=0, mat_ndim=(2, 2, 2, 2),
+           mat_dims=((100, 3), (100, 8), (100, 3), (100, 8)), mat_double=(0, 0, 0, 0), numel=(1, 1, 1),
+           sc_double=(0, 0, 0), mode=3, cplx=1, fd=5e6):
+    d = L.DasDesc()
+    st = L.Status()
+    arr = lambda t, v: (t * len(v))(*v)
+    rc = L.lib().ustbx_das_check_mex_args(
+        nrhs, nlhs, ch_ndim, arr(ctypes.c_int64, list(ch_dims) + [0] * (4 - len(ch_dims))), ch_double,
+        arr(ctypes.c_int32, mat_ndim), arr(ctypes.c_int64, [x for p in mat_dims for x in p]),
+        arr(ctypes.c_int32, mat_double), arr(ctypes.c_int64, numel), arr(ctypes.c_int32, sc_double), mode, cplx,
+        fd, ctypes.byref(d), ctypes.byref(st))
+    return rc, L.lib().ustbx_status_id(rc).decode(), st.message.decode(), d
+
+
+def test_mex_args_accept_valid_call():
+    rc, ident, msg, d = _check()
+    assert rc == 0 and (d.L, d.N, d.W, d.F, d.P, d.dim) == (64, 8, 3, 2, 100, 3)
+    rc, _, _, d = _check(ch_ndim=2, ch_dims=(64, 8), mat_dims=((100, 1), (100, 8), (100, 1), (100, 8)),
+                         mat_ndim=(2, 2, 2, 2))
+    assert rc == 0 and (d.W, d.F) == (1, 1)
+
+
+@pytest.mark.parametrize("kw,ident,text", [
+    (dict(nrhs=8), "Toolbox:SRP_SRC:nrhs", "Too few input arguments"),
+    (dict(nrhs=11), "Toolbox:SRP_SRC:nrhs", "Too many input arguments"),
+    (dict(nlhs=2), "Toolbox:SRP_SRC:nlhs", "Too many output arguments"),
+    (dict(mode=4), "Toolbox:SRP_SRC:Argument", "Unknown sum mode"),
+    (dict(ch_ndim=5), "Toolbox:SRP_SRC:Dimensions", "Unknown channel data format"),
+    (dict(ch_double=1), "Toolbox:SRP_SRC:Float", "channel data should be single"),
+    (dict(mat_ndim=(2, 2, 3, 2)), "Toolbox:SRP_SRC:Dimensions", "Unknown transmit delay format"),
+    (dict(mat_dims=((100, 3), (100, 8), (100, 4), (100, 8))), "Toolbox:SRP_SRC:Dimensions",
+     "Number of waves in channel data & transmit delays"),
+    (dict(mat_dims=((100, 3), (100, 8), (100, 3), (99, 8))), "Toolbox:SRP_SRC:Dimensions",
+     "Number of pixels in transmit and receive delays"),
+    (dict(mat_dims=((100, 3), (100, 8), (100, 3), (100, 9))), "Toolbox:SRP_SRC:Dimensions",
+     "Number of channels in channel data & receive delays"),
+    (dict(mat_dims=((101, 3), (100, 8), (100, 3), (100, 8))), "Toolbox:SRP_SRC:Dimensions",
+     "Number of pixels in transmit apodization"),
+    (dict(mat_dims=((100, 3), (100, 7), (100, 3), (100, 8))), "Toolbox:SRP_SRC:Dimensions",
+     "Number of channels in channel data & receive apodization"),
+    (dict(mat_double=(0, 0, 1, 0)), "Toolbox:SRP_SRC:Float", "transmiy delays"),
+    (dict(numel=(2, 1, 1)), "Toolbox:SRP_SRC:Scalar", "sampling frequency should be an escalar"),
+    (dict(sc_double=(0, 1, 0)), "Toolbox:SRP_SRC:Float", "initial time should be single"),
+    (dict(numel=(1, 1, 3)), "Toolbox:SRP_SRC:Scalar", "Modulation frequency 'fd' should be an escalar"),
+    (dict(cplx=0), "Toolbox:SRP_SRC:Argument", "input data is real"),
+])
+def test_mex_arg_errors(kw, ident, text):
+    rc, got_ident, msg, _ = _check(**kw)
+    assert rc < 0 and got_ident == ident and text in msg
+
+
+def test_real_data_without_modulation_is_accepted():
+    rc, *_ = _check(cplx=0, fd=0.0)
+    assert rc == 0
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    monkeypatch.setattr(L, "_lib", None)
+    monkeypatch.setattr(L, "LIB_PATH", "/nonexistent/libustbx.so")
+    with pytest.raises(RuntimeError, match="not built"):
+        L.lib()
+
+
+def test_ops_reject_host_tensors():
+    import torch
+
+    from ustb_amd import ops
+
+    x = torch.zeros((1, 1, 2, 8), dtype=torch.complex64)
+    m = torch.zeros((1, 4))
+    with pytest.raises(TypeError, match="device tensors"):
+        ops.das(x, 1.0, 0.0, m, torch.zeros((2, 4)), m, torch.zeros((2, 4)), 0.0, 3)
+
+
+def test_enums_match_reference():
+    assert [int(d) for d in Dimension] == [0, 1, 2, 3]           # dimension.m:27-32
+    assert (Code.matlab, Code.mex, Code.matlab_gpu, Code.mex_gpu) == (0, 1, 3, 4)  # code.m:23-28
+    assert Code.python == 2 and Code.mex_hip == 5
+    assert (Window.none, Window.boxcar, Window.hanning, Window.tukey25, Window.sta, Window.scanline,
+            Window.triangle) == (0, 1, 2, 4, 7, 8, 9)              # window.m:24-37
+    assert (Wavefront.plane, Wavefront.spherical, Wavefront.photoacoustic) == (0, 1, 2)
+    assert SphericalTransmitDelayModel.hybrid == 3
+
+
+def test_synth_deterministic_across_thread_counts():
+    prb = LinearArray(8, 0.3e-3)
+    seq = [Wave.plane(0.1), Wave.spherical(1e-3, 0.0, -5e-3)]
+    rng = np.random.default_rng(0)
+    pts = np.stack([rng.uniform(-3e-3, 3e-3, 50), np.zeros(50), rng.uniform(5e-3, 20e-3, 50)], 1)
+    amps = rng.standard_normal(50)
+    for iq in (True, False):
+        kw = dict(fs=20e6, n_samples=512, f0=5e6, bandwidth=0.6, iq=iq)
+        a = synth.echoes(prb, seq, pts, amps, threads=1, **kw)
+        b = synth.echoes(prb, seq, pts, amps, threads=7, **kw)
+        assert np.array_equal(a, b)
+        assert a.shape == (1, 2, 8, 512) and np.abs(a).max() > 0

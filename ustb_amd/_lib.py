@@ -101,6 +101,7 @@ EXPORTS = {
                                          ctypes.POINTER(Status)]),
     "ustbx_plan_run": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(Status)]),
     "ustbx_plan_set_tiles": (ctypes.c_int, [c_void_p, c_int32, c_int32, ctypes.POINTER(Status)]),
+    "ustbx_plan_set_variant": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
     "ustbx_plan_materialize": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                               c_void_p, ctypes.POINTER(Status)]),
     "ustbx_plan_pixels": (c_int64, [c_void_p]),
@@ -108,6 +109,7 @@ EXPORTS = {
     "ustbx_plan_destroy": (None, [c_void_p]),
     "ustbx_synth_echoes": (ctypes.c_int, [ctypes.POINTER(SynthDesc), c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p]),
+    "ustbx_hilbert": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, ctypes.POINTER(Status)]),
 }
 
 _lib = None

@@ -59,7 +59,8 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
         if failed:
             raise RuntimeError("hipcc failed for: " + ", ".join(os.path.basename(f[0]) for f in failed))
         tmp_lib = LIB + ".tmp"
-        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp_lib, *objs]
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp_lib, *objs,
+               "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
         p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
         if p.returncode != 0:
             raise RuntimeError("link failed:\n" + p.stdout)

@@ -47,6 +47,20 @@ float dasc_wd(float fd) {
 
 int auto_split(int P) { return P >= (1 << 18) ? 1 : 4; }
 
+// The shared-receive-state kernel covers dimension both with the separable
+// phase on one frame.
+bool shared_rx_applies(const GeomLaunch& G) { return G.dim == 3 && G.separable && G.F == 1 && G.wd != 0.0f; }
+
+// Default tiles of the shared-receive kernel: 8 wave-splits, the smallest
+// instantiated waves-per-split covering W in one round (else 8 per round).
+void shared_rx_default_tiles(int W, int& wc, int& ws) {
+  ws = 8;
+  const int need = (W + 7) / 8;
+  wc = need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : need <= 16 ? 16 : 8;
+}
+
+bool shared_rx_wc_ok(int wc) { return wc == 8 || wc == 10 || wc == 12 || wc == 16; }
+
 }  // namespace
 
 extern "C" {
@@ -345,7 +359,9 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   const int64_t P64 = (int64_t)sc.n0 * sc.n1 * sc.n2;
   if (P64 > INT32_MAX) return set_status(st, USTBX_E_CAPACITY, "more than 2^31-1 pixels");
   if (!sc.a0 || !sc.a1 || !sc.a2) return set_status(st, USTBX_E_ARG, "NULL scan axis");
-  if ((int64_t)g->L * g->N * g->W * 8 > INT32_MAX)
+  // gathers use 32-bit byte offsets within a frame, and an offset just below
+  // 2^31 as the always-out-of-range position of skipped terms
+  if ((int64_t)g->L * g->N * g->W * 8 > 0x7FFFFE00)
     return set_status(st, USTBX_E_CAPACITY, "one frame of channel data must be below 2 GiB");
   const int P = (int)P64, N = g->N, W = g->W;
 
@@ -583,6 +599,11 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   const bool dasc_order = (g->flags & USTBX_FLAG_DASC_ORDER) != 0;
   G.wave_chunk = dasc_order ? 1 : (G.separable ? (W >= 16 ? 16 : 8) : 8);
   G.split = dasc_order ? 1 : ((G.dim == 3 || G.dim == 2) ? auto_split(P) : 1);
+  G.variant = 0;
+  if (shared_rx_applies(G)) {
+    G.variant = 1;
+    shared_rx_default_tiles(W, G.wave_chunk, G.split);
+  }
 
   // receive aperture culling: one line of elements sorted by x, a window that
   // is zero above ratio 1/2, no tilt (see GeomLaunch::rx_cull)
@@ -616,8 +637,34 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   return ok(st);
 }
 
+int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (variant != 0 && variant != 1) return set_status(st, USTBX_E_ARG, "variant must be 0 or 1");
+  if (variant == 1 && !shared_rx_applies(plan->G))
+    return set_status(st, USTBX_E_UNSUPPORTED,
+                      "the shared-receive kernel needs dimension both, IQ data, the separable phase and one frame");
+  if (variant == plan->G.variant) return ok(st);
+  plan->G.variant = variant;
+  if (variant == 1) {
+    shared_rx_default_tiles(plan->G.W, plan->G.wave_chunk, plan->G.split);
+  } else {
+    plan->G.wave_chunk = plan->G.separable ? 16 : 8;
+    plan->G.split = auto_split(plan->G.P);
+  }
+  return ok(st);
+}
+
 int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split, ustbx_status* st) {
   if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (plan->G.variant == 1) {
+    if (wave_chunk && !shared_rx_wc_ok(wave_chunk))
+      return set_status(st, USTBX_E_ARG, "waves per split must be 8, 10, 12 or 16");
+    if (split && split != 4 && split != 8)
+      return set_status(st, USTBX_E_ARG, "wave-splits per block must be 4 or 8");
+    if (wave_chunk) plan->G.wave_chunk = wave_chunk;
+    if (split) plan->G.split = split;
+    return ok(st);
+  }
   if (wave_chunk) {
     if (wave_chunk != 1 && wave_chunk != 8 && wave_chunk != 16)
       return set_status(st, USTBX_E_ARG, "wave_chunk must be 1, 8 or 16");

@@ -382,6 +382,142 @@ void das_geom_kernel(GeomLaunch G) {
   }  // frame chunk
 }
 
+// ---------------------------------------------------------------------------
+// dimension.both, separable phase, one frame: the benchmark path.
+//
+// A block is one tile of 64 pixels (lanes) x 4 wave-splits (the block's four
+// wave64s). Split s owns transmit waves w = r0 + s + WS*j (j < WC) of each
+// round of WS*WC waves (dealt round-robin, so splits stay balanced), with their tx
+// delay and tx weight T in VGPRs. The
+// receive state of a group of CG channels -- delay rd and weight
+// R = ra * exp(i wd rd) per (pixel, channel) -- is computed once per block,
+// each wave taking CG/4 channels, and shared through LDS; every split then
+// applies it to its own waves. Receive geometry is thus evaluated once per
+// (pixel, channel) instead of once per wave chunk. The four partial sums of a
+// pixel are added in split order at the end (deterministic).
+// ---------------------------------------------------------------------------
+template <int INTERP, int WC, int WS>
+__global__ __launch_bounds__(64 * WS) __attribute__((amdgpu_waves_per_eu(4)))
+void das_both_shared_rx_kernel(GeomLaunch G) {
+  constexpr int TP = 64, CG = 16;
+  __shared__ float4 rxs[CG][TP];
+  __shared__ float2 red[WS][TP];
+  const int lane = threadIdx.x & (TP - 1);
+  const int s = threadIdx.x / TP;
+  const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int p = tile * TP + lane;
+  const bool active = p < G.P;
+  const size_t PS = (size_t)G.P;
+  const float4* __restrict__ txt = G.tx_table;
+  const __amdgpu_buffer_rsrc_t rs = make_rsrc(G.ch, G.L * G.N * G.W * 8);
+  const int l8 = G.L * 8;
+  const int ln8 = G.L * G.N * 8;
+  const float lim = (float)(INTERP == INTERP_LINEAR ? G.L - 1 : G.L - 2);
+  const float lo_ok = (G.edge == EDGE_PYTHON && INTERP == INTERP_LINEAR) ? 0.0f : 1.0f;
+
+  double px = 0.0, py = 0.0, pz = 0.0, rref = 0.0;
+  if (active) pixel_xyz(G.scan, p, px, py, pz, rref);
+  int nlo = G.N, nhi = 0;
+  if (active) rx_range(G, px, py, pz, nlo, nhi);
+  nlo = wave_min(nlo);  // every wave of the block holds the same 64 pixels,
+  nhi = wave_max(nhi);  // so the range is uniform across the block
+
+  float2 acc = make_float2(0.0f, 0.0f);
+  for (int r0 = 0; r0 < G.W; r0 += WS * WC) {
+    float td[WC];
+    float2 T[WC];
+#pragma unroll
+    for (int j = 0; j < WC; ++j) {
+      const int w = r0 + s + WS * j;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (active && w < G.W) v = txt[(size_t)w * PS + p];
+      td[j] = v.x;
+      T[j] = make_float2(v.z, v.w);
+    }
+    for (int g0 = nlo; g0 < nhi; g0 += CG) {
+      // receive state of channels g0 .. g0+CG-1 for this tile
+#pragma unroll
+      for (int c = s; c < CG; c += WS) {
+        const int n = g0 + c;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (active && n < nhi) {
+          const double ex = G.ex[n], ey = G.ey[n], ez = G.ez[n];
+          const float ra = rx_apod_f32(G.rx, px, py, pz, ex, ey, ez, n, p, G.P);
+          if (ra != 0.0f) {
+            const float rd = rx_delay_f32(px, py, pz, ex, ey, ez, G.c, G.lens);
+            float sv, cv;
+            sincos_wd_hw(G.wd, rd, sv, cv);
+            v = make_float4(rd, ra * cv, ra * sv, 1.0f);
+          }
+        }
+        rxs[c][lane] = v;
+      }
+      __syncthreads();
+      const int cn = min(CG, nhi - g0);
+      for (int c = 0; c < cn; ++c) {
+        const float4 rv = rxs[c][lane];
+        if (rv.w != 0.0f) {
+          const float rd = rv.x;
+          const int noff = (g0 + c) * l8;
+          float2 part = make_float2(0.0f, 0.0f);
+#pragma unroll
+          for (int j = 0; j < WC; ++j) {
+            const int w = r0 + s + WS * j;  // waves dealt round-robin over the splits
+            if (w < G.W && (T[j].x != 0.0f || T[j].y != 0.0f)) {
+              float d;
+              {
+#pragma clang fp contract(off)
+                d = ((td[j] + rd) - G.t0) * G.fs;  // das_c.cpp:372,375
+              }
+              if (d >= lo_ok && d < lim) {
+                const int n0 = (int)d;
+                const float b = __builtin_amdgcn_fractf(d);
+                const int off = noff + w * ln8 + (n0 << 3);
+                float2 sm;
+                if (INTERP == INTERP_LINEAR) {
+                  const float4 x = buf_load16(rs, off);
+                  sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
+                } else {
+                  const float4 lo = buf_load16(rs, off - 8);
+                  const float4 hi = buf_load16(rs, off + 8);
+                  float c0, c1, c2, c3;
+                  keys_weights(b, c0, c1, c2, c3);
+                  sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
+                                   fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
+                }
+                cfma(part, T[j], sm);
+              }
+            }
+          }
+          cfma(acc, make_float2(rv.y, rv.z), part);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  red[s][lane] = acc;
+  __syncthreads();
+  if (s == 0 && active) {
+    float2 v = red[0][lane];
+#pragma unroll
+    for (int k = 1; k < WS; ++k) cadd(v, red[k][lane]);
+    float2 refp = make_float2(1.0f, 0.0f);
+    if (G.ref_phase) {
+      double sv, cv;
+      sincos(-(G.w0_ref * rref), &sv, &cv);
+      refp = make_float2((float)cv, (float)sv);
+    }
+    reinterpret_cast<float2*>(G.out)[p] = ref_apply(v, refp, G.ref_phase);
+  }
+}
+
+template <int INTERP, int WC, int WS>
+hipError_t launch_both_shared_rx_t(const GeomLaunch& G, hipStream_t stream) {
+  const unsigned grid = (unsigned)((G.P + 63) / 64);
+  hipLaunchKernelGGL((das_both_shared_rx_kernel<INTERP, WC, WS>), dim3(grid), dim3(64 * WS), 0, stream, G);
+  return hipGetLastError();
+}
+
 template <int MODE, int INTERP, int PH, int SPLIT, int WC, int FC>
 hipError_t launch_geom_t(const GeomLaunch& G, hipStream_t stream) {
   constexpr int TP = kBlock / SPLIT;
@@ -392,6 +528,7 @@ hipError_t launch_geom_t(const GeomLaunch& G, hipStream_t stream) {
 }
 
 // per-mode dispatchers (one translation unit each)
+hipError_t launch_both_shared(const GeomLaunch& G, hipStream_t s);
 hipError_t launch_geom_both(const GeomLaunch& G, int ph, int wc, int split, hipStream_t s);
 hipError_t launch_geom_tx(const GeomLaunch& G, int ph, int wc, int split, hipStream_t s);
 hipError_t launch_geom_rx_none(const GeomLaunch& G, int ph, int wc, hipStream_t s);

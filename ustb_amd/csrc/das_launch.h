@@ -53,6 +53,10 @@ struct GeomLaunch {
   // a window with support ratio <= 1/2 and no tilt, so the channels a pixel
   // can see form one contiguous index range.
   int rx_cull;
+  // Kernel variant: 0 = per-lane wave chunks (all modes), 1 = receive state
+  // shared through LDS across four wave-splits (dimension both, separable
+  // phase, one frame).
+  int variant;
 };
 
 // Builds the transmit table of a plan (one lane per (wave, pixel)).

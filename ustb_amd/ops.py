@@ -68,6 +68,22 @@ def das(ch: torch.Tensor, fs: float, t0: float, tx_apo: torch.Tensor, rx_apo: to
     return out
 
 
+def hilbert(rf: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """Analytic signal along the last axis (MATLAB/scipy hilbert) of a
+    float32 device tensor; returns complex64 of the same shape."""
+    _require_cuda(rf)
+    if rf.dtype != torch.float32:
+        raise TypeError("RF data must be float32")
+    Ls = rf.shape[-1]
+    traces = rf.numel() // max(Ls, 1)
+    if out is None:
+        out = torch.empty(rf.shape, dtype=torch.complex64, device=rf.device)
+    st = L.Status()
+    L.check(L.lib().ustbx_hilbert(rf.data_ptr(), out.data_ptr(), Ls, traces, _stream_ptr(rf.device),
+                                  ctypes.byref(st)), st)
+    return out
+
+
 class GeometryPlan:
     """Device-resident geometry (probe, scan, sequence, apodization) for one
     acquisition configuration; delays and apodization are computed on the fly."""
@@ -83,6 +99,9 @@ class GeometryPlan:
         L.check(rc, st)
         self.N, self.W, self.L, self.F = geometry.N, geometry.W, geometry.L, geometry.F
         self.dim = geometry.dim
+        self.interp = geometry.interp
+        self.separable = bool(geometry.flags & L.FLAG_SEPARABLE_PHASE)
+        self.fd = geometry.fd
         self.P = int(L.lib().ustbx_plan_pixels(self.handle))
         self._keepalive = None  # host arrays were copied into the plan
 
@@ -93,6 +112,12 @@ class GeometryPlan:
     @property
     def device_bytes(self) -> int:
         return int(L.lib().ustbx_plan_device_bytes(self.handle))
+
+    def set_variant(self, variant: int):
+        """0: per-lane wave chunks; 1: receive state shared in LDS (dimension
+        both, IQ, separable phase, one frame)."""
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_set_variant(self.handle, int(variant), ctypes.byref(st)), st)
 
     def set_tiles(self, wave_chunk: int = 0, split: int = 0):
         st = L.Status()
