@@ -63,8 +63,9 @@ typedef struct ustbx_status {
 
 /* flags */
 #define USTBX_FLAG_DASC_ORDER 1u /* one thread per pixel, das_c summation order
-                                    (w outer, n inner): bit-identical to das_c
-                                    whenever fd == 0 */
+                                    (w outer, n inner) and das_c's phase:
+                                    bit-identical to das_c whenever fd == 0;
+                                    overrides USTBX_FLAG_SEPARABLE_PHASE */
 #define USTBX_FLAG_SEPARABLE_PHASE 2u /* geometry path: exp(i wd (tx+rx)) taken as
                                          exp(i wd tx) * exp(i wd rx) */
 #define USTBX_FLAG_REF_PHASE 4u  /* geometry path: multiply by
@@ -230,15 +231,27 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan,
 /* ch: device [F][W][N][L] complex64; out: device [F][W'][N'][P] complex64. */
 int ustbx_plan_run(const ustbx_plan* plan, const void* ch, void* out,
                    void* stream, ustbx_status* st);
-/* Tile shape override (0 = keep): waves per register chunk (1, 8, 16) and
- * channel groups per pixel tile (1 or 4; dimension both/transmit). */
+/* Tile shape override (0 = keep). Variant 0: waves per register chunk (1, 8,
+ * 16) and channel groups per pixel tile (1 or 4; dimension both/transmit).
+ * Variants 1-3: waves per wave-split (variant 1: 8, 10, 12, 16; variants 2
+ * and 3: 4, 6, 8, 10, 12, 16) and wave-splits per 64-pixel tile (4 or 8). */
 int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
                          ustbx_status* st);
-/* Kernel variant: 0 = per-lane wave chunks (every mode), 1 = receive state
- * shared in LDS by four wave-splits of a 64-pixel tile (dimension both, IQ,
- * separable phase, one frame; the default where it applies). For variant 1
- * set_tiles' wave_chunk is the number of waves per split (8..32). */
+/* Kernel variant (results agree to FP32 rounding; no reference counterpart):
+ *   0 = per-lane wave chunks (every mode);
+ *   1 = receive state shared in LDS by the wave-splits of a 64-pixel tile
+ *       (dimension both, IQ, separable phase, one frame);
+ *   2 = resident receive table and per-tile wave/channel lists (dimension
+ *       both, separable or RF phase, any frame count, receive table within
+ *       8 GiB, finite nonzero weights and delays; the default where it
+ *       applies);
+ *   3 = variant 2 over cubic-interpolation coefficients: the plan holds a
+ *       buffer of 32 bytes per channel sample that every run rewrites, so
+ *       runs of one variant-3 plan on different streams must not overlap.
+ * USTBX_E_UNSUPPORTED where the variant does not apply. */
 int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st);
+/* Current variant (or -1 for NULL) and its tile shape. */
+int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split);
 /* Writes the plan's FP32 delay/apodization matrices (what das.m hands to the
  * MEX) into device buffers: tx [W][P], rx [N][P]; any may be NULL. */
 int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,

@@ -156,7 +156,8 @@ def test_geometry_reference_phase(cuda, name):
 
 
 @pytest.mark.parametrize("variant,wc,split", [(0, 8, 1), (0, 8, 4), (0, 16, 1), (0, 16, 4), (1, 8, 4),
-                                              (1, 10, 4), (1, 12, 8), (1, 16, 8)])
+                                              (1, 10, 4), (1, 12, 8), (1, 16, 8), (2, 4, 4), (2, 6, 8),
+                                              (2, 8, 8), (2, 10, 4), (2, 12, 8), (2, 16, 4)])
 def test_tile_shapes_agree(cuda, variant, wc, split):
     case = CASES["cpwc_tukey"]
     ch = torch.from_numpy(channel_noise(case, seed=13)).to(cuda)
@@ -168,4 +169,57 @@ def test_tile_shapes_agree(cuda, variant, wc, split):
     ref = O.beamform_fp64(ch.cpu().numpy(), case["fs"], 0.0, ta, ra, td, rd, case["fd"], 3,
                           w0=O.dasc_wd(case["fd"]), index="fp32")
     assert _rel(got, ref) <= TOL
+    plan.close()
+
+
+def _nonfinite_weight(case):
+    """Whether a nonzero weight of the oracle's matrices is not finite."""
+    ta, ra, td, rd = oracle_matrices(case)
+    return bool(np.any((ta != 0) & ~np.isfinite(ta)) or np.any((ra != 0) & ~np.isfinite(ra)))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("interp", [Interpolation.linear, Interpolation.cubic])
+def test_receive_table_variant(cuda, name, interp):
+    """Variant 2 (resident receive table, branch-free gathers) applies exactly
+    when every nonzero weight is finite; where it applies it matches the
+    FP64 oracle (das_c's wd and FP32 index) and variant 0 on two frames."""
+    case = CASES[name]
+    plan = _plan(case, dim=3, F=2, separable_phase=True, interpolation=int(interp))
+    applies = plan.has_variant(2)
+    assert applies == (not _nonfinite_weight(case))
+    if not applies:
+        assert plan.variant[0] != 2
+        plan.close()
+        return
+    assert plan.variant[0] == 2  # the default where it applies
+    ch = channel_noise(case, F=2, seed=21)
+    chd = torch.from_numpy(ch).to(cuda)
+    got = plan.run(chd).cpu().numpy()
+    if interp == Interpolation.cubic:  # the coefficient form of the same polynomial
+        plan.set_variant(3)
+        for wc, ws in ((6, 4), (10, 8), (16, 8)):
+            plan.set_tiles(wc, ws)
+            assert _rel(plan.run(chd).cpu().numpy(), got) <= TOL
+    plan.set_variant(0)
+    v0 = plan.run(chd).cpu().numpy()
+    assert _rel(got, v0) <= TOL
+    if interp == Interpolation.linear:
+        ta, ra, td, rd = oracle_matrices(case)
+        ref = O.beamform_fp64(ch, case["fs"], 0.0, ta, ra, td, rd, case["fd"], 3, w0=O.dasc_wd(case["fd"]),
+                              index="fp32")
+        assert _rel(got, ref) <= TOL
+    plan.close()
+
+
+def test_receive_table_rejected_modes(cuda):
+    """Variant 2 covers dimension both only, and not das_c's summation order."""
+    case = CASES["cpwc_tukey"]
+    for dim in (0, 1, 2):
+        plan = _plan(case, dim=dim, separable_phase=True)
+        assert not plan.has_variant(2)
+        plan.close()
+    plan = _plan(case, dim=3, dasc_order=True)
+    assert not plan.has_variant(2)
+    assert plan.variant[0] == 0
     plan.close()

@@ -21,6 +21,8 @@ from . import _lib as L
 VARIANT0 = [(0, wc, sp) for wc in (8, 16) for sp in (1, 4)]
 VARIANT0_EXACT = [(0, 8, 1), (0, 8, 4)]
 VARIANT1 = [(1, wc, ws) for ws in (4, 8) for wc in (8, 10, 12, 16)]
+VARIANT2 = [(2, wc, ws) for ws in (4, 8) for wc in (4, 6, 8, 10, 12, 16)]
+VARIANT3 = [(3, wc, ws) for ws in (4, 8) for wc in (6, 10, 16)]
 
 _cache: dict = {}
 _lock = threading.Lock()
@@ -38,6 +40,10 @@ def candidates(plan):
         c = [(0, 8, 1)] if not plan.separable else [(0, 8, 1), (0, 16, 1)]
     if plan.dim == L.DIM_BOTH and plan.separable and plan.F == 1 and plan.fd != 0.0:
         c += VARIANT1
+    if plan.dim == L.DIM_BOTH and plan.has_variant(2):
+        c += VARIANT2
+        if plan.interp == L.INTERP_CUBIC and plan.has_variant(3):
+            c += VARIANT3
     return c
 
 
@@ -63,7 +69,8 @@ def tune(plan, ch: torch.Tensor, reps: int = 3, cache_file: str | None = None, v
     ref = None
     times = {}
     stream = torch.cuda.current_stream(ch.device)
-    for cand in candidates(plan):
+    cands = candidates(plan)
+    for cand in cands:
         apply(plan, cand)
         plan.run(ch, out)  # warm-up (and first-touch of code)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -80,7 +87,7 @@ def tune(plan, ch: torch.Tensor, reps: int = 3, cache_file: str | None = None, v
             peak = torch.nan_to_num(ref).abs().max().item()
             err = torch.nan_to_num(img - ref).abs().max().item()
             if not err <= 1e-5 * max(peak, 1e-30):
-                raise RuntimeError(f"tile candidate {cand} disagrees with {candidates(plan)[0]}: {err / peak:.2e}")
+                raise RuntimeError(f"tile candidate {cand} disagrees with {cands[0]}: {err / peak:.2e}")
         times[cand] = ms
         if verbose:
             print(f"[autotune] {key} {cand}: {ms:.3f} ms", flush=True)

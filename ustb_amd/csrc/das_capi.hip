@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -60,6 +61,20 @@ void shared_rx_default_tiles(int W, int& wc, int& ws) {
 }
 
 bool shared_rx_wc_ok(int wc) { return wc == 8 || wc == 10 || wc == 12 || wc == 16; }
+
+// The receive-table kernel covers dimension both in the separable (or RF)
+// phase, outside das_c's summation order.
+constexpr size_t kRxTableMaxBytes = size_t(8) << 30;
+
+bool rxtab_wc_ok(int wc) { return wc == 4 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
+
+// 8 wave-splits; the smallest waves-per-split covering the busiest tile in
+// one round (else 16 per round).
+void rxtab_default_tiles(int max_nw, int& wc, int& ws) {
+  ws = 8;
+  const int need = (max_nw + 7) / 8;
+  wc = need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : 16;
+}
 
 }  // namespace
 
@@ -313,9 +328,97 @@ struct ustbx_plan {
   size_t dev_bytes;
   float4* tx_table;      // resident [W][P] transmit table
   size_t tx_table_bytes;
+  bool dasc_order;
+  // variant 2 (receive table): 0 not built, 1 built, -1 not applicable
+  int rxtab_state;
+  int max_nw;            // largest active-wave count of a tile
+  char* rxtab;           // one allocation: receive table, tile data, reference phase
+  size_t rxtab_bytes;
+  size_t coef_bytes;     // variant 3's coefficient buffer (G.coef)
 };
 
 namespace {
+
+bool rxtab_applies(const ustbx_plan* pl) {
+  const GeomLaunch& G = pl->G;
+  if (G.dim != 3 || pl->dasc_order || !(G.separable || G.wd == 0.0f)) return false;
+  if (G.edge == USTBX_EDGE_PYTHON && G.interp == USTBX_INTERP_LINEAR) return false;
+  if ((size_t)G.N * (size_t)G.P * sizeof(float4) > kRxTableMaxBytes) return false;
+  return (int64_t)G.L * G.N * G.W * 8 <= 0x7FFFFE00;
+}
+
+// Variant 3 adds a coefficient buffer of 32 bytes per channel sample (cubic).
+bool coef_applies(const ustbx_plan* pl) {
+  const GeomLaunch& G = pl->G;
+  return G.interp == USTBX_INTERP_CUBIC && (int64_t)G.L * G.N * G.W * 32 <= 0x7FFFFE00;
+}
+
+// Builds the variant-2 state once; returns false (and frees it) when the
+// variant does not apply or a nonzero weight or its delay is not finite.
+bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
+  err = hipSuccess;
+  if (pl->rxtab_state != 0) return pl->rxtab_state > 0;
+  pl->rxtab_state = -1;
+  if (!rxtab_applies(pl)) return false;
+  GeomLaunch& G = pl->G;
+  const size_t P = (size_t)G.P, tiles = (P + 63) / 64;
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t b_rx = al(sizeof(float4) * (size_t)G.N * P);
+  const size_t b_ch = al(sizeof(int2) * tiles);
+  const size_t b_wl = al(sizeof(int) * tiles * (size_t)G.W);
+  const size_t b_nw = al(sizeof(int) * tiles);
+  const size_t b_bad = al(sizeof(int) * tiles);
+  const size_t b_ref = G.ref_phase ? al(sizeof(float2) * P) : 0;
+  const size_t total = b_rx + b_ch + b_wl + b_nw + b_bad + b_ref;
+  char* m = nullptr;
+  err = hipMalloc(&m, total);
+  if (err != hipSuccess) return false;
+  float4* rx = (float4*)m;
+  int2* tch = (int2*)(m + b_rx);
+  int* twl = (int*)(m + b_rx + b_ch);
+  int* tnw = (int*)(m + b_rx + b_ch + b_wl);
+  int* tbad = (int*)(m + b_rx + b_ch + b_wl + b_nw);
+  float2* ref = b_ref ? (float2*)(m + b_rx + b_ch + b_wl + b_nw + b_bad) : nullptr;
+  err = launch_rxtab_tables(G, rx, ref, tch, twl, tnw, tbad, nullptr);
+  if (err == hipSuccess) err = hipDeviceSynchronize();
+  std::vector<int> nw(tiles), bad(tiles);
+  if (err == hipSuccess) err = hipMemcpy(nw.data(), tnw, sizeof(int) * tiles, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(bad.data(), tbad, sizeof(int) * tiles, hipMemcpyDeviceToHost);
+  bool finite = true;
+  int max_nw = 0;
+  for (size_t t = 0; t < tiles; ++t) {
+    finite = finite && bad[t] == 0;
+    max_nw = std::max(max_nw, nw[t]);
+  }
+  if (err != hipSuccess || !finite) {
+    (void)hipFree(m);
+    return false;
+  }
+  pl->rxtab = m;
+  pl->rxtab_bytes = total;
+  pl->max_nw = max_nw;
+  pl->rxtab_state = 1;
+  G.rx_table = rx;
+  G.tile_ch = tch;
+  G.tile_waves = twl;
+  G.tile_nw = tnw;
+  G.ref_table = ref;
+  return true;
+}
+
+bool coef_build(ustbx_plan* pl, hipError_t& err) {
+  err = hipSuccess;
+  if (pl->G.coef) return true;
+  if (!coef_applies(pl)) return false;
+  const GeomLaunch& G = pl->G;
+  const size_t bytes = (size_t)32 * G.L * G.N * G.W * (size_t)G.F;
+  float4* c = nullptr;
+  err = hipMalloc(&c, bytes);
+  if (err != hipSuccess) return false;
+  pl->G.coef = c;
+  pl->coef_bytes = bytes;
+  return true;
+}
 
 struct Arena {
   std::vector<char> host;
@@ -572,7 +675,8 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   G.dim = g->dim;
   G.interp = g->interp;
   G.edge = g->edge;
-  G.separable = (g->flags & USTBX_FLAG_SEPARABLE_PHASE) ? 1 : 0;
+  // das_c's summation order comes with das_c's phase exp(i wd (tx+rx))
+  G.separable = ((g->flags & USTBX_FLAG_SEPARABLE_PHASE) && !(g->flags & USTBX_FLAG_DASC_ORDER)) ? 1 : 0;
   G.ref_phase = ((g->flags & USTBX_FLAG_REF_PHASE) && G.wd != 0.0f) ? 1 : 0;
   // das.m:308: exp(-1j*2*w0*r_ref/c); w0 = 2*pi*single(fd) is the FP32 value
   // das_c calls wd (das.m:88-89), the rest is evaluated in FP64 here
@@ -600,7 +704,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   G.wave_chunk = dasc_order ? 1 : (G.separable ? (W >= 16 ? 16 : 8) : 8);
   G.split = dasc_order ? 1 : ((G.dim == 3 || G.dim == 2) ? auto_split(P) : 1);
   G.variant = 0;
-  if (shared_rx_applies(G)) {
+  if (!dasc_order && shared_rx_applies(G)) {
     G.variant = 1;
     shared_rx_default_tiles(W, G.wave_chunk, G.split);
   }
@@ -633,19 +737,49 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     delete pl;
     return hip_fail(st, e, "transmit table");
   }
+  // the receive-table kernel where it applies
+  pl->dasc_order = dasc_order;
+  if (rxtab_build(pl, e)) {
+    G.variant = 2;
+    rxtab_default_tiles(pl->max_nw, G.wave_chunk, G.split);
+  } else if (e != hipSuccess) {
+    (void)hipFree(pl->tx_table);
+    (void)hipFree(pl->dev);
+    delete pl;
+    return hip_fail(st, e, "receive table");
+  }
   *plan = pl;
   return ok(st);
 }
 
 int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) {
   if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
-  if (variant != 0 && variant != 1) return set_status(st, USTBX_E_ARG, "variant must be 0 or 1");
-  if (variant == 1 && !shared_rx_applies(plan->G))
+  if (variant < 0 || variant > 3) return set_status(st, USTBX_E_ARG, "variant must be 0, 1, 2 or 3");
+  if (variant == 1 && (plan->dasc_order || !shared_rx_applies(plan->G)))
     return set_status(st, USTBX_E_UNSUPPORTED,
                       "the shared-receive kernel needs dimension both, IQ data, the separable phase and one frame");
+  if (variant >= 2) {
+    hipError_t e;
+    if (!rxtab_build(plan, e)) {
+      if (e != hipSuccess) return hip_fail(st, e, "receive table");
+      return set_status(st, USTBX_E_UNSUPPORTED,
+                        "the receive-table kernel needs dimension both, the separable (or RF) phase, a receive "
+                        "table within 8 GiB and finite nonzero weights and delays");
+    }
+    if (variant == 3 && !coef_build(plan, e)) {
+      if (e != hipSuccess) return hip_fail(st, e, "coefficient buffer");
+      return set_status(st, USTBX_E_UNSUPPORTED,
+                        "the coefficient variant needs cubic interpolation and 32 bytes per sample within 2 GiB "
+                        "per frame");
+    }
+  }
   if (variant == plan->G.variant) return ok(st);
+  const bool same_family = variant >= 2 && plan->G.variant >= 2;
   plan->G.variant = variant;
-  if (variant == 1) {
+  if (same_family) return ok(st);
+  if (variant >= 2) {
+    rxtab_default_tiles(plan->max_nw, plan->G.wave_chunk, plan->G.split);
+  } else if (variant == 1) {
     shared_rx_default_tiles(plan->G.W, plan->G.wave_chunk, plan->G.split);
   } else {
     plan->G.wave_chunk = plan->G.separable ? 16 : 8;
@@ -656,6 +790,15 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) 
 
 int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split, ustbx_status* st) {
   if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (plan->G.variant >= 2) {
+    if (wave_chunk && !rxtab_wc_ok(wave_chunk))
+      return set_status(st, USTBX_E_ARG, "waves per split must be 4, 6, 8, 10, 12 or 16");
+    if (split && split != 4 && split != 8)
+      return set_status(st, USTBX_E_ARG, "wave-splits per block must be 4 or 8");
+    if (wave_chunk) plan->G.wave_chunk = wave_chunk;
+    if (split) plan->G.split = split;
+    return ok(st);
+  }
   if (plan->G.variant == 1) {
     if (wave_chunk && !shared_rx_wc_ok(wave_chunk))
       return set_status(st, USTBX_E_ARG, "waves per split must be 8, 10, 12 or 16");
@@ -708,11 +851,20 @@ int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
 int64_t ustbx_plan_pixels(const ustbx_plan* plan) { return plan ? plan->G.P : 0; }
 
 int64_t ustbx_plan_device_bytes(const ustbx_plan* plan) {
-  return plan ? (int64_t)(plan->dev_bytes + plan->tx_table_bytes) : 0;
+  return plan ? (int64_t)(plan->dev_bytes + plan->tx_table_bytes + plan->rxtab_bytes + plan->coef_bytes) : 0;
+}
+
+int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split) {
+  if (!plan) return -1;
+  if (wave_chunk) *wave_chunk = plan->G.wave_chunk;
+  if (split) *split = plan->G.split;
+  return plan->G.variant;
 }
 
 void ustbx_plan_destroy(ustbx_plan* plan) {
   if (!plan) return;
+  (void)hipFree(plan->G.coef);
+  (void)hipFree(plan->rxtab);
   (void)hipFree(plan->tx_table);
   (void)hipFree(plan->dev);
   delete plan;

@@ -54,13 +54,31 @@ struct GeomLaunch {
   // can see form one contiguous index range.
   int rx_cull;
   // Kernel variant: 0 = per-lane wave chunks (all modes), 1 = receive state
-  // shared through LDS across four wave-splits (dimension both, separable
-  // phase, one frame).
+  // shared through LDS across wave-splits (dimension both, separable phase,
+  // one frame), 2 = resident receive table (dimension both, separable phase,
+  // finite weights and delays).
   int variant;
+  // Variant 2 state. Resident: receive table [N][P] of {rx delay, rx
+  // apodization, rx apodization * exp(i wd rx delay)}; per 64-pixel tile the
+  // channel range with nonzero receive weights and the list of waves with a
+  // nonzero transmit weight ([tiles][W], count in tile_nw); the reference
+  // phase per pixel (or null). Per run: interpolation coefficients [F][W][N][L]
+  // of two float4 (cubic only).
+  const float4* rx_table;
+  const int2* tile_ch;
+  const int* tile_waves;
+  const int* tile_nw;
+  const float2* ref_table;
+  float4* coef;
 };
 
 // Builds the transmit table of a plan (one lane per (wave, pixel)).
 hipError_t launch_geom_tx_table(const GeomLaunch& g, float4* table, hipStream_t stream);
+
+// Builds the variant-2 tables; tile_bad[t] = 1 where a nonzero weight or its
+// delay is not finite.
+hipError_t launch_rxtab_tables(const GeomLaunch& g, float4* rx_table, float2* ref_table, int2* tile_ch,
+                               int* tile_waves, int* tile_nw, int* tile_bad, hipStream_t stream);
 
 hipError_t launch_das_geom(const GeomLaunch& g, hipStream_t stream);
 

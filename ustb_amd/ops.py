@@ -100,7 +100,7 @@ class GeometryPlan:
         self.N, self.W, self.L, self.F = geometry.N, geometry.W, geometry.L, geometry.F
         self.dim = geometry.dim
         self.interp = geometry.interp
-        self.separable = bool(geometry.flags & L.FLAG_SEPARABLE_PHASE)
+        self.separable = bool(geometry.flags & L.FLAG_SEPARABLE_PHASE) and not (geometry.flags & L.FLAG_DASC_ORDER)
         self.fd = geometry.fd
         self.P = int(L.lib().ustbx_plan_pixels(self.handle))
         self._keepalive = None  # host arrays were copied into the plan
@@ -115,9 +115,31 @@ class GeometryPlan:
 
     def set_variant(self, variant: int):
         """0: per-lane wave chunks; 1: receive state shared in LDS (dimension
-        both, IQ, separable phase, one frame)."""
+        both, IQ, separable phase, one frame); 2: resident receive table
+        (dimension both, separable or RF phase, finite weights)."""
         st = L.Status()
         L.check(L.lib().ustbx_plan_set_variant(self.handle, int(variant), ctypes.byref(st)), st)
+
+    def has_variant(self, variant: int) -> bool:
+        """Whether the kernel variant applies to this plan (selecting it
+        builds its tables); the current selection is kept."""
+        cur = self.variant
+        st = L.Status()
+        rc = L.lib().ustbx_plan_set_variant(self.handle, int(variant), ctypes.byref(st))
+        if rc == 0:
+            self.set_variant(cur[0])
+            self.set_tiles(cur[1], cur[2])
+            return True
+        if rc != L.E_UNSUPPORTED:
+            L.check(rc, st)
+        return False
+
+    @property
+    def variant(self) -> tuple[int, int, int]:
+        """(variant, waves per chunk or split, split) currently selected."""
+        wc, sp = ctypes.c_int32(0), ctypes.c_int32(0)
+        v = L.lib().ustbx_plan_variant(self.handle, ctypes.byref(wc), ctypes.byref(sp))
+        return int(v), int(wc.value), int(sp.value)
 
     def set_tiles(self, wave_chunk: int = 0, split: int = 0):
         st = L.Status()
