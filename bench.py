@@ -27,7 +27,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-VALU_PEAK_TFLOPS = 157.3     # MI355X published FP32 vector peak
+VALU_PEAK_TFLOPS = 157.3     # MI355X published FP32 vector peak (packed FMA)
+VALU_NONPACKED_TFLOPS = 78.6  # the same without packed FP32 (one FMA per lane per cycle)
 HBM_PEAK_GBS = 8000.0        # MI355X published HBM3E peak
 
 
@@ -42,7 +43,33 @@ def parse():
     ap.add_argument("--phase", default="separable", choices=["separable", "dasc"])
     ap.add_argument("--wave-chunk", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=-1, help="kernel variant (default: the plan's choice)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only time a 1 GiB device-to-device copy (achievable HBM bandwidth; also the "
+                         "known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+COPY_BYTES = 1 << 30
+
+
+def copy_probe(torch, dev, reps=10):
+    """Streaming copy of COPY_BYTES (read once, written once) timed with HIP
+    events: the achievable HBM bandwidth, counting read + write bytes."""
+    src = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return {"bytes_per_copy": 2 * COPY_BYTES, "ms": round(ms, 4), "GBs": round(2 * COPY_BYTES / (ms * 1e-3) / 1e9, 1)}
 
 
 def flops_per_unit(interp: int, iq: bool) -> int:
@@ -75,6 +102,9 @@ def main():
     from ustb_amd.geometry import make_plan
 
     dev = torch.device("cuda", 0)
+    if args.copy_probe:
+        print(json.dumps({"copy_probe": copy_probe(torch, dev)}))
+        return
     t_gen = time.time()
     acq = configs.BUILDERS[args.config]()
     print(f"[bench] {acq.name}: data {acq.data.shape} generated in {time.time() - t_gen:.1f}s", file=sys.stderr,
@@ -82,8 +112,11 @@ def main():
     ch = torch.from_numpy(acq.data).to(dev)
     plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=(args.phase == "separable"),
                      **acq.plan_kwargs())
+    if args.variant >= 0:
+        plan.set_variant(args.variant)
     if args.wave_chunk or args.split:
         plan.set_tiles(args.wave_chunk, args.split)
+    variant, wave_chunk, split = plan.variant
     out = torch.empty(plan.out_shape, dtype=torch.complex64, device=dev)
     stream = torch.cuda.current_stream(dev)
     for _ in range(args.warmup):
@@ -114,8 +147,9 @@ def main():
     out_bytes = int(np.prod(plan.out_shape)) * 8
     alg_bytes = in_bytes + out_bytes
     achieved_gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    copy = copy_probe(torch, dev)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"das_geom_{args.config.lower()}_traffic.txt")
+    tpath = os.path.join(ROOT, "profiles", f"das_both_rxtab_{args.config.lower()}_traffic.txt")
     if os.path.exists(tpath):
         for line in open(tpath):
             if line.startswith("traffic_bytes_per_launch"):
@@ -138,14 +172,19 @@ def main():
                    "waves": len(acq.sequence), "frames": acq.n_frames, "samples": acq.n_samples,
                    "interp": "cubic" if acq.interpolation == 1 else "linear",
                    "dimension": int(acq.dimension), "phase": args.phase,
-                   "geometry": "on-device (delays + apodization per term)"},
+                   "geometry": "on-device, resident per-configuration tables",
+                   "kernel_variant": variant, "waves_per_split": wave_chunk, "splits": split},
         "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": VALU_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / VALU_PEAK_TFLOPS, 4),
                      "traffic": traffic,
                      "flops_per_active_unit": flops_per_unit(int(acq.interpolation), iq),
-                     "active_units": act, "kernel_ms": round(kernel_ms, 4)},
+                     "active_units": act, "kernel_ms": round(kernel_ms, 4),
+                     "peak_nonpacked": VALU_NONPACKED_TFLOPS,
+                     "frac_nonpacked": round(achieved_tflops / VALU_NONPACKED_TFLOPS, 4)},
         "hbm": {"algorithmic_bytes": alg_bytes, "achieved_GBs": round(achieved_gbs, 1),
-                "peak_GBs": HBM_PEAK_GBS, "frac": round(achieved_gbs / HBM_PEAK_GBS, 5)},
+                "peak_GBs": HBM_PEAK_GBS, "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
+                "achievable_copy_GBs": copy["GBs"],
+                "frac_of_achievable": round(achieved_gbs / copy["GBs"], 5)},
     }
 
     if not args.no_check:

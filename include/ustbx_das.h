@@ -250,6 +250,11 @@ int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
  *       runs of one variant-3 plan on different streams must not overlap.
  * USTBX_E_UNSUPPORTED where the variant does not apply. */
 int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st);
+/* Tile visiting order of variants 2 and 3 (no reference counterpart):
+ * 0 = scan order; 1 = depth band by depth band within each XCD's share of the
+ * scan lines (the default; it keeps the sample windows that co-resident
+ * blocks read in L2). Results do not depend on it. */
+int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st);
 /* Current variant (or -1 for NULL) and its tile shape. */
 int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split);
 /* Writes the plan's FP32 delay/apodization matrices (what das.m hands to the

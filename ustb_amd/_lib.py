@@ -103,6 +103,7 @@ EXPORTS = {
     "ustbx_plan_set_tiles": (ctypes.c_int, [c_void_p, c_int32, c_int32, ctypes.POINTER(Status)]),
     "ustbx_plan_set_variant": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
     "ustbx_plan_variant": (c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32)]),
+    "ustbx_plan_set_order": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
     "ustbx_plan_materialize": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                               c_void_p, ctypes.POINTER(Status)]),
     "ustbx_plan_pixels": (c_int64, [c_void_p]),

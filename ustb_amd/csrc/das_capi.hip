@@ -335,9 +335,34 @@ struct ustbx_plan {
   char* rxtab;           // one allocation: receive table, tile data, reference phase
   size_t rxtab_bytes;
   size_t coef_bytes;     // variant 3's coefficient buffer (G.coef)
+  const int* tile_order; // band-major order (G.tile_order when selected)
 };
 
 namespace {
+
+// Tile order of the receive-table kernel. Blocks go to the 8 XCDs round-robin
+// and each XCD takes a contiguous range of tile indices (the XCD remap), i.e.
+// whole scan lines (depth-fastest pixel order) with every depth, so the XCDs'
+// work stays balanced. Within its range an XCD visits the tiles depth band by
+// depth band across its lines: blocks resident together then cover a compact
+// patch of the image, whose delays span a short window of every trace, and
+// those windows stay in the XCD's L2. Identity where the scan has no
+// line-of-depths layout in whole tiles.
+std::vector<int> band_major_order(const DevScan& sc, int tiles) {
+  std::vector<int> order(tiles);
+  for (int t = 0; t < tiles; ++t) order[t] = t;
+  if (!(sc.kind == SCAN_LINEAR || sc.kind == SCAN_SECTOR) || sc.n0 % 64 != 0 || tiles < 8) return order;
+  const int tpl = sc.n0 / 64;  // tiles per line
+  const int q = tiles >> 3, r = tiles & 7;
+  for (int k = 0; k < 8; ++k) {
+    const int s0 = k * q + std::min(k, r);
+    const int cnt = q + (k < r ? 1 : 0);
+    std::stable_sort(order.begin() + s0, order.begin() + s0 + cnt, [tpl](int a, int b) {
+      return a % tpl != b % tpl ? a % tpl < b % tpl : a / tpl < b / tpl;
+    });
+  }
+  return order;
+}
 
 bool rxtab_applies(const ustbx_plan* pl) {
   const GeomLaunch& G = pl->G;
@@ -369,7 +394,8 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   const size_t b_nw = al(sizeof(int) * tiles);
   const size_t b_bad = al(sizeof(int) * tiles);
   const size_t b_ref = G.ref_phase ? al(sizeof(float2) * P) : 0;
-  const size_t total = b_rx + b_ch + b_wl + b_nw + b_bad + b_ref;
+  const size_t b_ord = al(sizeof(int) * tiles);
+  const size_t total = b_rx + b_ch + b_wl + b_nw + b_bad + b_ref + b_ord;
   char* m = nullptr;
   err = hipMalloc(&m, total);
   if (err != hipSuccess) return false;
@@ -379,6 +405,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   int* tnw = (int*)(m + b_rx + b_ch + b_wl);
   int* tbad = (int*)(m + b_rx + b_ch + b_wl + b_nw);
   float2* ref = b_ref ? (float2*)(m + b_rx + b_ch + b_wl + b_nw + b_bad) : nullptr;
+  int* ord = (int*)(m + b_rx + b_ch + b_wl + b_nw + b_bad + b_ref);
   err = launch_rxtab_tables(G, rx, ref, tch, twl, tnw, tbad, nullptr);
   if (err == hipSuccess) err = hipDeviceSynchronize();
   std::vector<int> nw(tiles), bad(tiles);
@@ -394,8 +421,15 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
     (void)hipFree(m);
     return false;
   }
+  const std::vector<int> order = band_major_order(G.scan, (int)tiles);
+  err = hipMemcpy(ord, order.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    (void)hipFree(m);
+    return false;
+  }
   pl->rxtab = m;
   pl->rxtab_bytes = total;
+  pl->tile_order = ord;
   pl->max_nw = max_nw;
   pl->rxtab_state = 1;
   G.rx_table = rx;
@@ -403,6 +437,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   G.tile_waves = twl;
   G.tile_nw = tnw;
   G.ref_table = ref;
+  G.tile_order = ord;
   return true;
 }
 
@@ -852,6 +887,15 @@ int64_t ustbx_plan_pixels(const ustbx_plan* plan) { return plan ? plan->G.P : 0;
 
 int64_t ustbx_plan_device_bytes(const ustbx_plan* plan) {
   return plan ? (int64_t)(plan->dev_bytes + plan->tx_table_bytes + plan->rxtab_bytes + plan->coef_bytes) : 0;
+}
+
+int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (order != 0 && order != 1) return set_status(st, USTBX_E_ARG, "order must be 0 or 1");
+  if (plan->rxtab_state <= 0)
+    return set_status(st, USTBX_E_UNSUPPORTED, "the tile order applies to the receive-table variants");
+  plan->G.tile_order = order ? plan->tile_order : nullptr;
+  return ok(st);
 }
 
 int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split) {

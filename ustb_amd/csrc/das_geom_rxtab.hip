@@ -258,7 +258,8 @@ __global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(GeomLaunch G) {
   __shared__ float2 red[WS][TP];
   const int lane = threadIdx.x & (TP - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);  // wave-uniform
-  const int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (G.tile_order) tile = G.tile_order[tile];
   const int f = blockIdx.y;
   const int p = tile * TP + lane;
   const bool active = p < G.P;

@@ -70,6 +70,9 @@ struct GeomLaunch {
   const int* tile_nw;
   const float2* ref_table;
   float4* coef;
+  // Variant 2/3 tile visiting order: block b takes tile tile_order[x(b)],
+  // x = the XCD-contiguous remap of b; null = x(b) itself.
+  const int* tile_order;
 };
 
 // Builds the transmit table of a plan (one lane per (wave, pixel)).

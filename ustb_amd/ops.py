@@ -141,6 +141,12 @@ class GeometryPlan:
         v = L.lib().ustbx_plan_variant(self.handle, ctypes.byref(wc), ctypes.byref(sp))
         return int(v), int(wc.value), int(sp.value)
 
+    def set_order(self, order: int):
+        """Tile visiting order of variants 2/3: 0 scan order, 1 band-major
+        per XCD (default)."""
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_set_order(self.handle, int(order), ctypes.byref(st)), st)
+
     def set_tiles(self, wave_chunk: int = 0, split: int = 0):
         st = L.Status()
         L.check(L.lib().ustbx_plan_set_tiles(self.handle, wave_chunk, split, ctypes.byref(st)), st)
