@@ -242,9 +242,9 @@ int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
  *   1 = receive state shared in LDS by the wave-splits of a 64-pixel tile
  *       (dimension both, IQ, separable phase, one frame);
  *   2 = resident receive table and per-tile wave/channel lists (dimension
- *       both, separable or RF phase, any frame count, receive table within
- *       8 GiB, finite nonzero weights and delays; the default where it
- *       applies);
+ *       both, separable or RF phase, any frame count, receive table within half
+ *       the device memory, finite nonzero weights and delays; the default
+ *       where it applies);
  *   3 = variant 2 over cubic-interpolation coefficients: the plan holds a
  *       buffer of 32 bytes per channel sample that every run rewrites, so
  *       runs of one variant-3 plan on different streams must not overlap.
@@ -257,6 +257,40 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st);
 int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st);
 /* Current variant (or -1 for NULL) and its tile shape. */
 int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split);
+
+/* ---------------------------------------------------------------------------
+ * 3. Batches: many acquisitions in one persistent launch (no reference
+ *    counterpart; das.m beamforms one acquisition per call).
+ * ------------------------------------------------------------------------ */
+typedef struct ustbx_batch ustbx_batch;
+
+/* count acquisitions, acquisition i beamformed with plans[i] (a plan may be
+ * listed more than once; every plan must have variant-2 state -- see
+ * ustbx_plan_set_variant -- and all share one interpolation). Dimension both;
+ * each image equals ustbx_plan_run of its plan with the same tile shape, bit
+ * for bit. The plans must outlive the batch. */
+int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batch** batch,
+                       ustbx_status* st);
+/* Device buffers of every acquisition: ch[i] [F][W][N][L] complex64, out[i]
+ * [F][P] complex64. Synchronous; a captured graph then uses the new buffers. */
+int ustbx_batch_set_io(ustbx_batch* batch, const void* const* ch, void* const* out,
+                       ustbx_status* st);
+/* Tile shape of the persistent kernel (0 = keep): waves per split (4, 6, 8,
+ * 10, 12, 16) and wave-splits per 64-pixel tile (4 or 8). Drops a captured
+ * graph. */
+int ustbx_batch_set_tiles(ustbx_batch* batch, int32_t wave_chunk, int32_t split, ustbx_status* st);
+/* Enqueues the batch on stream: a counter-reset kernel and the persistent
+ * kernel; allocates nothing and can be captured in a hipGraph. Launches of
+ * one batch must not overlap (they share its claim counters). */
+int ustbx_batch_launch(const ustbx_batch* batch, void* stream, ustbx_status* st);
+/* The same work through a hipGraph captured on the first call and launched on
+ * stream. */
+int ustbx_batch_replay(ustbx_batch* batch, void* stream, ustbx_status* st);
+/* Acquisition count (or -1 for NULL); tile shape, persistent blocks and work
+ * items (tiles x frames). */
+int32_t ustbx_batch_shape(const ustbx_batch* batch, int32_t* wave_chunk, int32_t* split, int32_t* blocks,
+                          int64_t* items);
+void ustbx_batch_destroy(ustbx_batch* batch);
 /* Writes the plan's FP32 delay/apodization matrices (what das.m hands to the
  * MEX) into device buffers: tx [W][P], rx [N][P]; any may be NULL. */
 int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
@@ -274,7 +308,7 @@ int ustbx_hilbert(const float* rf, void* analytic, int32_t L, int64_t traces,
                   void* stream, ustbx_status* st);
 
 /* ---------------------------------------------------------------------------
- * 3. Library information.
+ * 4. Library information.
  * ------------------------------------------------------------------------- */
 int32_t ustbx_abi_version(void);
 /* Number of HIP devices visible (0 when none); never fails. */

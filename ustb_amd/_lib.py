@@ -109,6 +109,16 @@ EXPORTS = {
     "ustbx_plan_pixels": (c_int64, [c_void_p]),
     "ustbx_plan_device_bytes": (c_int64, [c_void_p]),
     "ustbx_plan_destroy": (None, [c_void_p]),
+    "ustbx_batch_create": (ctypes.c_int, [ctypes.POINTER(c_void_p), c_int32, ctypes.POINTER(c_void_p),
+                                          ctypes.POINTER(Status)]),
+    "ustbx_batch_set_io": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                          ctypes.POINTER(Status)]),
+    "ustbx_batch_set_tiles": (ctypes.c_int, [c_void_p, c_int32, c_int32, ctypes.POINTER(Status)]),
+    "ustbx_batch_launch": (ctypes.c_int, [c_void_p, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_batch_replay": (ctypes.c_int, [c_void_p, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_batch_shape": (c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
+                                    ctypes.POINTER(c_int32), ctypes.POINTER(c_int64)]),
+    "ustbx_batch_destroy": (None, [c_void_p]),
     "ustbx_synth_echoes": (ctypes.c_int, [ctypes.POINTER(SynthDesc), c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p]),
     "ustbx_hilbert": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, ctypes.POINTER(Status)]),

@@ -1,0 +1,353 @@
+// das_batch.hip — many acquisitions (different probes, wave counts, scan
+// sizes; dimension both) beamformed by one persistent launch.
+//
+// Work items are (acquisition, 64-pixel tile, frame) of the acquisitions'
+// plans (variant-2 tables). They sit in 8 queues, one per XCD: queue k holds,
+// for every acquisition, the tiles the one-plan kernel would run on XCD k (a
+// balanced share of every depth), in the plan's band-major order, with the
+// acquisitions in decreasing order of work. The grid is the number of blocks
+// the device holds at once; each block claims items from its own XCD's queue
+// with an atomic counter and, when that queue is empty, from the others in
+// turn. Every item runs the same tile body as the one-plan kernel, so each
+// image equals the one-plan launch with the same tile shape bit for bit.
+//
+// A launch is a counter-reset kernel followed by the persistent kernel (no
+// memset, no allocation, no host synchronization), so it can be captured in a
+// hipGraph; the acquisitions' buffers are read from a device descriptor array
+// that set_io rewrites, so a captured graph serves new buffers.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "../../include/ustbx_das.h"
+#include "das_plan.h"
+#include "das_rxtab.h"
+#include "das_status.h"
+
+using namespace ustbx;
+
+namespace {
+
+constexpr int kQueues = 8;
+
+struct BatchLaunch {
+  const RxAcq* acqs;
+  const int4* items;  // {acquisition, tile, frame, 0}
+  const int* qoff;    // [kQueues + 1] queue offsets into items
+  int* ctr;           // [kQueues] claim counters, zeroed before each launch
+};
+
+__global__ void batch_reset_kernel(int* ctr) {
+  if (threadIdx.x < kQueues) ctr[threadIdx.x] = 0;
+}
+
+template <int SM, int WC, int WS>
+__global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
+  __shared__ float2 red[WS][64];
+  __shared__ int claimed;
+  const int home = (int)(blockIdx.x % kQueues);  // blocks reach the XCDs round-robin
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int it = -1;
+      for (int k = 0; k < kQueues && it < 0; ++k) {
+        const int q = (home + k) % kQueues;
+        const int n = B.qoff[q + 1] - B.qoff[q];
+        if (n == 0) continue;
+        const int i = atomicAdd(&B.ctr[q], 1);
+        if (i < n) it = B.qoff[q] + i;
+      }
+      claimed = it;
+    }
+    __syncthreads();
+    // block-uniform: scalar loads of the item and its descriptor
+    const int it = __builtin_amdgcn_readfirstlane(claimed);
+    if (it < 0) break;  // every queue is empty
+    const int4 w = B.items[it];
+    const int acq = __builtin_amdgcn_readfirstlane(w.x);
+    const int tile = __builtin_amdgcn_readfirstlane(w.y);
+    const int frame = __builtin_amdgcn_readfirstlane(w.z);
+    rxtab_tile<SM, WC, WS>(B.acqs[acq], tile, frame, red);
+    __syncthreads();  // all threads are past this item before the next claim
+  }
+}
+
+template <int SM, int WC, int WS>
+hipError_t batch_launch_t(const BatchLaunch& B, int nblocks, hipStream_t stream) {
+  hipLaunchKernelGGL((das_batch_kernel<SM, WC, WS>), dim3(nblocks), dim3(64 * WS), 0, stream, B);
+  return hipGetLastError();
+}
+
+template <int SM, int WC, int WS>
+int batch_occupancy_t() {
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, das_batch_kernel<SM, WC, WS>, 64 * WS, 0) != hipSuccess)
+    return 0;
+  return occ;
+}
+
+// Dispatch over the instantiated (sampling mode, waves per split, splits):
+// f is a functor with a template <int SM, int WC, int WS> call operator.
+template <int SM, int WS, typename F>
+auto dispatch_wc(int wc, const F& f) {
+  switch (wc) {
+    case 4: return f.template operator()<SM, 4, WS>();
+    case 6: return f.template operator()<SM, 6, WS>();
+    case 8: return f.template operator()<SM, 8, WS>();
+    case 10: return f.template operator()<SM, 10, WS>();
+    case 12: return f.template operator()<SM, 12, WS>();
+    default: return f.template operator()<SM, 16, WS>();
+  }
+}
+
+template <typename F>
+auto dispatch(int sm, int wc, int ws, const F& f) {
+  if (sm == SM_LINEAR) return ws == 4 ? dispatch_wc<SM_LINEAR, 4>(wc, f) : dispatch_wc<SM_LINEAR, 8>(wc, f);
+  return ws == 4 ? dispatch_wc<SM_CUBIC, 4>(wc, f) : dispatch_wc<SM_CUBIC, 8>(wc, f);
+}
+
+struct OccupancyFn {
+  template <int SM, int WC, int WS>
+  int operator()() const {
+    return batch_occupancy_t<SM, WC, WS>();
+  }
+};
+
+struct LaunchFn {
+  const BatchLaunch& B;
+  int nblocks;
+  hipStream_t stream;
+  template <int SM, int WC, int WS>
+  hipError_t operator()() const {
+    return batch_launch_t<SM, WC, WS>(B, nblocks, stream);
+  }
+};
+
+bool wc_ok(int wc) { return wc == 4 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
+
+}  // namespace
+
+struct ustbx_batch {
+  int device;
+  int count;
+  int sm;
+  int wc, ws;
+  int nblocks;
+  int64_t n_items;
+  std::vector<const ustbx_plan*> plans;
+  std::vector<RxAcq> h_acqs;
+  char* dev;  // descriptors | items | queue offsets | counters
+  RxAcq* d_acqs;
+  int4* d_items;
+  int* d_qoff;
+  int* d_ctr;
+  hipStream_t capture_stream;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+namespace {
+
+int set_shape(ustbx_batch* b, int wc, int ws, ustbx_status* st) {
+  if (!wc_ok(wc)) return set_status(st, USTBX_E_ARG, "waves per split must be 4, 6, 8, 10, 12 or 16");
+  if (ws != 4 && ws != 8) return set_status(st, USTBX_E_ARG, "wave-splits per block must be 4 or 8");
+  int cus = 0;
+  hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  if (e != hipSuccess) return hip_fail(st, e, "hipDeviceGetAttribute");
+  const int occ = dispatch(b->sm, wc, ws, OccupancyFn{});
+  if (occ <= 0) return set_status(st, USTBX_E_GPU, "the batch kernel does not fit on a compute unit");
+  b->wc = wc;
+  b->ws = ws;
+  b->nblocks = ((cus * occ + kQueues - 1) / kQueues) * kQueues;
+  if (b->exec) {
+    (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    (void)hipGraphDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  return ok(st);
+}
+
+hipError_t launch(const ustbx_batch* b, hipStream_t stream) {
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, stream, b->d_ctr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  BatchLaunch B{b->d_acqs, b->d_items, b->d_qoff, b->d_ctr};
+  return dispatch(b->sm, b->wc, b->ws, LaunchFn{B, b->nblocks, stream});
+}
+
+}  // namespace
+
+extern "C" {
+
+int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batch** batch, ustbx_status* st) {
+  if (!batch) return set_status(st, USTBX_E_ARG, "NULL batch");
+  *batch = nullptr;
+  if (!plans || count <= 0) return set_status(st, USTBX_E_ARG, "a batch needs at least one plan");
+  int device = 0;
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) return hip_fail(st, e, "hipGetDevice");
+  int interp = -1, max_nw = 0;
+  for (int i = 0; i < count; ++i) {
+    const ustbx_plan* pl = plans[i];
+    if (!pl) return set_status(st, USTBX_E_ARG, "NULL plan %d", i);
+    if (pl->device != device) return set_status(st, USTBX_E_ARG, "plan %d belongs to another device", i);
+    if (pl->rxtab_state != 1)
+      return set_status(st, USTBX_E_UNSUPPORTED,
+                        "plan %d has no receive-table state (select variant 2 first; it needs dimension both, "
+                        "the separable or RF phase and finite weights)",
+                        i);
+    if (interp >= 0 && pl->G.interp != interp)
+      return set_status(st, USTBX_E_ARG, "the plans of a batch share one interpolation");
+    interp = pl->G.interp;
+    max_nw = std::max(max_nw, pl->max_nw);
+  }
+  ustbx_batch* b = new (std::nothrow) ustbx_batch();
+  if (!b) return set_status(st, USTBX_E_CAPACITY, "host allocation failed");
+  b->device = device;
+  b->count = count;
+  b->sm = interp == USTBX_INTERP_CUBIC ? SM_CUBIC : SM_LINEAR;
+  b->plans.assign(plans, plans + count);
+
+  // queues: per acquisition (largest work first) the XCD shares of its tiles
+  std::vector<int64_t> work(count, 0);
+  for (int i = 0; i < count; ++i) {
+    const ustbx_plan* pl = plans[i];
+    work[i] = std::accumulate(pl->h_cost.begin(), pl->h_cost.end(), int64_t(0)) * pl->G.F;
+  }
+  std::vector<int> by_work(count);
+  std::iota(by_work.begin(), by_work.end(), 0);
+  std::stable_sort(by_work.begin(), by_work.end(), [&](int a, int c) { return work[a] > work[c]; });
+  std::vector<std::vector<int4>> q(kQueues);
+  for (int i : by_work) {
+    const ustbx_plan* pl = plans[i];
+    const int tiles = (int)pl->h_order.size();
+    const int qq = tiles / kQueues, r = tiles % kQueues;
+    for (int k = 0; k < kQueues; ++k) {
+      const int s0 = k * qq + std::min(k, r), cnt = qq + (k < r ? 1 : 0);
+      for (int f = 0; f < pl->G.F; ++f)
+        for (int x = s0; x < s0 + cnt; ++x) q[k].push_back(make_int4(i, pl->h_order[x], f, 0));
+    }
+  }
+  std::vector<int> qoff(kQueues + 1, 0);
+  for (int k = 0; k < kQueues; ++k) qoff[k + 1] = qoff[k] + (int)q[k].size();
+  b->n_items = qoff[kQueues];
+  std::vector<int4> items;
+  items.reserve(b->n_items);
+  for (int k = 0; k < kQueues; ++k) items.insert(items.end(), q[k].begin(), q[k].end());
+
+  b->h_acqs.resize(count);
+  for (int i = 0; i < count; ++i) {
+    b->h_acqs[i] = rx_acq(plans[i]->G, b->sm);
+    b->h_acqs[i].data = nullptr;
+    b->h_acqs[i].out = nullptr;
+  }
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  const size_t b_acq = al(sizeof(RxAcq) * count), b_items = al(sizeof(int4) * items.size());
+  const size_t b_qoff = al(sizeof(int) * (kQueues + 1)), b_ctr = al(sizeof(int) * kQueues);
+  e = hipMalloc(&b->dev, b_acq + b_items + b_qoff + b_ctr);
+  if (e == hipSuccess) {
+    b->d_acqs = (RxAcq*)b->dev;
+    b->d_items = (int4*)(b->dev + b_acq);
+    b->d_qoff = (int*)(b->dev + b_acq + b_items);
+    b->d_ctr = (int*)(b->dev + b_acq + b_items + b_qoff);
+    e = hipMemcpy(b->d_acqs, b->h_acqs.data(), sizeof(RxAcq) * count, hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_qoff, qoff.data(), sizeof(int) * (kQueues + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreate(&b->capture_stream);  // default priority
+  if (e != hipSuccess) {
+    (void)hipFree(b->dev);
+    delete b;
+    return hip_fail(st, e, "batch setup");
+  }
+  int wc = 0, ws = 0;
+  {
+    ws = 8;
+    const int need = (max_nw + 7) / 8;
+    wc = need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : 16;
+  }
+  const int rc = set_shape(b, wc, ws, st);
+  if (rc != USTBX_OK) {
+    ustbx_batch_destroy(b);
+    return rc;
+  }
+  *batch = b;
+  return ok(st);
+}
+
+int ustbx_batch_set_io(ustbx_batch* b, const void* const* ch, void* const* out, ustbx_status* st) {
+  if (!b) return set_status(st, USTBX_E_ARG, "NULL batch");
+  if (!ch || !out) return set_status(st, USTBX_E_ARG, "NULL buffer arrays");
+  for (int i = 0; i < b->count; ++i) {
+    if (!ch[i] || !out[i]) return set_status(st, USTBX_E_ARG, "NULL buffer for acquisition %d", i);
+    b->h_acqs[i].data = ch[i];
+    b->h_acqs[i].out = reinterpret_cast<float2*>(out[i]);
+  }
+  const hipError_t e = hipMemcpy(b->d_acqs, b->h_acqs.data(), sizeof(RxAcq) * b->count, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(st, e, "batch descriptors");
+  return ok(st);
+}
+
+int ustbx_batch_set_tiles(ustbx_batch* b, int32_t wave_chunk, int32_t split, ustbx_status* st) {
+  if (!b) return set_status(st, USTBX_E_ARG, "NULL batch");
+  return set_shape(b, wave_chunk ? wave_chunk : b->wc, split ? split : b->ws, st);
+}
+
+int ustbx_batch_launch(const ustbx_batch* b, void* stream, ustbx_status* st) {
+  if (!b) return set_status(st, USTBX_E_ARG, "NULL batch");
+  if (!b->h_acqs[0].data) return set_status(st, USTBX_E_ARG, "set the batch's buffers first");
+  const hipError_t e = launch(b, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(st, e, "das_batch_kernel launch");
+  return ok(st);
+}
+
+int ustbx_batch_replay(ustbx_batch* b, void* stream, ustbx_status* st) {
+  if (!b) return set_status(st, USTBX_E_ARG, "NULL batch");
+  if (!b->h_acqs[0].data) return set_status(st, USTBX_E_ARG, "set the batch's buffers first");
+  hipError_t e = hipSuccess;
+  if (!b->exec) {
+    e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(st, e, "hipStreamBeginCapture");
+    const hipError_t el = launch(b, b->capture_stream);
+    e = hipStreamEndCapture(b->capture_stream, &b->graph);
+    if (el != hipSuccess) e = el;
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      if (b->graph) (void)hipGraphDestroy(b->graph);
+      b->graph = nullptr;
+      b->exec = nullptr;
+      return hip_fail(st, e, "batch graph capture");
+    }
+  }
+  e = hipGraphLaunch(b->exec, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(st, e, "hipGraphLaunch");
+  return ok(st);
+}
+
+int32_t ustbx_batch_shape(const ustbx_batch* b, int32_t* wave_chunk, int32_t* split, int32_t* blocks,
+                          int64_t* items) {
+  if (!b) return -1;
+  if (wave_chunk) *wave_chunk = b->wc;
+  if (split) *split = b->ws;
+  if (blocks) *blocks = b->nblocks;
+  if (items) *items = b->n_items;
+  return b->count;
+}
+
+void ustbx_batch_destroy(ustbx_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  (void)hipFree(b->dev);
+  delete b;
+}
+
+}  // extern "C"

@@ -12,33 +12,12 @@
 
 #include "../../include/ustbx_das.h"
 #include "das_launch.h"
+#include "das_plan.h"
+#include "das_status.h"
 
 using namespace ustbx;
 
 namespace {
-
-int set_status(ustbx_status* st, int code, const char* fmt, ...) {
-  if (st) {
-    st->code = code;
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(st->message, sizeof(st->message), fmt, ap);
-    va_end(ap);
-  }
-  return code;
-}
-
-int ok(ustbx_status* st) {
-  if (st) {
-    st->code = USTBX_OK;
-    st->message[0] = 0;
-  }
-  return USTBX_OK;
-}
-
-int hip_fail(ustbx_status* st, hipError_t e, const char* what) {
-  return set_status(st, USTBX_E_GPU, "HIP error: %s in %s", hipGetErrorString(e), what);
-}
 
 // das_c: wd = 2*(float)PI*fd if |fd| > 1e-6 else 0 (das_c.cpp:32-33,240-244)
 float dasc_wd(float fd) {
@@ -64,7 +43,8 @@ bool shared_rx_wc_ok(int wc) { return wc == 8 || wc == 10 || wc == 12 || wc == 1
 
 // The receive-table kernel covers dimension both in the separable (or RF)
 // phase, outside das_c's summation order.
-constexpr size_t kRxTableMaxBytes = size_t(8) << 30;
+// The receive table may take at most this fraction of the device memory.
+constexpr double kRxTableMaxFraction = 0.5;
 
 bool rxtab_wc_ok(int wc) { return wc == 4 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
 
@@ -321,22 +301,6 @@ int ustbx_das_run_host(const ustbx_das_desc* d, const void* ch, const float* tx_
 // ---------------------------------------------------------------------------
 // Geometry plans
 // ---------------------------------------------------------------------------
-struct ustbx_plan {
-  int device;
-  GeomLaunch G;
-  char* dev;
-  size_t dev_bytes;
-  float4* tx_table;      // resident [W][P] transmit table
-  size_t tx_table_bytes;
-  bool dasc_order;
-  // variant 2 (receive table): 0 not built, 1 built, -1 not applicable
-  int rxtab_state;
-  int max_nw;            // largest active-wave count of a tile
-  char* rxtab;           // one allocation: receive table, tile data, reference phase
-  size_t rxtab_bytes;
-  size_t coef_bytes;     // variant 3's coefficient buffer (G.coef)
-  const int* tile_order; // band-major order (G.tile_order when selected)
-};
 
 namespace {
 
@@ -368,7 +332,9 @@ bool rxtab_applies(const ustbx_plan* pl) {
   const GeomLaunch& G = pl->G;
   if (G.dim != 3 || pl->dasc_order || !(G.separable || G.wd == 0.0f)) return false;
   if (G.edge == USTBX_EDGE_PYTHON && G.interp == USTBX_INTERP_LINEAR) return false;
-  if ((size_t)G.N * (size_t)G.P * sizeof(float4) > kRxTableMaxBytes) return false;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+  if ((size_t)G.N * (size_t)G.P * sizeof(float4) > (size_t)(kRxTableMaxFraction * (double)total_b)) return false;
   return (int64_t)G.L * G.N * G.W * 8 <= 0x7FFFFE00;
 }
 
@@ -389,27 +355,36 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   const size_t P = (size_t)G.P, tiles = (P + 63) / 64;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   const size_t b_rx = al(sizeof(float4) * (size_t)G.N * P);
-  const size_t b_ch = al(sizeof(int2) * tiles);
+  const size_t b_nc = al(sizeof(int) * tiles);
+  const size_t b_cl = al(sizeof(int) * tiles * (size_t)G.N);
   const size_t b_wl = al(sizeof(int) * tiles * (size_t)G.W);
   const size_t b_nw = al(sizeof(int) * tiles);
   const size_t b_bad = al(sizeof(int) * tiles);
   const size_t b_ref = G.ref_phase ? al(sizeof(float2) * P) : 0;
   const size_t b_ord = al(sizeof(int) * tiles);
-  const size_t total = b_rx + b_ch + b_wl + b_nw + b_bad + b_ref + b_ord;
+  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + b_ord;
   char* m = nullptr;
   err = hipMalloc(&m, total);
   if (err != hipSuccess) return false;
-  float4* rx = (float4*)m;
-  int2* tch = (int2*)(m + b_rx);
-  int* twl = (int*)(m + b_rx + b_ch);
-  int* tnw = (int*)(m + b_rx + b_ch + b_wl);
-  int* tbad = (int*)(m + b_rx + b_ch + b_wl + b_nw);
-  float2* ref = b_ref ? (float2*)(m + b_rx + b_ch + b_wl + b_nw + b_bad) : nullptr;
-  int* ord = (int*)(m + b_rx + b_ch + b_wl + b_nw + b_bad + b_ref);
-  err = launch_rxtab_tables(G, rx, ref, tch, twl, tnw, tbad, nullptr);
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    char* q = m + off;
+    off += b;
+    return q;
+  };
+  float4* rx = (float4*)take(b_rx);
+  int* tnc = (int*)take(b_nc);
+  int* tcl = (int*)take(b_cl);
+  int* twl = (int*)take(b_wl);
+  int* tnw = (int*)take(b_nw);
+  int* tbad = (int*)take(b_bad);
+  float2* ref = b_ref ? (float2*)take(b_ref) : nullptr;
+  int* ord = (int*)take(b_ord);
+  err = launch_rxtab_tables(G, rx, ref, tnc, tcl, twl, tnw, tbad, nullptr);
   if (err == hipSuccess) err = hipDeviceSynchronize();
-  std::vector<int> nw(tiles), bad(tiles);
+  std::vector<int> nw(tiles), nc(tiles), bad(tiles);
   if (err == hipSuccess) err = hipMemcpy(nw.data(), tnw, sizeof(int) * tiles, hipMemcpyDeviceToHost);
+  if (err == hipSuccess) err = hipMemcpy(nc.data(), tnc, sizeof(int) * tiles, hipMemcpyDeviceToHost);
   if (err == hipSuccess) err = hipMemcpy(bad.data(), tbad, sizeof(int) * tiles, hipMemcpyDeviceToHost);
   bool finite = true;
   int max_nw = 0;
@@ -421,7 +396,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
     (void)hipFree(m);
     return false;
   }
-  const std::vector<int> order = band_major_order(G.scan, (int)tiles);
+  std::vector<int> order = band_major_order(G.scan, (int)tiles);
   err = hipMemcpy(ord, order.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     (void)hipFree(m);
@@ -432,8 +407,12 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   pl->tile_order = ord;
   pl->max_nw = max_nw;
   pl->rxtab_state = 1;
+  pl->h_order = std::move(order);
+  pl->h_cost.resize(tiles);
+  for (size_t t = 0; t < tiles; ++t) pl->h_cost[t] = (int64_t)nw[t] * nc[t];
   G.rx_table = rx;
-  G.tile_ch = tch;
+  G.tile_nc = tnc;
+  G.tile_chl = tcl;
   G.tile_waves = twl;
   G.tile_nw = tnw;
   G.ref_table = ref;
@@ -680,9 +659,8 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   const size_t off_waves = ar.add(waves.data(), sizeof(DevWave) * W);
 
   // ---- upload ----
-  ustbx_plan* pl = new (std::nothrow) ustbx_plan;
+  ustbx_plan* pl = new (std::nothrow) ustbx_plan();  // value-initialized
   if (!pl) return set_status(st, USTBX_E_CAPACITY, "host allocation failed");
-  memset(pl, 0, sizeof(*pl));
   hipError_t e = hipGetDevice(&pl->device);
   if (e != hipSuccess) {
     delete pl;
@@ -799,7 +777,7 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) 
       if (e != hipSuccess) return hip_fail(st, e, "receive table");
       return set_status(st, USTBX_E_UNSUPPORTED,
                         "the receive-table kernel needs dimension both, the separable (or RF) phase, a receive "
-                        "table within 8 GiB and finite nonzero weights and delays");
+                        "table within half the device memory and finite nonzero weights and delays");
     }
     if (variant == 3 && !coef_build(plan, e)) {
       if (e != hipSuccess) return hip_fail(st, e, "coefficient buffer");

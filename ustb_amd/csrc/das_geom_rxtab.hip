@@ -1,39 +1,19 @@
-// das_geom_rxtab.hip — the receive-table kernel (dimension both) and the
-// plan-time tables it reads.
+// das_geom_rxtab.hip — the receive-table kernel of one plan (dimension
+// both) and the plan-time tables it reads (the tile body is in das_rxtab.h).
 //
 // Per plan (constant for a probe configuration):
 //   * receive table [N][P] float4 {rd, ra, ra*cos(wd rd), ra*sin(wd rd)},
 //     the same values the shared-receive kernel computes per tile;
-//   * per 64-pixel tile: the channel range holding every nonzero receive
-//     weight, and the list of transmit waves with a nonzero transmit weight
-//     on some pixel of the tile;
+//   * per 64-pixel tile: the lists of channels and of transmit waves with a
+//     nonzero weight on some pixel of the tile;
 //   * the reference phase per pixel.
-// Per run: for cubic interpolation, the Catmull-Rom (Keys a = -1/2)
+// Per run (variant 3, cubic only): the Catmull-Rom (Keys a = -1/2)
 // polynomial coefficients of every sample interval, so that a term costs a
-// Horner evaluation instead of four weights and four complex products.
-//
-// The main kernel holds no FP64 geometry, so it runs at high occupancy, and
-// its term loop has no data-dependent branch: a term outside the sample
-// window reads through an out-of-range buffer offset, which the hardware
-// returns as zeros. That is exact only when every nonzero weight and delay
-// is finite (das_c skips a zero-weight or out-of-window term before it
-// multiplies, so a NaN weight on a skipped term must not reach the sum);
-// the plan checks this when it builds the tables and otherwise keeps the
-// exact kernels.
-#include "das_geom_impl.h"
+// Horner evaluation instead of four weights and four complex products (at
+// four times the bytes per sample).
+#include "das_rxtab.h"
 
 namespace ustbx {
-
-namespace {
-
-// Delay that puts every term it enters outside the sample window: finite,
-// so that the fraction of the (huge, integral) index stays 0.
-constexpr float kFar = -1.0e20f;
-// Byte offset beyond any buffer the plan accepts (frames are capped at
-// 0x7FFFFE00 bytes), read back as zeros.
-constexpr int kOOB = 0x7FFFFF00;
-
-}  // namespace
 
 // Receive table: one lane per (pixel, channel); grid.y = channel.
 __global__ __launch_bounds__(kBlock) void geom_rx_table_kernel(GeomLaunch G, float4* __restrict__ table) {
@@ -61,11 +41,13 @@ __global__ __launch_bounds__(kBlock) void geom_ref_table_kernel(GeomLaunch G, fl
   ref[p] = make_float2((float)cv, (float)sv);
 }
 
-// One wave64 per 64-pixel tile: active-wave list, channel range, and whether
-// a nonzero weight or its delay is not finite.
+// One wave64 per 64-pixel tile: the lists of waves and channels with a
+// nonzero weight on some pixel of the tile, and whether a nonzero weight or
+// its delay is not finite.
 __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const float4* __restrict__ rxt,
-                                                            int2* __restrict__ tile_ch, int* __restrict__ tile_waves,
-                                                            int* __restrict__ tile_nw, int* __restrict__ tile_bad) {
+                                                            int* __restrict__ tile_nc, int* __restrict__ tile_chl,
+                                                            int* __restrict__ tile_waves, int* __restrict__ tile_nw,
+                                                            int* __restrict__ tile_bad) {
   const int tile = blockIdx.x;
   const int p = tile * 64 + (int)threadIdx.x;
   const bool active = p < G.P;
@@ -81,20 +63,19 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
       ++cnt;
     }
   }
-  int lo = G.N, hi = 0;
+  int nc = 0;
   for (int n = 0; n < G.N; ++n) {
     const float4 v = active ? rxt[(size_t)n * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool on = v.y != 0.0f;
     if (on && !(isfinite(v.x) && isfinite(v.z) && isfinite(v.w))) bad = true;
     if (__any(on)) {
-      lo = min(lo, n);
-      hi = n + 1;
+      if (threadIdx.x == 0) tile_chl[(size_t)tile * G.N + nc] = n;
+      ++nc;
     }
   }
-  if (lo >= hi) lo = hi = 0;
   const bool any_bad = __any(bad);
   if (threadIdx.x == 0) {
-    tile_ch[tile] = make_int2(lo, hi);
+    tile_nc[tile] = nc;
     tile_nw[tile] = cnt;
     tile_bad[tile] = any_bad ? 1 : 0;
   }
@@ -126,203 +107,19 @@ __global__ __launch_bounds__(kBlock) void spline_coef_kernel(const float2* __res
   coef[2 * i + 1] = q1;
 }
 
-// Sampling modes of the term loop:
-//   SM_LINEAR: channel data, one 16-byte gather {x[n0], x[n0+1]};
-//   SM_CUBIC:  channel data, two 16-byte gathers {x[n0-1], x[n0]},
-//              {x[n0+1], x[n0+2]} and Keys' weights (a = -1/2);
-//   SM_COEF:   the Catmull-Rom coefficient buffer, two 16-byte gathers and a
-//              Horner evaluation (fewer operations, four times the bytes).
-enum { SM_LINEAR = 0, SM_CUBIC = 1, SM_COEF = 2 };
-
-template <int SM>
-struct SampleMode {
-  static constexpr int kStride = SM == SM_COEF ? 32 : 8;  // bytes per sample
-  static constexpr int kLoads = SM == SM_LINEAR ? 1 : 2;
-  static constexpr int kLo = SM == SM_CUBIC ? -8 : 0;     // byte offsets of the gathers
-  static constexpr int kHi = SM == SM_CUBIC ? 8 : 16;
-};
-
-// Keys' cubic weights (a = -1/2) in Horner form.
-__device__ __forceinline__ void keys_weights_horner(float b, float& c0, float& c1, float& c2, float& c3) {
-  const float b2 = b * b;
-  c0 = b * fmaf(b, fmaf(-0.5f, b, 1.0f), -0.5f);
-  c1 = fmaf(b2, fmaf(1.5f, b, -2.5f), 1.0f);
-  c2 = b * fmaf(b, fmaf(-1.5f, b, 2.0f), 0.5f);
-  c3 = b2 * fmaf(0.5f, b, -0.5f);
-}
-
-// One channel of one split: part = sum_j T_j * sample_j over the split's
-// waves, in batches of RB terms whose gathers are all issued before the
-// first is used. Slots j >= nj are skipped when GUARD (a wave-uniform branch
-// per batch); without it the slots past the tile's waves hold the far delay
-// and read zeros.
-constexpr int RB = 5;
-
-template <int SM, int WC, bool GUARD>
-__device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
-                                                const float2 (&T)[WC], const int (&wb)[WC], int nj, float rd,
-                                                int nb, float t0, float fs, float lo_ok, float lim) {
-  using M = SampleMode<SM>;
-  float2 part = make_float2(0.0f, 0.0f);
-#pragma unroll
-  for (int j0 = 0; j0 < WC; j0 += RB) {
-    if (GUARD && j0 >= nj) break;
-    float4 q[RB][M::kLoads];
-    float bb[RB];
-#pragma unroll
-    for (int i = 0; i < RB; ++i) {
-      const int j = j0 + i;
-      if (j < WC) {
-        float d;
-        {
-#pragma clang fp contract(off)
-          d = ((td[j] + rd) - t0) * fs;  // das_c.cpp:372,375
-        }
-        const bool ok = d >= lo_ok && d < lim;
-        const int n0 = (int)d;
-        bb[i] = __builtin_amdgcn_fractf(d);
-        const int off = ok ? n0 * M::kStride + (nb + wb[j]) : kOOB;
-        q[i][0] = buf_load16(rs, off + M::kLo);
-        if (M::kLoads == 2) q[i][M::kLoads - 1] = buf_load16(rs, off + M::kHi);
-      }
-    }
-    // keep the batch's gathers ahead of their uses (the scheduler would
-    // otherwise pair each gather with its use to save registers)
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int i = 0; i < RB; ++i) {
-      const int j = j0 + i;
-      if (j < WC) {
-        const float b = bb[i];
-        float2 sm;
-        if (SM == SM_LINEAR) {
-          const float4 x = q[i][0];
-          sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
-        } else if (SM == SM_CUBIC) {
-          const float4 lo = q[i][0], hi = q[i][M::kLoads - 1];
-          float c0, c1, c2, c3;
-          keys_weights_horner(b, c0, c1, c2, c3);
-          sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
-                           fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
-        } else {
-          const float4 q0 = q[i][0], q1 = q[i][M::kLoads - 1];
-          sm = make_float2(fmaf(fmaf(fmaf(q1.z, b, q1.x), b, q0.z), b, q0.x),
-                           fmaf(fmaf(fmaf(q1.w, b, q1.y), b, q0.w), b, q0.y));
-        }
-        cfma(part, T[j], sm);
-      }
-    }
-    // one batch in flight per wave: the batch's sum is complete here (an empty
-    // asm that takes it), so no pass can defer this batch's arithmetic past
-    // the next batch's gathers
-    asm volatile("" : "+v"(part.x), "+v"(part.y));
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  return part;
-}
-
-template <int SM, int WC, bool GUARD>
-__device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
-                                               const float2 (&T)[WC], const int (&wb)[WC], int nj,
-                                               const float4* __restrict__ rxt, size_t PS, int p, bool active,
-                                               int2 cr, int lst, float t0, float fs, float lo_ok, float lim,
-                                               float2& acc) {
-  float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (active) rn = rxt[(size_t)cr.x * PS + p];
-  for (int n = cr.x; n < cr.y; ++n) {
-    const float4 rv = rn;
-    if (active && n + 1 < cr.y) rn = rxt[(size_t)(n + 1) * PS + p];
-    const bool ron = rv.y != 0.0f;
-    const float rd = ron ? rv.x : kFar;
-    const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
-    const float2 part = rxtab_channel<SM, WC, GUARD>(rs, td, T, wb, nj, rd, n * lst, t0, fs, lo_ok, lim);
-    cfma(acc, R, part);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// dimension.both from the resident tables.
-//
-// A block is one tile of 64 pixels (lanes) x WS wave-splits. Split s takes
-// the tile's active waves k = k0 + s + WS*j (j < WC), with their transmit
-// delay and weight T in VGPRs, and runs over the tile's channel range,
-// reading the receive row of the next channel ahead. Per term: the sample
-// index in das_c's operation order, one or two 16-byte gathers, the
-// interpolated sample, then part += T * sample; per channel acc += R * part.
-// The WS partial sums of a pixel are added in split order.
-// ---------------------------------------------------------------------------
+// dimension.both from a plan's resident tables: one block per (tile, frame).
 template <int SM, int WC, int WS>
-__global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(GeomLaunch G) {
-  constexpr int TP = 64;
-  constexpr int ST = SampleMode<SM>::kStride;
-  __shared__ float2 red[WS][TP];
-  const int lane = threadIdx.x & (TP - 1);
-  const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);  // wave-uniform
+__global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(RxAcq A, const int* __restrict__ order) {
+  __shared__ float2 red[WS][64];
   int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (G.tile_order) tile = G.tile_order[tile];
-  const int f = blockIdx.y;
-  const int p = tile * TP + lane;
-  const bool active = p < G.P;
-  const size_t PS = (size_t)G.P;
-  const int2 cr = G.tile_ch[tile];
-  const int nw = G.tile_nw[tile];
-  const int* __restrict__ wl = G.tile_waves + (size_t)tile * G.W;
-  const float4* __restrict__ txt = G.tx_table;
-  const int lst = G.L * ST;           // bytes per trace
-  const int wst = lst * G.N;          // bytes per wave
-  const int frame_bytes = wst * G.W;  // < 0x7FFFFE00, checked by the plan
-  const char* fbase = (SM == SM_COEF ? (const char*)G.coef : (const char*)G.ch) + (size_t)f * frame_bytes;
-  const __amdgpu_buffer_rsrc_t rs = make_rsrc(fbase, frame_bytes);
-  const float lim = (float)(SM == SM_LINEAR ? G.L - 1 : G.L - 2);
-  const float lo_ok = 1.0f;  // das_c's window (the plan excludes the Python edge rule here)
-
-  float2 acc = make_float2(0.0f, 0.0f);
-  for (int k0 = 0; k0 < nw; k0 += WS * WC) {
-    // number of this split's waves in the round (wave-uniform)
-    const int nj = min(WC, max(0, (nw - k0 - s + WS - 1) / WS));
-    if (nj == 0) break;
-    float td[WC];
-    float2 T[WC];
-    int wb[WC];
-#pragma unroll
-    for (int j = 0; j < WC; ++j) {
-      td[j] = kFar;
-      T[j] = make_float2(0.0f, 0.0f);
-      wb[j] = 0;
-      if (j < nj) {
-        const int w = wl[k0 + s + WS * j];
-        wb[j] = w * wst;
-        if (active) {
-          const float4 v = txt[(size_t)w * PS + p];
-          if (v.y != 0.0f) {
-            td[j] = v.x;
-            T[j] = make_float2(v.z, v.w);
-          }
-        }
-      }
-    }
-    if (nj == WC)
-      rxtab_channels<SM, WC, false>(rs, td, T, wb, nj, G.rx_table, PS, p, active, cr, lst, G.t0, G.fs, lo_ok,
-                                        lim, acc);
-    else
-      rxtab_channels<SM, WC, true>(rs, td, T, wb, nj, G.rx_table, PS, p, active, cr, lst, G.t0, G.fs, lo_ok,
-                                       lim, acc);
-  }
-  red[s][lane] = acc;
-  __syncthreads();
-  if (s == 0 && active) {
-    float2 v = red[0][lane];
-#pragma unroll
-    for (int k = 1; k < WS; ++k) cadd(v, red[k][lane]);
-    if (G.ref_phase) v = cmul(v, G.ref_table[p]);
-    reinterpret_cast<float2*>(G.out)[(size_t)f * PS + p] = v;
-  }
+  if (order) tile = order[tile];
+  rxtab_tile<SM, WC, WS>(A, tile, (int)blockIdx.y, red);
 }
 
 template <int SM, int WC, int WS>
 static hipError_t launch_rxtab_t(const GeomLaunch& G, hipStream_t stream) {
   const dim3 grid((unsigned)((G.P + 63) / 64), (unsigned)G.F);
-  hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS>), grid, dim3(64 * WS), 0, stream, G);
+  hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS>), grid, dim3(64 * WS), 0, stream, rx_acq(G, SM), G.tile_order);
   return hipGetLastError();
 }
 
@@ -363,8 +160,8 @@ hipError_t launch_both_rxtab(const GeomLaunch& G, hipStream_t stream) {
   return launch_rxtab_ws<SM_COEF>(G, stream);
 }
 
-hipError_t launch_rxtab_tables(const GeomLaunch& G, float4* rx_table, float2* ref_table, int2* tile_ch,
-                               int* tile_waves, int* tile_nw, int* tile_bad, hipStream_t stream) {
+hipError_t launch_rxtab_tables(const GeomLaunch& G, float4* rx_table, float2* ref_table, int* tile_nc,
+                               int* tile_chl, int* tile_waves, int* tile_nw, int* tile_bad, hipStream_t stream) {
   const dim3 rgrid((unsigned)((G.P + kBlock - 1) / kBlock), (unsigned)G.N);
   hipLaunchKernelGGL(geom_rx_table_kernel, rgrid, dim3(kBlock), 0, stream, G, rx_table);
   hipError_t e = hipGetLastError();
@@ -376,8 +173,8 @@ hipError_t launch_rxtab_tables(const GeomLaunch& G, float4* rx_table, float2* re
     if (e != hipSuccess) return e;
   }
   const unsigned tiles = (unsigned)((G.P + 63) / 64);
-  hipLaunchKernelGGL(geom_tile_scan_kernel, dim3(tiles), dim3(64), 0, stream, G, rx_table, tile_ch, tile_waves,
-                     tile_nw, tile_bad);
+  hipLaunchKernelGGL(geom_tile_scan_kernel, dim3(tiles), dim3(64), 0, stream, G, rx_table, tile_nc, tile_chl,
+                     tile_waves, tile_nw, tile_bad);
   return hipGetLastError();
 }
 

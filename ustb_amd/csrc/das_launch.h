@@ -60,12 +60,13 @@ struct GeomLaunch {
   int variant;
   // Variant 2 state. Resident: receive table [N][P] of {rx delay, rx
   // apodization, rx apodization * exp(i wd rx delay)}; per 64-pixel tile the
-  // channel range with nonzero receive weights and the list of waves with a
-  // nonzero transmit weight ([tiles][W], count in tile_nw); the reference
-  // phase per pixel (or null). Per run: interpolation coefficients [F][W][N][L]
-  // of two float4 (cubic only).
+  // lists of channels ([tiles][N], count in tile_nc) and of waves ([tiles][W],
+  // count in tile_nw) with a nonzero weight on some pixel of the tile; the
+  // reference phase per pixel (or null). Per run (variant 3): interpolation
+  // coefficients [F][W][N][L] of two float4.
   const float4* rx_table;
-  const int2* tile_ch;
+  const int* tile_nc;
+  const int* tile_chl;
   const int* tile_waves;
   const int* tile_nw;
   const float2* ref_table;
@@ -80,8 +81,8 @@ hipError_t launch_geom_tx_table(const GeomLaunch& g, float4* table, hipStream_t 
 
 // Builds the variant-2 tables; tile_bad[t] = 1 where a nonzero weight or its
 // delay is not finite.
-hipError_t launch_rxtab_tables(const GeomLaunch& g, float4* rx_table, float2* ref_table, int2* tile_ch,
-                               int* tile_waves, int* tile_nw, int* tile_bad, hipStream_t stream);
+hipError_t launch_rxtab_tables(const GeomLaunch& g, float4* rx_table, float2* ref_table, int* tile_nc,
+                               int* tile_chl, int* tile_waves, int* tile_nw, int* tile_bad, hipStream_t stream);
 
 hipError_t launch_das_geom(const GeomLaunch& g, hipStream_t stream);
 

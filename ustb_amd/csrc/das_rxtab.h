@@ -1,0 +1,256 @@
+// das_rxtab.h — the tile body of the receive-table kernels (dimension both):
+// shared by the one-plan kernel (das_geom_rxtab.hip) and the persistent
+// batch kernel (das_batch.hip), so both compute a tile with the same
+// operations in the same order.
+//
+// A tile is 64 pixels (lanes) x WS wave-splits (the block's wave64s). Split
+// s takes the tile's active waves k = k0 + s + WS*j (j < WC), holding their
+// transmit delay and weight T in VGPRs, and runs over the tile's channel
+// list, reading the receive row of the next channel ahead. Per term: the
+// sample index in das_c's operation order, one or two 16-byte gathers, the
+// interpolated sample, then part += T * sample; per channel acc += R * part.
+// The WS partial sums of a pixel are added in split order through LDS.
+//
+// The term loop has no data-dependent branch: a term outside the sample
+// window (or with a zero weight, whose delay is replaced by kFar) reads
+// through an out-of-range buffer offset, which the hardware returns as zeros.
+// That is exact only when every nonzero weight and its delay are finite
+// (das_c skips a zero-weight or out-of-window term before it multiplies, so
+// a NaN weight on a skipped term must not reach the sum); plans check this
+// when they build their tables and otherwise keep the exact kernels.
+#pragma once
+#include "das_geom_impl.h"
+
+namespace ustbx {
+
+// Delay that puts every term it enters outside the sample window: finite,
+// so that the fraction of the (huge, integral) index stays 0.
+constexpr float kFar = -1.0e20f;
+// Byte offset beyond any buffer a plan accepts (frames are capped at
+// 0x7FFFFE00 bytes), read back as zeros.
+constexpr int kOOB = 0x7FFFFF00;
+
+// Sampling modes of the term loop:
+//   SM_LINEAR: channel data, one 16-byte gather {x[n0], x[n0+1]};
+//   SM_CUBIC:  channel data, two 16-byte gathers {x[n0-1], x[n0]},
+//              {x[n0+1], x[n0+2]} and Keys' weights (a = -1/2);
+//   SM_COEF:   Catmull-Rom coefficients (32 bytes per sample), two 16-byte
+//              gathers and a Horner evaluation.
+enum { SM_LINEAR = 0, SM_CUBIC = 1, SM_COEF = 2 };
+
+template <int SM>
+struct SampleMode {
+  static constexpr int kStride = SM == SM_COEF ? 32 : 8;  // bytes per sample
+  static constexpr int kLoads = SM == SM_LINEAR ? 1 : 2;
+  static constexpr int kLo = SM == SM_CUBIC ? -8 : 0;     // byte offsets of the gathers
+  static constexpr int kHi = SM == SM_CUBIC ? 8 : 16;
+};
+
+// One acquisition as the tile body sees it (device pointers).
+struct RxAcq {
+  const void* data;        // [F][W][N][L] complex64 (SM_COEF: coefficients)
+  float2* out;             // [F][P]
+  const float4* tx;        // [W][P] {td, ta, ta*cos(wd td), ta*sin(wd td)}
+  const float4* rx;        // [N][P] {rd, ra, ra*cos(wd rd), ra*sin(wd rd)}
+  const int* tile_nc;      // [tiles] channels with a nonzero receive weight
+  const int* tile_chl;     // [tiles][N] their indices, ascending
+  const int* tile_nw;      // [tiles] waves with a nonzero transmit weight
+  const int* tile_waves;   // [tiles][W] their indices, ascending
+  const float2* ref;       // [P] reference phase, or null
+  int P, N, W, L;
+  float t0, fs;
+  int frame_bytes;         // W * N * L * stride (< 0x7FFFFE00)
+  int pad_;
+};
+
+// Keys' cubic weights (a = -1/2) in Horner form.
+__device__ __forceinline__ void keys_weights_horner(float b, float& c0, float& c1, float& c2, float& c3) {
+  const float b2 = b * b;
+  c0 = b * fmaf(b, fmaf(-0.5f, b, 1.0f), -0.5f);
+  c1 = fmaf(b2, fmaf(1.5f, b, -2.5f), 1.0f);
+  c2 = b * fmaf(b, fmaf(-1.5f, b, 2.0f), 0.5f);
+  c3 = b2 * fmaf(0.5f, b, -0.5f);
+}
+
+// One channel of one split: part = sum_j T_j * sample_j over the split's
+// waves, in batches of RB terms whose gathers are all issued before the
+// first is used. Slots j >= nj are skipped when GUARD (a wave-uniform branch
+// per batch); without it the slots past the tile's waves hold the far delay
+// and read zeros.
+constexpr int RB = 5;
+
+template <int SM, int WC, bool GUARD>
+__device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
+                                                const float2 (&T)[WC], const int (&wb)[WC], int nj, float rd,
+                                                int nb, float t0, float fs, float lim) {
+  using M = SampleMode<SM>;
+  constexpr float lo_ok = 1.0f;  // das_c's window: 0 < trunc(d)
+  float2 part = make_float2(0.0f, 0.0f);
+#pragma unroll
+  for (int j0 = 0; j0 < WC; j0 += RB) {
+    if (GUARD && j0 >= nj) break;
+    float4 q[RB][M::kLoads];
+    float bb[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int j = j0 + i;
+      if (j < WC) {
+        float d;
+        {
+#pragma clang fp contract(off)
+          d = ((td[j] + rd) - t0) * fs;  // das_c.cpp:372,375
+        }
+        const bool ok = d >= lo_ok && d < lim;
+        const int n0 = (int)d;
+        bb[i] = __builtin_amdgcn_fractf(d);
+        const int off = ok ? n0 * M::kStride + (nb + wb[j]) : kOOB;
+        q[i][0] = buf_load16(rs, off + M::kLo);
+        if (M::kLoads == 2) q[i][M::kLoads - 1] = buf_load16(rs, off + M::kHi);
+      }
+    }
+    // keep the batch's gathers ahead of their uses (the scheduler would
+    // otherwise pair each gather with its use to save registers)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int j = j0 + i;
+      if (j < WC) {
+        const float b = bb[i];
+        float2 sm;
+        if (SM == SM_LINEAR) {
+          const float4 x = q[i][0];
+          sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
+        } else if (SM == SM_CUBIC) {
+          const float4 lo = q[i][0], hi = q[i][M::kLoads - 1];
+          float c0, c1, c2, c3;
+          keys_weights_horner(b, c0, c1, c2, c3);
+          sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
+                           fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
+        } else {
+          const float4 q0 = q[i][0], q1 = q[i][M::kLoads - 1];
+          sm = make_float2(fmaf(fmaf(fmaf(q1.z, b, q1.x), b, q0.z), b, q0.x),
+                           fmaf(fmaf(fmaf(q1.w, b, q1.y), b, q0.w), b, q0.y));
+        }
+        cfma(part, T[j], sm);
+      }
+    }
+    // one batch in flight per wave: the batch's sum is complete here (an empty
+    // asm that takes it), so no pass can defer this batch's arithmetic past
+    // the next batch's gathers
+    asm volatile("" : "+v"(part.x), "+v"(part.y));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  return part;
+}
+
+template <int SM, int WC, bool GUARD>
+__device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
+                                               const float2 (&T)[WC], const int (&wb)[WC], int nj,
+                                               const float4* __restrict__ rxt, const int* __restrict__ chl,
+                                               int nc, size_t PS, int p, bool active, int lst, float t0, float fs,
+                                               float lim, float2& acc) {
+  float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
+  int n = chl[0];
+  if (active) rn = rxt[(size_t)n * PS + p];
+  for (int k = 0; k < nc; ++k) {
+    const float4 rv = rn;
+    const int nb = n * lst;
+    if (k + 1 < nc) {
+      n = chl[k + 1];
+      if (active) rn = rxt[(size_t)n * PS + p];
+    }
+    const bool ron = rv.y != 0.0f;
+    const float rd = ron ? rv.x : kFar;
+    const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
+    const float2 part = rxtab_channel<SM, WC, GUARD>(rs, td, T, wb, nj, rd, nb, t0, fs, lim);
+    cfma(acc, R, part);
+  }
+}
+
+// One tile of one frame; every thread of the block calls it (it ends with a
+// block barrier and the split-ordered reduction through red).
+template <int SM, int WC, int WS>
+__device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, float2 (*red)[64]) {
+  constexpr int TP = 64;
+  constexpr int ST = SampleMode<SM>::kStride;
+  const int lane = threadIdx.x & (TP - 1);
+  const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);  // wave-uniform
+  const int p = tile * TP + lane;
+  const bool active = p < A.P;
+  const size_t PS = (size_t)A.P;
+  const int nc = A.tile_nc[tile];
+  const int nw = A.tile_nw[tile];
+  const int* __restrict__ wl = A.tile_waves + (size_t)tile * A.W;
+  const int* __restrict__ chl = A.tile_chl + (size_t)tile * A.N;
+  const float4* __restrict__ txt = A.tx;
+  const int lst = A.L * ST;   // bytes per trace
+  const int wst = lst * A.N;  // bytes per wave
+  const __amdgpu_buffer_rsrc_t rs =
+      make_rsrc((const char*)A.data + (size_t)f * A.frame_bytes, A.frame_bytes);
+  const float lim = (float)(SM == SM_LINEAR ? A.L - 1 : A.L - 2);
+
+  float2 acc = make_float2(0.0f, 0.0f);
+  for (int k0 = 0; nc > 0 && k0 < nw; k0 += WS * WC) {
+    // number of this split's waves in the round (wave-uniform)
+    const int nj = min(WC, max(0, (nw - k0 - s + WS - 1) / WS));
+    if (nj == 0) break;
+    float td[WC];
+    float2 T[WC];
+    int wb[WC];
+#pragma unroll
+    for (int j = 0; j < WC; ++j) {
+      td[j] = kFar;
+      T[j] = make_float2(0.0f, 0.0f);
+      wb[j] = 0;
+      if (j < nj) {
+        const int w = wl[k0 + s + WS * j];
+        wb[j] = w * wst;
+        if (active) {
+          const float4 v = txt[(size_t)w * PS + p];
+          if (v.y != 0.0f) {
+            td[j] = v.x;
+            T[j] = make_float2(v.z, v.w);
+          }
+        }
+      }
+    }
+    if (nj == WC)
+      rxtab_channels<SM, WC, false>(rs, td, T, wb, nj, A.rx, chl, nc, PS, p, active, lst, A.t0, A.fs, lim, acc);
+    else
+      rxtab_channels<SM, WC, true>(rs, td, T, wb, nj, A.rx, chl, nc, PS, p, active, lst, A.t0, A.fs, lim, acc);
+  }
+  red[s][lane] = acc;
+  __syncthreads();
+  if (s == 0 && active) {
+    float2 v = red[0][lane];
+#pragma unroll
+    for (int k = 1; k < WS; ++k) cadd(v, red[k][lane]);
+    if (A.ref) v = cmul(v, A.ref[p]);
+    A.out[(size_t)f * PS + p] = v;
+  }
+}
+
+// Descriptor of a plan's variant-2/3 tables for the tile body.
+inline RxAcq rx_acq(const GeomLaunch& G, int sm) {
+  RxAcq A;
+  A.data = sm == SM_COEF ? (const void*)G.coef : G.ch;
+  A.out = reinterpret_cast<float2*>(G.out);
+  A.tx = G.tx_table;
+  A.rx = G.rx_table;
+  A.tile_nc = G.tile_nc;
+  A.tile_chl = G.tile_chl;
+  A.tile_nw = G.tile_nw;
+  A.tile_waves = G.tile_waves;
+  A.ref = G.ref_phase ? G.ref_table : nullptr;
+  A.P = G.P;
+  A.N = G.N;
+  A.W = G.W;
+  A.L = G.L;
+  A.t0 = G.t0;
+  A.fs = G.fs;
+  A.frame_bytes = G.L * G.N * G.W * (sm == SM_COEF ? 32 : 8);
+  A.pad_ = 0;
+  return A;
+}
+
+}  // namespace ustbx
