@@ -35,9 +35,9 @@ HBM_PEAK_GBS = 8000.0        # MI355X published HBM3E peak
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3"])
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; C4: 5)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 3; C4: 1)")
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--phase", default="separable", choices=["separable", "dasc"])
@@ -47,7 +47,12 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a 1 GiB device-to-device copy (achievable HBM bandwidth; also the "
                          "known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 5 if args.config == "C4" else 20
+    if args.warmup is None:
+        args.warmup = 1 if args.config == "C4" else 3
+    return args
 
 
 COPY_BYTES = 1 << 30
@@ -82,12 +87,13 @@ def flops_per_unit(interp: int, iq: bool) -> int:
 
 
 def active_units(plan, torch) -> int:
-    """Number of (pixel, channel, wave) terms with nonzero apodization: the
-    terms das_c evaluates (das_c.cpp:366,371)."""
-    ta, ra, _, _ = plan.materialize()
-    nt = (ta != 0).sum(0, dtype=torch.int64)
-    nr = (ra != 0).sum(0, dtype=torch.int64)
-    return int((nt * nr).sum().item()) * plan.F
+    """Number of (pixel, channel, wave, frame) terms with nonzero apodization:
+    the terms das_c evaluates (das_c.cpp:366,371), counted on the device."""
+    return plan.active_terms()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -104,6 +110,9 @@ def main():
     dev = torch.device("cuda", 0)
     if args.copy_probe:
         print(json.dumps({"copy_probe": copy_probe(torch, dev)}))
+        return
+    if args.config == "C4":
+        run_batch(args, torch, dev)
         return
     t_gen = time.time()
     acq = configs.BUILDERS[args.config]()
@@ -192,6 +201,137 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(acq, plan, torch)
     plan.close()
+    print(json.dumps(result))
+
+
+def run_batch(args, torch, dev):
+    """C4: the ragged batch in one persistent launch (hipGraph replay) against
+    the plain loop of one launch per acquisition on the same buffers."""
+    from ustb_amd import configs
+    from ustb_amd.batch import Batch
+    from ustb_amd.geometry import make_plan
+
+    t = time.time()
+    wl = configs.c4()
+    log(f"{wl.name}: {len(wl.acquisitions)} acquisitions of {len(wl.configs)} configurations, "
+        f"data in {time.time() - t:.1f}s")
+    plans = []
+    for cfg in wl.configs:
+        t = time.time()
+        p = make_plan(cfg.probe, cfg.scan, cfg.sequence, separable_phase=True, **cfg.plan_kwargs())
+        torch.cuda.synchronize()
+        log(f"plan {cfg.name}: variant {p.variant}, {p.device_bytes / 2**30:.2f} GiB, {time.time() - t:.1f}s")
+        plans.append(p)
+    dsets = [[torch.from_numpy(d).to(dev) for d in ds] for ds in wl.datasets]
+    chs = [dsets[k][d] for k, d in wl.acquisitions]
+    bplans = [plans[k] for k, _ in wl.acquisitions]
+    b = Batch(bplans)
+    outs = b.set_io(chs)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, (time.perf_counter() - t0) * 1e3 / reps
+
+    # tile shape: the default (work-weighted) and neighbours, timed on the batch
+    shapes = {}
+    default = (b.shape["wave_chunk"], b.shape["split"])
+    for wc, ws in dict.fromkeys([default, (4, 8), (5, 5), (8, 4), (4, 4)]):
+        b.set_tiles(wc, ws)
+        b.launch()
+        shapes[(wc, ws)] = timed(b.launch, 1)[0]
+        log(f"batch tiles ({wc},{ws}): {shapes[(wc, ws)]:.2f} ms")
+    best = min(shapes, key=shapes.get)
+    b.set_tiles(*best)
+    for _ in range(args.warmup):
+        b.replay()
+    batch_ms, batch_wall = timed(b.replay, args.steps)
+    plain_outs = [torch.empty_like(o) for o in outs]
+
+    def plain():
+        for p, c, o in zip(bplans, chs, plain_outs):
+            p.run(c, o)
+    for _ in range(args.warmup):
+        plain()
+    plain_ms, plain_wall = timed(plain, args.steps)
+    log(f"batch {batch_ms:.2f} ms (wall {batch_wall:.2f}), plain loop {plain_ms:.2f} ms (wall {plain_wall:.2f})")
+
+    # parity: the batch against the plain loop (tile shapes may differ:
+    # FP32 regrouping), and bit for bit against plan runs with its own tiles
+    err = 0.0
+    for o, q in zip(outs, plain_outs):
+        peak = float(q.abs().max().item())
+        err = max(err, float((o - q).abs().max().item()) / max(peak, 1e-30))
+    bitwise = True
+    seen = set()
+    for i, (k, _) in enumerate(wl.acquisitions):
+        if k in seen:
+            continue
+        seen.add(k)
+        p = bplans[i]
+        v = p.variant
+        p.set_tiles(*best)
+        ref = p.run(chs[i])
+        bitwise = bitwise and bool(torch.equal(ref, outs[i]))
+        p.set_tiles(v[1], v[2])
+    units = wl.units
+    act_k = {k: plans[k].active_terms() for k in set(k for k, _ in wl.acquisitions)}
+    act = sum(act_k[k] for k, _ in wl.acquisitions)
+    fl = sum(flops_per_unit(int(wl.configs[k].interpolation), wl.configs[k].fd != 0.0) * act_k[k]
+             for k, _ in wl.acquisitions)
+    achieved = fl / (batch_ms * 1e-3) / 1e12
+    in_bytes = sum(int(c.numel()) * 8 for c in chs)
+    out_bytes = sum(int(o.numel()) * 8 for o in outs)
+    copy = copy_probe(torch, dev)
+    gbs = (in_bytes + out_bytes) / (batch_ms * 1e-3) / 1e9
+    kinds = {}
+    for k, _ in wl.acquisitions:
+        kinds[wl.configs[k].name] = kinds.get(wl.configs[k].name, 0) + 1
+    result = {
+        "metric": "DAS Gsamples/s (pixel x channel x wave)",
+        "value": round(units / (batch_wall * 1e-3) / 1e9, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_wall, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "acquisitions": len(wl.acquisitions), "mix": kinds,
+                   "units_per_batch": units, "kernel_variant": "batch (persistent, hipGraph)",
+                   "waves_per_split": best[0], "splits": best[1], "blocks": b.shape["blocks"],
+                   "work_items": b.shape["items"]},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / VALU_PEAK_TFLOPS, 4), "traffic": None, "active_units": act,
+                     "kernel_ms": round(batch_ms, 3), "peak_nonpacked": VALU_NONPACKED_TFLOPS,
+                     "frac_nonpacked": round(achieved / VALU_NONPACKED_TFLOPS, 4)},
+        "hbm": {"algorithmic_bytes": in_bytes + out_bytes, "achieved_GBs": round(gbs, 1), "peak_GBs": HBM_PEAK_GBS,
+                "frac": round(gbs / HBM_PEAK_GBS, 5), "achievable_copy_GBs": copy["GBs"]},
+        "plain_loop": {"ms": round(plain_wall, 3), "kernel_ms": round(plain_ms, 3),
+                       "Gsamples_s": round(units / (plain_wall * 1e-3) / 1e9, 3),
+                       "batch_speedup": round(plain_wall / batch_wall, 3)},
+        "tile_shapes_ms": {f"{k[0]}x{k[1]}": round(v, 3) for k, v in shapes.items()},
+        "errors": {"batch_vs_plain": err, "batch_bitwise_vs_plan_same_tiles": bitwise},
+    }
+    if not args.no_cpu_baseline:
+        cpwc = [k for k, c in enumerate(wl.configs) if "CPWC" in c.name]
+        cfg = wl.configs[cpwc[0]] if cpwc else wl.configs[0]
+        cfg.data = wl.datasets[wl.configs.index(cfg)][0]
+        result["cpu_baseline"] = cpu_baseline(cfg, plans[wl.configs.index(cfg)], torch)
+        result["cpu_baseline"]["sample"] = f"{cfg.name}: " + result["cpu_baseline"]["sample"]
+    b.close()
+    for p in plans:
+        p.close()
     print(json.dumps(result))
 
 

@@ -234,7 +234,8 @@ int ustbx_plan_run(const ustbx_plan* plan, const void* ch, void* out,
 /* Tile shape override (0 = keep). Variant 0: waves per register chunk (1, 8,
  * 16) and channel groups per pixel tile (1 or 4; dimension both/transmit).
  * Variants 1-3: waves per wave-split (variant 1: 8, 10, 12, 16; variants 2
- * and 3: 4, 6, 8, 10, 12, 16) and wave-splits per 64-pixel tile (4 or 8). */
+ * and 3: 4, 5, 6, 8, 10, 12, 16) and wave-splits per 64-pixel tile (variant
+ * 1: 4 or 8; variants 2 and 3: 4, 5 or 8). */
 int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
                          ustbx_status* st);
 /* Kernel variant (results agree to FP32 rounding; no reference counterpart):
@@ -275,9 +276,10 @@ int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batc
  * [F][P] complex64. Synchronous; a captured graph then uses the new buffers. */
 int ustbx_batch_set_io(ustbx_batch* batch, const void* const* ch, void* const* out,
                        ustbx_status* st);
-/* Tile shape of the persistent kernel (0 = keep): waves per split (4, 6, 8,
- * 10, 12, 16) and wave-splits per 64-pixel tile (4 or 8). Drops a captured
- * graph. */
+/* Tile shape of the persistent kernel (0 = keep): waves per split (4, 5, 6,
+ * 8, 10, 12, 16) and wave-splits per 64-pixel tile (4, 5 or 8). The default
+ * takes 8 splits and covers in one round the wave count below which half of
+ * the batch's work lies. Drops a captured graph. */
 int ustbx_batch_set_tiles(ustbx_batch* batch, int32_t wave_chunk, int32_t split, ustbx_status* st);
 /* Enqueues the batch on stream: a counter-reset kernel and the persistent
  * kernel; allocates nothing and can be captured in a hipGraph. Launches of
@@ -296,6 +298,9 @@ void ustbx_batch_destroy(ustbx_batch* batch);
 int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
                            float* tx_delay, float* rx_delay, void* stream,
                            ustbx_status* st);
+/* Number of (pixel, channel, wave, frame) terms with nonzero transmit and
+ * receive weight: the terms das_c evaluates (das_c.cpp:366,371). Synchronous. */
+int ustbx_plan_active_terms(const ustbx_plan* plan, int64_t* terms, ustbx_status* st);
 int64_t ustbx_plan_pixels(const ustbx_plan* plan);
 int64_t ustbx_plan_device_bytes(const ustbx_plan* plan);
 void ustbx_plan_destroy(ustbx_plan* plan);

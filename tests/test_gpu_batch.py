@@ -80,7 +80,7 @@ def test_batch_equals_plan_runs(cuda, interp):
     _close(plans)
 
 
-@pytest.mark.parametrize("wc,ws", [(4, 4), (6, 8), (10, 8), (16, 4)])
+@pytest.mark.parametrize("wc,ws", [(4, 4), (5, 5), (6, 8), (10, 8), (16, 4)])
 def test_batch_tile_shapes(cuda, wc, ws):
     plans, datas = _mix(Interpolation.linear)
     chs = [torch.from_numpy(d).to(cuda) for d in datas]

@@ -157,7 +157,7 @@ def test_geometry_reference_phase(cuda, name):
 
 @pytest.mark.parametrize("variant,wc,split", [(0, 8, 1), (0, 8, 4), (0, 16, 1), (0, 16, 4), (1, 8, 4),
                                               (1, 10, 4), (1, 12, 8), (1, 16, 8), (2, 4, 4), (2, 6, 8),
-                                              (2, 8, 8), (2, 10, 4), (2, 12, 8), (2, 16, 4)])
+                                              (2, 8, 8), (2, 10, 4), (2, 12, 8), (2, 16, 4), (2, 5, 5)])
 def test_tile_shapes_agree(cuda, variant, wc, split):
     case = CASES["cpwc_tukey"]
     ch = torch.from_numpy(channel_noise(case, seed=13)).to(cuda)

@@ -106,6 +106,7 @@ EXPORTS = {
     "ustbx_plan_set_order": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
     "ustbx_plan_materialize": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                               c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_plan_active_terms": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(Status)]),
     "ustbx_plan_pixels": (c_int64, [c_void_p]),
     "ustbx_plan_device_bytes": (c_int64, [c_void_p]),
     "ustbx_plan_destroy": (None, [c_void_p]),

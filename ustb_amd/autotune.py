@@ -21,7 +21,7 @@ from . import _lib as L
 VARIANT0 = [(0, wc, sp) for wc in (8, 16) for sp in (1, 4)]
 VARIANT0_EXACT = [(0, 8, 1), (0, 8, 4)]
 VARIANT1 = [(1, wc, ws) for ws in (4, 8) for wc in (8, 10, 12, 16)]
-VARIANT2 = [(2, wc, ws) for ws in (4, 8) for wc in (4, 6, 8, 10, 12, 16)]
+VARIANT2 = [(2, wc, ws) for ws in (4, 5, 8) for wc in (4, 5, 6, 8, 10, 12, 16)]
 VARIANT3 = [(3, wc, ws) for ws in (4, 8) for wc in (6, 10, 16)]
 
 _cache: dict = {}

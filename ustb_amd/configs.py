@@ -13,6 +13,22 @@ C3  phased array (64 el, 0.2 mm, f0 2.5 MHz), 31 diverging waves from
     sampled at 10 MHz, 2048 samples, 32 frames with scatterers moving
     0.1 mm/frame; sector scan 256 beams x 1024 depths (1-120 mm);
     dimension transmit.
+C4  ragged batch of 64 acquisitions drawn (seeded) from
+    (a) STA, 128 el linear array: 128 transmits (sources at the elements)
+        x 128 channels x 2048 RF samples at 20.8 MHz (f0 5.2 MHz), beamformed
+        from the analytic signal (das.m's hilbert step, applied when the
+        data is made);
+    (b) CPWC, same array: 11 plane waves -16..16 deg x 128 channels x 1024 IQ
+        samples (10.4 MHz, fd 5.2 MHz);
+    (c) 3-D diverging waves: 32 x 32 matrix array (0.3 mm) x 25 virtual
+        sources on a 5 x 5 grid 10 mm behind it x 512 IQ samples (7.5 MHz,
+        fd 3 MHz), onto a 128 x 128 x 256 voxel volume (x, y +-10 mm,
+        z 5-35 mm);
+    (a) and (b) on square scans of 128^2, 256^2 or 512^2 pixels (x +-19 mm,
+    z 5-50 mm). Speckle plus point targets; rx Tukey-25 f# 1.7; tx none for
+    (a) and (c), Tukey-25 f# 1.7 for (b); linear interpolation; dimension
+    both. Acquisitions of one configuration share its plan; each draws one of
+    a few seeded datasets of its kind.
 
 ``scale`` shrinks pixels/elements/waves/samples for tests.
 """
@@ -25,7 +41,7 @@ import numpy as np
 
 from . import synth
 from .enums import Dimension, Interpolation, Window
-from .uff import Apodization, LinearArray, LinearScan, Probe, Scan, SectorScan, Wave
+from .uff import Apodization, LinearArray, LinearScan, MatrixArray, Probe, Scan, SectorScan, Wave
 
 
 @dataclass
@@ -149,6 +165,115 @@ def c3(seed: int = 3, scale: float = 1.0, with_data: bool = True, n_speckle: int
         acq.data = synth.echoes(prb, seq, spk, rng.standard_normal(S), fs=fs, n_samples=L, f0=fd,
                                 bandwidth=bw, iq=True, n_frames=F, frame_offsets=offs)
     return acq
+
+
+@dataclass
+class BatchWorkload:
+    """Acquisitions of a ragged batch: configs[k] are the distinct probe
+    configurations (an Acquisition without data), datasets[k] the seeded
+    channel data of configuration k, and acquisitions the (configuration,
+    dataset) pair of every acquisition."""
+    name: str
+    configs: list
+    datasets: list
+    acquisitions: list
+
+    @property
+    def units(self) -> int:
+        return sum(self.configs[k].units for k, _ in self.acquisitions)
+
+
+def _c4_sta(P_side: int, scale: float) -> Acquisition:
+    N = max(8, int(round(128 * scale)))
+    n = max(8, int(round(P_side * scale)))
+    L = max(128, int(round(2048 * scale)))
+    c = 1540.0
+    prb = LinearArray(N, 0.3e-3)
+    seq = [Wave.spherical(x, 0.0, 0.0, sound_speed=c, delay=abs(x) / c) for x in prb.x]
+    scan = LinearScan(np.linspace(-19e-3, 19e-3, n), np.linspace(5e-3, 50e-3, n))
+    return Acquisition(f"C4a STA RF {N}x{N}, {n}^2", prb, scan, seq, 20.8e6, 0.0, L, 1,
+                       Apodization(Window.tukey25, f_number=1.7), Apodization(), Dimension.both,
+                       Interpolation.linear, meta={"f0": 5.2e6, "rf": True})
+
+
+def _c4_cpwc(P_side: int, scale: float) -> Acquisition:
+    N = max(8, int(round(128 * scale)))
+    n = max(8, int(round(P_side * scale)))
+    L = max(128, int(round(1024 * scale)))
+    prb = LinearArray(N, 0.3e-3)
+    seq = [Wave.plane(a, sound_speed=1540.0) for a in np.deg2rad(np.linspace(-16.0, 16.0, 11))]
+    scan = LinearScan(np.linspace(-19e-3, 19e-3, n), np.linspace(5e-3, 50e-3, n))
+    return Acquisition(f"C4b CPWC IQ 11 PW, {n}^2", prb, scan, seq, 10.4e6, 5.2e6, L, 1,
+                       Apodization(Window.tukey25, f_number=1.7), Apodization(Window.tukey25, f_number=1.7),
+                       Dimension.both, Interpolation.linear, meta={"f0": 5.2e6})
+
+
+def _c4_3d(scale: float) -> Acquisition:
+    m = max(4, int(round(32 * scale)))
+    nxy = max(8, int(round(128 * scale)))
+    nz = max(16, int(round(256 * scale)))
+    L = max(128, int(round(512 * scale)))
+    c = 1540.0
+    prb = MatrixArray(m, m, 0.3e-3)
+    seq = []
+    for x in np.linspace(-3e-3, 3e-3, 5):
+        for y in np.linspace(-3e-3, 3e-3, 5):
+            seq.append(Wave.spherical(x, y, -10e-3, sound_speed=c, origin=(x, y, -10e-3)))
+    scan = LinearScan(np.linspace(-10e-3, 10e-3, nxy), np.linspace(5e-3, 35e-3, nz), np.linspace(-10e-3, 10e-3, nxy))
+    return Acquisition(f"C4c 3-D DW {m}x{m}, {nxy}x{nxy}x{nz}", prb, scan, seq, 7.5e6, 3e6, L, 1,
+                       Apodization(Window.tukey25, f_number=(1.7, 1.7)), Apodization(), Dimension.both,
+                       Interpolation.linear, meta={"f0": 3e6})
+
+
+def _c4_data(acq: Acquisition, rng, n_speckle: int) -> np.ndarray:
+    """Seeded speckle + 5 point targets in the acquisition's scan volume."""
+    sc = acq.scan
+    x0, x1 = float(sc.x.min()), float(sc.x.max())
+    y0, y1 = float(sc.y.min()), float(sc.y.max())
+    z0, z1 = float(sc.z.min()), float(sc.z.max())
+    S = n_speckle
+    spk = np.stack([rng.uniform(x0, x1, S), rng.uniform(y0, y1, S) if y1 > y0 else np.zeros(S),
+                    rng.uniform(z0, z1, S)], 1)
+    tgt = np.stack([np.linspace(x0, x1, 7)[1:-1], np.zeros(5), np.linspace(z0, z1, 7)[1:-1]], 1)
+    amps = np.concatenate([rng.standard_normal(S), np.full(5, 20.0)])
+    bw = synth.pulse_bandwidth_for_cycles(2.5)
+    f0 = acq.meta["f0"]
+    if acq.meta.get("rf"):
+        rf = synth.echoes(acq.probe, acq.sequence, np.concatenate([spk, tgt]), amps, fs=acq.fs,
+                          n_samples=acq.n_samples, f0=f0, bandwidth=bw, iq=False)
+        return synth.analytic(rf)
+    return synth.echoes(acq.probe, acq.sequence, np.concatenate([spk, tgt]), amps, fs=acq.fs,
+                        n_samples=acq.n_samples, f0=acq.fd, bandwidth=bw, iq=True)
+
+
+def c4(seed: int = 4, scale: float = 1.0, with_data: bool = True, n_acq: int = 64, datasets_per_kind: int = 2,
+       n_speckle: int = 2000) -> BatchWorkload:
+    rng = np.random.default_rng(seed)
+    sides = (128, 256, 512)
+    kinds = rng.integers(0, 3, n_acq)
+    side = rng.integers(0, 3, n_acq)
+    data_idx = rng.integers(0, datasets_per_kind, n_acq)
+    keys, configs = {}, []
+    acqs = []
+    for k, sd, d in zip(kinds, side, data_idx):
+        key = (int(k), int(sd) if k < 2 else 0)
+        if key not in keys:
+            keys[key] = len(configs)
+            configs.append(_c4_sta(sides[key[1]], scale) if k == 0 else
+                           _c4_cpwc(sides[key[1]], scale) if k == 1 else _c4_3d(scale))
+        acqs.append((keys[key], int(d)))
+    datasets = [[] for _ in configs]
+    if with_data:
+        # one dataset family per kind: configurations of one kind share data
+        # (their probes, sequences and samples agree; only the scan differs)
+        by_kind = {}
+        for (k, _), ci in keys.items():
+            if k not in by_kind:
+                by_kind[k] = [_c4_data(configs[ci], np.random.default_rng(seed * 100 + 10 * k + i),
+                                       max(50, int(round(n_speckle * scale)))) for i in range(datasets_per_kind)]
+            datasets[ci] = by_kind[k]
+    return BatchWorkload(f"C4 ragged batch of {n_acq} (STA RF / CPWC IQ / 3-D DW), linear, both",
+                         configs, datasets, acqs)
 
 
 BUILDERS = {"C1": c1, "C2": c2, "C3": c3}

@@ -41,6 +41,18 @@ struct BatchLaunch {
   int* ctr;           // [kQueues] claim counters, zeroed before each launch
 };
 
+// A descriptor copied word by word through the constant address space, so
+// the compiler reads it with scalar loads and keeps it in SGPRs.
+__device__ __forceinline__ RxAcq load_constant(const RxAcq* p) {
+  static_assert(sizeof(RxAcq) % 4 == 0, "RxAcq is a whole number of words");
+  RxAcq a;
+  const __attribute__((address_space(4))) uint32_t* src = (const __attribute__((address_space(4))) uint32_t*)p;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&a);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(RxAcq) / 4); ++i) dst[i] = src[i];
+  return a;
+}
+
 __global__ void batch_reset_kernel(int* ctr) {
   if (threadIdx.x < kQueues) ctr[threadIdx.x] = 0;
 }
@@ -63,14 +75,14 @@ __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
       claimed = it;
     }
     __syncthreads();
-    // block-uniform: scalar loads of the item and its descriptor
+    // block-uniform: the item and its descriptor are read through the
+    // constant address space (scalar loads into SGPRs; both are read-only
+    // while the kernel runs)
     const int it = __builtin_amdgcn_readfirstlane(claimed);
     if (it < 0) break;  // every queue is empty
-    const int4 w = B.items[it];
-    const int acq = __builtin_amdgcn_readfirstlane(w.x);
-    const int tile = __builtin_amdgcn_readfirstlane(w.y);
-    const int frame = __builtin_amdgcn_readfirstlane(w.z);
-    rxtab_tile<SM, WC, WS>(B.acqs[acq], tile, frame, red);
+    const __attribute__((address_space(4))) int* w = (const __attribute__((address_space(4))) int*)B.items + 4 * it;
+    const int acq = w[0], tile = w[1], frame = w[2];
+    rxtab_tile<SM, WC, WS>(load_constant(B.acqs + acq), tile, frame, red);
     __syncthreads();  // all threads are past this item before the next claim
   }
 }
@@ -95,6 +107,7 @@ template <int SM, int WS, typename F>
 auto dispatch_wc(int wc, const F& f) {
   switch (wc) {
     case 4: return f.template operator()<SM, 4, WS>();
+    case 5: return f.template operator()<SM, 5, WS>();
     case 6: return f.template operator()<SM, 6, WS>();
     case 8: return f.template operator()<SM, 8, WS>();
     case 10: return f.template operator()<SM, 10, WS>();
@@ -103,10 +116,16 @@ auto dispatch_wc(int wc, const F& f) {
   }
 }
 
+template <int SM, typename F>
+auto dispatch_ws(int wc, int ws, const F& f) {
+  if (ws == 4) return dispatch_wc<SM, 4>(wc, f);
+  if (ws == 5) return dispatch_wc<SM, 5>(wc, f);
+  return dispatch_wc<SM, 8>(wc, f);
+}
+
 template <typename F>
 auto dispatch(int sm, int wc, int ws, const F& f) {
-  if (sm == SM_LINEAR) return ws == 4 ? dispatch_wc<SM_LINEAR, 4>(wc, f) : dispatch_wc<SM_LINEAR, 8>(wc, f);
-  return ws == 4 ? dispatch_wc<SM_CUBIC, 4>(wc, f) : dispatch_wc<SM_CUBIC, 8>(wc, f);
+  return sm == SM_LINEAR ? dispatch_ws<SM_LINEAR>(wc, ws, f) : dispatch_ws<SM_CUBIC>(wc, ws, f);
 }
 
 struct OccupancyFn {
@@ -126,7 +145,7 @@ struct LaunchFn {
   }
 };
 
-bool wc_ok(int wc) { return wc == 4 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
+bool wc_ok(int wc) { return wc == 4 || wc == 5 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
 
 }  // namespace
 
@@ -152,8 +171,8 @@ struct ustbx_batch {
 namespace {
 
 int set_shape(ustbx_batch* b, int wc, int ws, ustbx_status* st) {
-  if (!wc_ok(wc)) return set_status(st, USTBX_E_ARG, "waves per split must be 4, 6, 8, 10, 12 or 16");
-  if (ws != 4 && ws != 8) return set_status(st, USTBX_E_ARG, "wave-splits per block must be 4 or 8");
+  if (!wc_ok(wc)) return set_status(st, USTBX_E_ARG, "waves per split must be 4, 5, 6, 8, 10, 12 or 16");
+  if (ws != 4 && ws != 5 && ws != 8) return set_status(st, USTBX_E_ARG, "wave-splits per block must be 4, 5 or 8");
   int cus = 0;
   hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
   if (e != hipSuccess) return hip_fail(st, e, "hipDeviceGetAttribute");
@@ -267,12 +286,25 @@ int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batc
     delete b;
     return hip_fail(st, e, "batch setup");
   }
-  int wc = 0, ws = 0;
+  // default tile shape: 8 splits, waves per split covering (in one round)
+  // the wave count below which half of the batch's work lies
+  int nw_med = max_nw;
   {
-    ws = 8;
-    const int need = (max_nw + 7) / 8;
-    wc = need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : 16;
+    std::vector<int> by_nw(count);
+    std::iota(by_nw.begin(), by_nw.end(), 0);
+    std::stable_sort(by_nw.begin(), by_nw.end(), [&](int a, int c) { return plans[a]->max_nw < plans[c]->max_nw; });
+    const int64_t total = std::accumulate(work.begin(), work.end(), int64_t(0));
+    int64_t acc = 0;
+    for (int i : by_nw) {
+      acc += work[i];
+      if (2 * acc >= total) {
+        nw_med = plans[i]->max_nw;
+        break;
+      }
+    }
   }
+  const int ws = 8, need = (std::max(nw_med, 1) + 7) / 8;
+  const int wc = need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : 16;
   const int rc = set_shape(b, wc, ws, st);
   if (rc != USTBX_OK) {
     ustbx_batch_destroy(b);

@@ -46,7 +46,7 @@ bool shared_rx_wc_ok(int wc) { return wc == 8 || wc == 10 || wc == 12 || wc == 1
 // The receive table may take at most this fraction of the device memory.
 constexpr double kRxTableMaxFraction = 0.5;
 
-bool rxtab_wc_ok(int wc) { return wc == 4 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
+bool rxtab_wc_ok(int wc) { return wc == 4 || wc == 5 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
 
 // 8 wave-splits; the smallest waves-per-split covering the busiest tile in
 // one round (else 16 per round).
@@ -805,9 +805,9 @@ int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split, us
   if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
   if (plan->G.variant >= 2) {
     if (wave_chunk && !rxtab_wc_ok(wave_chunk))
-      return set_status(st, USTBX_E_ARG, "waves per split must be 4, 6, 8, 10, 12 or 16");
-    if (split && split != 4 && split != 8)
-      return set_status(st, USTBX_E_ARG, "wave-splits per block must be 4 or 8");
+      return set_status(st, USTBX_E_ARG, "waves per split must be 4, 5, 6, 8, 10, 12 or 16");
+    if (split && split != 4 && split != 5 && split != 8)
+      return set_status(st, USTBX_E_ARG, "wave-splits per block must be 4, 5 or 8");
     if (wave_chunk) plan->G.wave_chunk = wave_chunk;
     if (split) plan->G.split = split;
     return ok(st);
@@ -858,6 +858,21 @@ int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
   hipError_t e = launch_geom_materialize(plan->G, tx_apo, rx_apo, tx_delay, rx_delay,
                                          reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(st, e, "geom_materialize_kernel launch");
+  return ok(st);
+}
+
+int ustbx_plan_active_terms(const ustbx_plan* plan, int64_t* terms, ustbx_status* st) {
+  if (!plan || !terms) return set_status(st, USTBX_E_ARG, "NULL plan or result");
+  unsigned long long* d = nullptr;
+  hipError_t e = hipMalloc(&d, sizeof(*d));
+  if (e != hipSuccess) return hip_fail(st, e, "hipMalloc (count)");
+  unsigned long long h = 0;
+  e = hipMemcpy(d, &h, sizeof(h), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_count_active(plan->G, d, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(st, e, "geom_count_active_kernel");
+  *terms = (int64_t)h * plan->G.F;
   return ok(st);
 }
 

@@ -63,6 +63,38 @@ hipError_t launch_geom_tx_table(const GeomLaunch& G, float4* table, hipStream_t 
   return hipGetLastError();
 }
 
+// Terms with nonzero transmit and receive weight (the terms das_c evaluates,
+// das_c.cpp:366,371): per pixel (#w: ta != 0) * (#n: ra != 0), summed per
+// block in LDS, then one 64-bit atomic per block.
+__global__ __launch_bounds__(kBlock) void geom_count_active_kernel(GeomLaunch G,
+                                                                   unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long part[kBlock];
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long v = 0;
+  if (p < G.P) {
+    double px, py, pz, rref;
+    pixel_xyz(G.scan, p, px, py, pz, rref);
+    unsigned long long nt = 0, nr = 0;
+    for (int w = 0; w < G.W; ++w) nt += tx_apod_f32(G.waves[w], G.tx, px, py, pz, w, p, G.P) != 0.0f;
+    for (int n = 0; n < G.N; ++n)
+      nr += rx_apod_f32(G.rx, px, py, pz, G.ex[n], G.ey[n], G.ez[n], n, p, G.P) != 0.0f;
+    v = nt * nr;
+  }
+  part[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && part[0]) atomicAdd(total, part[0]);
+}
+
+hipError_t launch_count_active(const GeomLaunch& G, unsigned long long* total, hipStream_t stream) {
+  const unsigned grid = (unsigned)((G.P + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(geom_count_active_kernel, dim3(grid), dim3(kBlock), 0, stream, G, total);
+  return hipGetLastError();
+}
+
 hipError_t launch_geom_materialize(const GeomLaunch& G, float* tx_apo, float* rx_apo,
                                    float* tx_delay, float* rx_delay, hipStream_t stream) {
   const unsigned grid = (unsigned)((G.P + kBlock - 1) / kBlock);

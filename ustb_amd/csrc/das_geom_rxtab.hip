@@ -112,7 +112,7 @@ template <int SM, int WC, int WS>
 __global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(RxAcq A, const int* __restrict__ order) {
   __shared__ float2 red[WS][64];
   int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (order) tile = order[tile];
+  if (order) tile = __builtin_amdgcn_readfirstlane(order[tile]);  // block-uniform
   rxtab_tile<SM, WC, WS>(A, tile, (int)blockIdx.y, red);
 }
 
@@ -127,6 +127,7 @@ template <int SM, int WS>
 static hipError_t launch_rxtab_wc(const GeomLaunch& G, hipStream_t stream) {
   switch (G.wave_chunk) {
     case 4: return launch_rxtab_t<SM, 4, WS>(G, stream);
+    case 5: return launch_rxtab_t<SM, 5, WS>(G, stream);
     case 6: return launch_rxtab_t<SM, 6, WS>(G, stream);
     case 8: return launch_rxtab_t<SM, 8, WS>(G, stream);
     case 10: return launch_rxtab_t<SM, 10, WS>(G, stream);
@@ -140,6 +141,7 @@ template <int SM>
 static hipError_t launch_rxtab_ws(const GeomLaunch& G, hipStream_t stream) {
   switch (G.split) {
     case 4: return launch_rxtab_wc<SM, 4>(G, stream);
+    case 5: return launch_rxtab_wc<SM, 5>(G, stream);
     case 8: return launch_rxtab_wc<SM, 8>(G, stream);
     default: return hipErrorInvalidValue;
   }

@@ -86,6 +86,10 @@ hipError_t launch_rxtab_tables(const GeomLaunch& g, float4* rx_table, float2* re
 
 hipError_t launch_das_geom(const GeomLaunch& g, hipStream_t stream);
 
+// Adds the number of terms with nonzero transmit and receive weight (per
+// frame) to *total (device).
+hipError_t launch_count_active(const GeomLaunch& g, unsigned long long* total, hipStream_t stream);
+
 // Writes the geometry's FP32 matrices (das.m's inputs to the MEX).
 hipError_t launch_geom_materialize(const GeomLaunch& g, float* tx_apo,
                                    float* rx_apo, float* tx_delay,

@@ -149,15 +149,19 @@ __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const 
                                                const float4* __restrict__ rxt, const int* __restrict__ chl,
                                                int nc, size_t PS, int p, bool active, int lst, float t0, float fs,
                                                float lim, float2& acc) {
+  // receive row of the next channel in flight during this one; its index
+  // one further ahead (a scalar load off the critical path)
   float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
-  int n = chl[0];
+  int n = __builtin_amdgcn_readfirstlane(chl[0]);
+  int n_next = nc > 1 ? __builtin_amdgcn_readfirstlane(chl[1]) : 0;
   if (active) rn = rxt[(size_t)n * PS + p];
   for (int k = 0; k < nc; ++k) {
     const float4 rv = rn;
     const int nb = n * lst;
     if (k + 1 < nc) {
-      n = chl[k + 1];
+      n = n_next;
       if (active) rn = rxt[(size_t)n * PS + p];
+      if (k + 2 < nc) n_next = __builtin_amdgcn_readfirstlane(chl[k + 2]);
     }
     const bool ron = rv.y != 0.0f;
     const float rd = ron ? rv.x : kFar;
@@ -178,8 +182,8 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
   const int p = tile * TP + lane;
   const bool active = p < A.P;
   const size_t PS = (size_t)A.P;
-  const int nc = A.tile_nc[tile];
-  const int nw = A.tile_nw[tile];
+  const int nc = __builtin_amdgcn_readfirstlane(A.tile_nc[tile]);
+  const int nw = __builtin_amdgcn_readfirstlane(A.tile_nw[tile]);
   const int* __restrict__ wl = A.tile_waves + (size_t)tile * A.W;
   const int* __restrict__ chl = A.tile_chl + (size_t)tile * A.N;
   const float4* __restrict__ txt = A.tx;
@@ -203,7 +207,7 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
       T[j] = make_float2(0.0f, 0.0f);
       wb[j] = 0;
       if (j < nj) {
-        const int w = wl[k0 + s + WS * j];
+        const int w = __builtin_amdgcn_readfirstlane(wl[k0 + s + WS * j]);
         wb[j] = w * wst;
         if (active) {
           const float4 v = txt[(size_t)w * PS + p];

@@ -166,6 +166,13 @@ class GeometryPlan:
                                        ctypes.byref(st)), st)
         return out
 
+    def active_terms(self) -> int:
+        """(pixel, channel, wave, frame) terms with nonzero tx and rx weight."""
+        v = ctypes.c_int64(0)
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_active_terms(self.handle, ctypes.byref(v), ctypes.byref(st)), st)
+        return int(v.value)
+
     def materialize(self):
         """FP32 (tx_apo, rx_apo, tx_delay, rx_delay) as das.m hands them to the MEX."""
         kw = dict(dtype=torch.float32, device=self.device)
