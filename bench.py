@@ -37,7 +37,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; C4: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 3; C4: 1)")
-    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4"])
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4", "C5"])
+    ap.add_argument("--frames", type=int, default=600, help="C5: requests served (the trace's first ones)")
+    ap.add_argument("--rate", type=float, default=2000.0, help="C5: Poisson arrival rate, frames/s")
+    ap.add_argument("--slice-ms", type=float, default=2.0, help="C5: estimated work per serving iteration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--phase", default="separable", choices=["separable", "dasc"])
@@ -113,6 +116,9 @@ def main():
         return
     if args.config == "C4":
         run_batch(args, torch, dev)
+        return
+    if args.config == "C5":
+        run_serve(args, torch, dev)
         return
     t_gen = time.time()
     acq = configs.BUILDERS[args.config]()
@@ -330,6 +336,73 @@ def run_batch(args, torch, dev):
         result["cpu_baseline"] = cpu_baseline(cfg, plans[wl.configs.index(cfg)], torch)
         result["cpu_baseline"]["sample"] = f"{cfg.name}: " + result["cpu_baseline"]["sample"]
     b.close()
+    for p in plans:
+        p.close()
+    print(json.dumps(result))
+
+
+def run_serve(args, torch, dev):
+    """C5: the continuous-batching server against the plain loop (one request
+    at a time, one launch per frame) on the same Poisson trace."""
+    from ustb_amd import configs
+    from ustb_amd.geometry import make_plan
+    from ustb_amd.serve import Server, poisson_trace
+
+    t = time.time()
+    cfgs, datasets = configs.c5()
+    plans = [make_plan(c.probe, c.scan, c.sequence, separable_phase=True, **c.plan_kwargs()) for c in cfgs]
+    torch.cuda.synchronize()
+    log(f"C5: {len(cfgs)} configurations, data and plans in {time.time() - t:.1f}s")
+    srv = Server(plans, datasets, hbm_budget=64 << 30, slice_ms=args.slice_ms, tiles=(8, 4))
+    log(f"calibrated ms per position {[round(x, 4) for x in srv.ms_per_pos]}, positions {srv.positions}")
+    names = [c.name for c in cfgs]
+    trace = poisson_trace(args.frames, rate=args.rate, n_configs=len(cfgs), n_data=len(datasets[0]), seed=5)
+    warm = poisson_trace(min(30, args.frames), rate=args.rate, n_configs=len(cfgs), n_data=len(datasets[0]),
+                         seed=6)
+    srv.run(warm)
+    srv.plain_loop(warm)
+    res = srv.run(trace)
+    log(f"served {res.summary(names)}")
+    plain = srv.plain_loop(trace)
+    log(f"plain {plain.summary(names)}")
+    # parity: a few requests alone (plan runs with the server's tiles) and in the mix
+    chk = poisson_trace(12, rate=args.rate, n_configs=len(cfgs), n_data=len(datasets[0]), seed=7)
+    mixed = srv.run(chk, keep_outputs=True)
+    plain_chk = srv.plain_loop(chk, keep_outputs=True)
+    same, err = True, 0.0
+    for r in chk:
+        p = plans[r.config]
+        v = p.variant
+        p.set_tiles(*srv.tiles)
+        alone = p.run(torch.from_numpy(datasets[r.config][r.data]).to(dev)).cpu().numpy()
+        p.set_tiles(v[1], v[2])
+        same = same and bool(np.array_equal(alone, mixed.outputs[r.rid]))
+        q = plain_chk.outputs[r.rid]
+        err = max(err, float(np.max(np.abs(q - alone)) / max(np.max(np.abs(alone)), 1e-30)))
+    units = sum(cfgs[r.config].units for r in trace)
+    result = {
+        "metric": "DAS Gsamples/s (pixel x channel x wave)",
+        "value": round(units / res.elapsed_s / 1e9, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": len(trace),
+        "warmup": len(warm),
+        "ms_per_step": round(res.elapsed_s * 1e3 / len(trace), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"C5 serving, Poisson {args.rate:g} frames/s, first {len(trace)} requests of the "
+                               f"trace, uniform over {names}", "hbm_budget_GiB": 64, "slice_ms": args.slice_ms,
+                   "waves_per_split": srv.tiles[0], "splits": srv.tiles[1],
+                   "units_per_frame": {n: c.units for n, c in zip(names, cfgs)}},
+        "serving": res.summary(names),
+        "plain_loop": plain.summary(names),
+        "serving_speedup_frames_per_s": round(res.frames_per_s / plain.frames_per_s, 3),
+        "errors": {"served_equals_alone_bitwise": same, "plain_vs_alone": err},
+    }
+    srv.close()
     for p in plans:
         p.close()
     print(json.dumps(result))

@@ -293,6 +293,30 @@ int ustbx_batch_replay(ustbx_batch* batch, void* stream, ustbx_status* st);
 int32_t ustbx_batch_shape(const ustbx_batch* batch, int32_t* wave_chunk, int32_t* split, int32_t* blocks,
                           int64_t* items);
 void ustbx_batch_destroy(ustbx_batch* batch);
+
+/* Batcher: a different set of acquisitions in every launch (serving loops
+ * that batch the frames that are ready). An acquisition's tiles are cut into
+ * 8 shares (one per XCD) of ustbx_plan_positions(plan) positions each; a
+ * submission computes positions [pos_begin[i], pos_end[i]) of every share of
+ * acquisition i, so a large frame can be spread over several submissions and
+ * is complete once all positions have run. Tiles are computed whole, so the
+ * images equal ustbx_plan_run with the same tile shape bit for bit. */
+typedef struct ustbx_batcher ustbx_batcher;
+
+/* Positions per share of a plan with variant-2 state (-1 otherwise). */
+int32_t ustbx_plan_positions(const ustbx_plan* plan);
+/* interp: the plans' interpolation; capacities: acquisitions and work items
+ * (tiles x frames) per submission; tile shape as ustbx_batch_set_tiles. */
+int ustbx_batcher_create(int32_t interp, int32_t max_acqs, int64_t max_items, int32_t wave_chunk, int32_t split,
+                         ustbx_batcher** batcher, ustbx_status* st);
+/* Uploads the submission's descriptors and work items (asynchronously, from
+ * pinned staging memory; it first waits until the previous submission's
+ * upload has left the staging memory) and launches the persistent kernel on
+ * stream. Allocates nothing. Submissions run in stream order. */
+int ustbx_batcher_submit(ustbx_batcher* batcher, int32_t count, const ustbx_plan* const* plans,
+                         const void* const* ch, void* const* out, const int32_t* pos_begin,
+                         const int32_t* pos_end, void* stream, ustbx_status* st);
+void ustbx_batcher_destroy(ustbx_batcher* batcher);
 /* Writes the plan's FP32 delay/apodization matrices (what das.m hands to the
  * MEX) into device buffers: tx [W][P], rx [N][P]; any may be NULL. */
 int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,

@@ -1,0 +1,69 @@
+"""The serving loop (continuous batching): every request's image is the same
+alone and in the mix. Requests arrive faster than one at a time can be
+served, large frames are sliced over several iterations, and a small HBM
+budget forces admission to wait; each served image must equal its plan's own
+run with the batcher's tile shape bit for bit, and the plain one-launch loop
+must agree within 1e-5 of the peak magnitude."""
+import numpy as np
+import pytest
+import torch
+
+from tests.cases import channel_noise, geom_cases
+from ustb_amd.geometry import make_plan
+from ustb_amd.serve import Server, poisson_trace
+
+pytestmark = pytest.mark.gpu
+CASES = geom_cases()
+NAMES = ["cpwc_tukey", "dw_sector", "matrix3d_dw"]
+
+
+def _setup():
+    plans, datas = [], []
+    for i, name in enumerate(NAMES):
+        case = CASES[name]
+        plan = make_plan(case["probe"], case["scan"], case["seq"], n_samples=case["L"], n_frames=1,
+                         sampling_frequency=case["fs"], initial_time=0.0, modulation_frequency=case["fd"],
+                         sound_speed=1540.0, receive_apodization=case["rx"], transmit_apodization=case["tx"],
+                         dimension=3, tx_model=case["model"], separable_phase=True)
+        assert plan.has_variant(2), name
+        plans.append(plan)
+        datas.append([channel_noise(case, seed=300 + 10 * i + j) for j in range(2)])
+    return plans, datas
+
+
+def _alone(plans, datas, tiles):
+    ref = {}
+    for k, p in enumerate(plans):
+        p.set_variant(2)
+        p.set_tiles(*tiles)
+        for d, x in enumerate(datas[k]):
+            ref[(k, d)] = p.run(torch.from_numpy(x).cuda()).cpu().numpy()
+    return ref
+
+
+@pytest.mark.parametrize("budget_frames,slice_ms", [(None, 4.0), (None, 0.0), (2.5, 0.01)])
+def test_served_images_equal_alone(cuda, budget_frames, slice_ms):
+    plans, datas = _setup()
+    kw = {}
+    if budget_frames is not None:
+        big = max(int(np.prod(p.out_shape)) * 8 + d[0].nbytes for p, d in zip(plans, datas))
+        kw["hbm_budget"] = int(budget_frames * big)
+    srv = Server(plans, datas, slice_ms=slice_ms, **kw)
+    trace = poisson_trace(60, rate=20000.0, n_configs=len(plans), n_data=2, seed=11)
+    res = srv.run(trace, keep_outputs=True)
+    assert len(res.outputs) == len(trace)
+    assert res.iterations >= 2
+    ref = _alone(plans, datas, srv.tiles)
+    for r in trace:
+        assert np.array_equal(res.outputs[r.rid], ref[(r.config, r.data)]), r
+    assert np.all(res.latency_s > 0)
+    # the plain loop: one launch per request with the plans' default tiles
+    for p in plans:
+        p.set_variant(2)
+    plain = srv.plain_loop(trace[:20], keep_outputs=True)
+    for r in trace[:20]:
+        a, b = plain.outputs[r.rid], ref[(r.config, r.data)]
+        assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(b))
+    srv.close()
+    for p in plans:
+        p.close()

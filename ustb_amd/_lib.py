@@ -120,6 +120,13 @@ EXPORTS = {
     "ustbx_batch_shape": (c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
                                     ctypes.POINTER(c_int32), ctypes.POINTER(c_int64)]),
     "ustbx_batch_destroy": (None, [c_void_p]),
+    "ustbx_plan_positions": (c_int32, [c_void_p]),
+    "ustbx_batcher_create": (ctypes.c_int, [c_int32, c_int32, c_int64, c_int32, c_int32, ctypes.POINTER(c_void_p),
+                                            ctypes.POINTER(Status)]),
+    "ustbx_batcher_submit": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                            ctypes.POINTER(c_void_p), ctypes.POINTER(c_int32),
+                                            ctypes.POINTER(c_int32), c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_batcher_destroy": (None, [c_void_p]),
     "ustbx_synth_echoes": (ctypes.c_int, [ctypes.POINTER(SynthDesc), c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p]),
     "ustbx_hilbert": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, ctypes.POINTER(Status)]),

@@ -13,6 +13,10 @@ C3  phased array (64 el, 0.2 mm, f0 2.5 MHz), 31 diverging waves from
     sampled at 10 MHz, 2048 samples, 32 frames with scatterers moving
     0.1 mm/frame; sector scan 256 beams x 1024 depths (1-120 mm);
     dimension transmit.
+C5  serving: requests of one frame each, Poisson arrivals, drawn uniformly
+    from linear CPWC 11 waves (C4b on 256^2 pixels), phased DW 31 waves
+    (C3's probe and scan, dimension both, rx Tukey-25 f# 1.7) and matrix
+    3-D 25 waves (C4c); complex64 IQ.
 C4  ragged batch of 64 acquisitions drawn (seeded) from
     (a) STA, 128 el linear array: 128 transmits (sources at the elements)
         x 128 channels x 2048 RF samples at 20.8 MHz (f0 5.2 MHz), beamformed
@@ -274,6 +278,32 @@ def c4(seed: int = 4, scale: float = 1.0, with_data: bool = True, n_acq: int = 6
             datasets[ci] = by_kind[k]
     return BatchWorkload(f"C4 ragged batch of {n_acq} (STA RF / CPWC IQ / 3-D DW), linear, both",
                          configs, datasets, acqs)
+
+
+def _c5_dw(scale: float) -> Acquisition:
+    """C3's phased array and diverging waves, one frame, dimension both."""
+    acq = c3(scale=scale, with_data=False, frames=1)
+    acq.name = "C5 phased DW 31 waves, sector 256x1024"
+    acq.dimension = Dimension.both
+    acq.tx_apod = Apodization()
+    acq.rx_apod = Apodization(Window.tukey25, f_number=1.7)
+    acq.meta = {"f0": 2.5e6}
+    return acq
+
+
+def c5(seed: int = 5, scale: float = 1.0, with_data: bool = True, datasets_per_config: int = 2,
+       n_speckle: int = 2000):
+    """C5 serving configurations: linear CPWC 11 waves (C4b, 256^2 pixels),
+    phased DW 31 waves (C3's probe and sector scan, one frame, dimension
+    both) and matrix 3-D 25 waves (C4c); complex64 IQ from seeded speckle.
+    Returns (configs, datasets)."""
+    configs = [_c4_cpwc(256, scale), _c5_dw(scale), _c4_3d(scale)]
+    datasets = [[] for _ in configs]
+    if with_data:
+        for k, acq in enumerate(configs):
+            datasets[k] = [_c4_data(acq, np.random.default_rng(seed * 100 + 10 * k + i),
+                                    max(50, int(round(n_speckle * scale)))) for i in range(datasets_per_config)]
+    return configs, datasets
 
 
 BUILDERS = {"C1": c1, "C2": c2, "C3": c3}

@@ -383,3 +383,153 @@ void ustbx_batch_destroy(ustbx_batch* b) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batcher: a new set of acquisitions (and slices of them) every submission,
+// for serving loops that batch whatever frames are ready. Each acquisition's
+// tiles are cut into the same 8 per-XCD shares as above; a submission takes
+// positions [begin, end) of every share, so a large frame can be spread over
+// several submissions (its tiles are independent: each is computed whole).
+// ---------------------------------------------------------------------------
+struct ustbx_batcher {
+  int device;
+  int sm;
+  int wc, ws;
+  int nblocks;
+  int max_acqs;
+  int64_t max_items;
+  char* dev;      // queue offsets | descriptors | items | counters
+  char* staging;  // pinned host copy of the first three
+  size_t b_qoff, b_acq;
+  hipEvent_t staged;  // the last submission's uploads have left staging
+  bool pending;
+};
+
+extern "C" {
+
+int32_t ustbx_plan_positions(const ustbx_plan* plan) {
+  if (!plan || plan->rxtab_state != 1) return -1;
+  const int tiles = (int)plan->h_order.size();
+  return (tiles + kQueues - 1) / kQueues;
+}
+
+int ustbx_batcher_create(int32_t interp, int32_t max_acqs, int64_t max_items, int32_t wave_chunk, int32_t split,
+                         ustbx_batcher** out, ustbx_status* st) {
+  if (!out) return set_status(st, USTBX_E_ARG, "NULL batcher");
+  *out = nullptr;
+  if (interp != USTBX_INTERP_LINEAR && interp != USTBX_INTERP_CUBIC)
+    return set_status(st, USTBX_E_ARG, "unknown interpolation");
+  if (max_acqs <= 0 || max_items <= 0) return set_status(st, USTBX_E_ARG, "capacities must be positive");
+  if (!wc_ok(wave_chunk)) return set_status(st, USTBX_E_ARG, "waves per split must be 4, 5, 6, 8, 10, 12 or 16");
+  if (split != 4 && split != 5 && split != 8) return set_status(st, USTBX_E_ARG, "wave-splits must be 4, 5 or 8");
+  ustbx_batcher* b = new (std::nothrow) ustbx_batcher();
+  if (!b) return set_status(st, USTBX_E_CAPACITY, "host allocation failed");
+  hipError_t e = hipGetDevice(&b->device);
+  b->sm = interp == USTBX_INTERP_CUBIC ? SM_CUBIC : SM_LINEAR;
+  b->wc = wave_chunk;
+  b->ws = split;
+  b->max_acqs = max_acqs;
+  b->max_items = max_items;
+  int cus = 0;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  const int occ = dispatch(b->sm, b->wc, b->ws, OccupancyFn{});
+  b->nblocks = ((cus * std::max(occ, 1) + kQueues - 1) / kQueues) * kQueues;
+  auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+  b->b_qoff = al(sizeof(int) * (kQueues + 1));
+  b->b_acq = al(sizeof(RxAcq) * (size_t)max_acqs);
+  const size_t b_items = al(sizeof(int4) * (size_t)max_items), b_ctr = al(sizeof(int) * kQueues);
+  if (e == hipSuccess) e = hipMalloc(&b->dev, b->b_qoff + b->b_acq + b_items + b_ctr);
+  if (e == hipSuccess) e = hipHostMalloc(&b->staging, b->b_qoff + b->b_acq + b_items, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    ustbx_batcher_destroy(b);
+    return hip_fail(st, e, "batcher setup");
+  }
+  *out = b;
+  return ok(st);
+}
+
+int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* const* plans, const void* const* ch,
+                         void* const* out, const int32_t* pos_begin, const int32_t* pos_end, void* stream,
+                         ustbx_status* st) {
+  if (!b) return set_status(st, USTBX_E_ARG, "NULL batcher");
+  if (count <= 0) return ok(st);
+  if (count > b->max_acqs) return set_status(st, USTBX_E_CAPACITY, "more acquisitions than the batcher holds");
+  if (!plans || !ch || !out || !pos_begin || !pos_end) return set_status(st, USTBX_E_ARG, "NULL argument array");
+  hipError_t e = hipSuccess;
+  if (b->pending) {
+    e = hipEventSynchronize(b->staged);  // staging is free again
+    if (e != hipSuccess) return hip_fail(st, e, "batcher staging");
+    b->pending = false;
+  }
+  int* qoff = reinterpret_cast<int*>(b->staging);
+  RxAcq* acqs = reinterpret_cast<RxAcq*>(b->staging + b->b_qoff);
+  int4* items = reinterpret_cast<int4*>(b->staging + b->b_qoff + b->b_acq);
+  // count first, so that capacity errors leave nothing half-written
+  int64_t n_items = 0;
+  for (int i = 0; i < count; ++i) {
+    const ustbx_plan* pl = plans[i];
+    if (!pl || pl->rxtab_state != 1)
+      return set_status(st, USTBX_E_UNSUPPORTED, "acquisition %d: plan without receive-table state", i);
+    if (pl->device != b->device) return set_status(st, USTBX_E_ARG, "acquisition %d: plan of another device", i);
+    if ((pl->G.interp == USTBX_INTERP_CUBIC) != (b->sm == SM_CUBIC))
+      return set_status(st, USTBX_E_ARG, "acquisition %d: interpolation differs from the batcher's", i);
+    if (!ch[i] || !out[i]) return set_status(st, USTBX_E_ARG, "acquisition %d: NULL buffer", i);
+    const int tiles = (int)pl->h_order.size(), q = tiles / kQueues, r = tiles % kQueues;
+    const int pb = std::max(0, pos_begin[i]);
+    for (int k = 0; k < kQueues; ++k) {
+      const int cnt = q + (k < r ? 1 : 0);
+      const int pe = std::min(cnt, pos_end[i]);
+      if (pe > pb) n_items += (int64_t)(pe - pb) * pl->G.F;
+    }
+  }
+  if (n_items > b->max_items) return set_status(st, USTBX_E_CAPACITY, "more work items than the batcher holds");
+  int64_t at = 0;
+  for (int k = 0; k < kQueues; ++k) {
+    qoff[k] = (int)at;
+    for (int i = 0; i < count; ++i) {
+      const ustbx_plan* pl = plans[i];
+      const int tiles = (int)pl->h_order.size(), q = tiles / kQueues, r = tiles % kQueues;
+      const int s0 = k * q + std::min(k, r), cnt = q + (k < r ? 1 : 0);
+      const int pb = std::max(0, pos_begin[i]), pe = std::min(cnt, pos_end[i]);
+      for (int f = 0; f < pl->G.F; ++f)
+        for (int x = pb; x < pe; ++x) items[at++] = make_int4(i, pl->h_order[s0 + x], f, 0);
+    }
+  }
+  qoff[kQueues] = (int)at;
+  for (int i = 0; i < count; ++i) {
+    acqs[i] = rx_acq(plans[i]->G, b->sm);
+    acqs[i].data = ch[i];
+    acqs[i].out = reinterpret_cast<float2*>(out[i]);
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  e = hipMemcpyAsync(b->dev, b->staging, b->b_qoff + sizeof(RxAcq) * count, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && at > 0)
+    e = hipMemcpyAsync(b->dev + b->b_qoff + b->b_acq, items, sizeof(int4) * at, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipEventRecord(b->staged, s);
+  if (e != hipSuccess) return hip_fail(st, e, "batcher upload");
+  b->pending = true;
+  if (at == 0) return ok(st);
+  int* d_ctr = reinterpret_cast<int*>(b->dev + b->b_qoff + b->b_acq + ((sizeof(int4) * b->max_items + 255) & ~size_t(255)));
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, d_ctr);
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    BatchLaunch B{reinterpret_cast<const RxAcq*>(b->dev + b->b_qoff),
+                  reinterpret_cast<const int4*>(b->dev + b->b_qoff + b->b_acq), reinterpret_cast<const int*>(b->dev),
+                  d_ctr};
+    e = dispatch(b->sm, b->wc, b->ws, LaunchFn{B, b->nblocks, s});
+  }
+  if (e != hipSuccess) return hip_fail(st, e, "das_batch_kernel launch");
+  return ok(st);
+}
+
+void ustbx_batcher_destroy(ustbx_batcher* b) {
+  if (!b) return;
+  if (b->pending) (void)hipEventSynchronize(b->staged);
+  if (b->staged) (void)hipEventDestroy(b->staged);
+  if (b->staging) (void)hipHostFree(b->staging);
+  (void)hipFree(b->dev);
+  delete b;
+}
+
+}  // extern "C"

@@ -1,0 +1,344 @@
+"""Serving loop: continuous batching of beamforming requests on one GPU.
+
+Requests (one frame each, of one of several probe configurations) arrive on
+a trace. The server keeps every configuration's plan (geometry, delays and
+apodization tables) resident, admits a request when its input and output
+buffers fit in the HBM budget (smallest input first, with a bounded amount
+of input queued on the copy stream), copies its channel data in on a copy
+stream,
+and runs iterations of the batcher on a compute stream: each iteration takes
+the ready requests in order of least remaining work (then arrival) and gives
+each a slice of its tiles until about `slice_ms` of estimated work is packed
+(`long_slices` times that while no small request is pending; small frames go
+whole, large frames over several iterations), so requests
+that arrive while others are on the GPU join the next iteration and small
+frames are not held behind large ones. Up to `depth` iterations are in
+flight. A request whose last slice has run is copied back on a third stream;
+its latency is arrival -> result in host memory.
+
+`plain_loop` is the baseline: one request at a time, one plan launch per
+frame (copy in, beamform, copy out, wait).
+"""
+from __future__ import annotations
+
+import ctypes
+import heapq
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .ops import GeometryPlan, _stream_ptr
+
+
+@dataclass
+class Request:
+    rid: int
+    config: int
+    arrival: float          # seconds after the trace starts
+    data: int               # dataset index of its configuration
+
+
+def poisson_trace(n: int, rate: float, n_configs: int, n_data: int, seed: int = 5) -> list[Request]:
+    """n requests with exponential inter-arrival times (mean 1/rate), each
+    drawing a configuration uniformly and one of n_data datasets."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    t -= t[0]
+    cfg = rng.integers(0, n_configs, n)
+    dat = rng.integers(0, n_data, n)
+    return [Request(i, int(c), float(a), int(d)) for i, (a, c, d) in enumerate(zip(t, cfg, dat))]
+
+
+class Batcher:
+    """ctypes wrapper of ustbx_batcher."""
+
+    def __init__(self, interp: int, max_acqs: int, max_items: int, wave_chunk: int, split: int):
+        self.handle = ctypes.c_void_p()
+        st = L.Status()
+        L.check(L.lib().ustbx_batcher_create(interp, max_acqs, max_items, wave_chunk, split,
+                                             ctypes.byref(self.handle), ctypes.byref(st)), st)
+
+    def submit(self, plans, chs, outs, begin, end, stream):
+        n = len(plans)
+        P = (ctypes.c_void_p * n)(*[p.handle.value for p in plans])
+        C = (ctypes.c_void_p * n)(*[c.data_ptr() for c in chs])
+        O = (ctypes.c_void_p * n)(*[o.data_ptr() for o in outs])
+        B = (ctypes.c_int32 * n)(*begin)
+        E = (ctypes.c_int32 * n)(*end)
+        st = L.Status()
+        L.check(L.lib().ustbx_batcher_submit(self.handle, n, P, C, O, B, E, stream.cuda_stream, ctypes.byref(st)),
+                st)
+
+    def close(self):
+        if self.handle:
+            L.lib().ustbx_batcher_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class _Live:
+    req: Request
+    bytes: int
+    ch: torch.Tensor
+    out: torch.Tensor
+    ready: torch.cuda.Event
+    next_pos: int = 0
+    done: torch.cuda.Event | None = None
+    host: torch.Tensor | None = None
+
+
+@dataclass
+class ServeResult:
+    latency_s: np.ndarray            # per request, in trace order
+    configs: np.ndarray              # configuration per request
+    elapsed_s: float                 # first arrival -> last result
+    iterations: int
+    outputs: dict = field(default_factory=dict)   # rid -> host image (keep_outputs)
+
+    @property
+    def frames_per_s(self) -> float:
+        return len(self.latency_s) / self.elapsed_s
+
+    def summary(self, names: list[str] | None = None) -> dict:
+        lat = self.latency_s * 1e3
+        res = {"frames": int(lat.size), "frames_per_s": round(self.frames_per_s, 2),
+               "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "elapsed_s": round(self.elapsed_s, 3)}
+        if self.iterations:
+            res["iterations"] = self.iterations
+        per = {}
+        for k in np.unique(self.configs):
+            m = self.configs == k
+            key = names[k] if names else str(int(k))
+            per[key] = {"frames": int(m.sum()), "p50_ms": round(float(np.percentile(lat[m], 50)), 3),
+                        "p99_ms": round(float(np.percentile(lat[m], 99)), 3)}
+        res["per_config"] = per
+        return res
+
+
+class Server:
+    def __init__(self, plans: list[GeometryPlan], datasets: list[list[np.ndarray]], *,
+                 hbm_budget: int = 64 << 30, slice_ms: float = 2.0, depth: int = 2,
+                 tiles: tuple[int, int] = (8, 4), max_acqs: int = 4096, copy_window: int = 512 << 20,
+                 long_slices: int = 8):
+        if not plans:
+            raise ValueError("no configurations")
+        interp = plans[0].interp
+        if any(p.interp != interp or p.dim != L.DIM_BOTH for p in plans):
+            raise ValueError("the configurations share one interpolation and dimension both")
+        for p in plans:
+            if p.variant[0] < 2:
+                p.set_variant(2)
+        self.plans = plans
+        self.device = plans[0].device
+        self.budget = int(hbm_budget)
+        self.copy_window = int(copy_window)
+        self.long_slices = int(long_slices)
+        self.slice_ms = float(slice_ms)
+        self.depth = int(depth)
+        self.positions = [int(L.lib().ustbx_plan_positions(p.handle)) for p in plans]
+        # host channel data in pinned memory (the requests' payloads)
+        self.host = [[torch.from_numpy(np.ascontiguousarray(d)).pin_memory() for d in ds] for ds in datasets]
+        self.in_bytes = [int(ds[0].nbytes) for ds in datasets]
+        self.out_bytes = [int(np.prod(p.out_shape)) * 8 for p in plans]
+        tiles_per_plan = [((p.P + 63) // 64) * p.F for p in plans]
+        self.max_items = max(4 * sum(tiles_per_plan), 1 << 16)
+        self.max_acqs = int(max_acqs)
+        self.batcher = Batcher(interp, self.max_acqs, self.max_items, *tiles)
+        self.tiles = tiles
+        self.s_in = torch.cuda.Stream(self.device)
+        self.s_run = torch.cuda.Stream(self.device)
+        self.s_out = torch.cuda.Stream(self.device)
+        self.ms_per_pos = self._calibrate()
+
+    def _small(self, k: int) -> bool:
+        """A configuration whose whole frame fits in one short iteration."""
+        return self.ms_per_pos[k] * self.positions[k] <= self.slice_ms
+
+    def _calibrate(self) -> list[float]:
+        """Time one whole frame of every configuration through the batcher;
+        the per-position cost drives slicing."""
+        res = []
+        with torch.cuda.device(self.device):
+            for k, p in enumerate(self.plans):
+                with torch.cuda.stream(self.s_run):
+                    ch = self.host[k][0].to(self.device, non_blocking=True)
+                    out = torch.empty(p.out_shape, dtype=torch.complex64, device=self.device)
+                for rep in range(2):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(self.s_run)
+                    self.batcher.submit([p], [ch], [out], [0], [self.positions[k]], self.s_run)
+                    e1.record(self.s_run)
+                    e1.synchronize()
+                res.append(e0.elapsed_time(e1) / max(self.positions[k], 1))
+                del ch, out
+        return res
+
+    def run(self, trace: list[Request], keep_outputs: bool = False) -> ServeResult:
+        dev = self.device
+        n = len(trace)
+        lat = np.zeros(n)
+        waiting: list = []  # heap of (input bytes, arrival, rid, request)
+        staged: deque[_Live] = deque()
+        copying = 0  # input bytes queued on the copy stream
+        ready: list[_Live] = []
+        running: deque[tuple[torch.cuda.Event, list[_Live]]] = deque()
+        draining: deque[_Live] = deque()
+        outputs = {}
+        in_flight = 0
+        completed = 0
+        nxt = 0
+        iters = 0
+        t0 = time.perf_counter()
+        last_done = 0.0
+        with torch.cuda.device(dev):
+            while completed < n:
+                now = time.perf_counter() - t0
+                progressed = False
+                while nxt < n and trace[nxt].arrival <= now:
+                    r = trace[nxt]
+                    heapq.heappush(waiting, (self.in_bytes[r.config], r.arrival, r.rid, r))
+                    nxt += 1
+                # admission: smallest input first, under the HBM budget, with
+                # at most copy_window bytes queued on the copy stream (so a
+                # small frame's copy never waits behind many large ones)
+                while waiting:
+                    r = waiting[0][3]
+                    need = self.in_bytes[r.config] + self.out_bytes[r.config]
+                    if in_flight + need > self.budget and in_flight > 0:
+                        break
+                    if copying + self.in_bytes[r.config] > self.copy_window and copying > 0:
+                        break
+                    heapq.heappop(waiting)
+                    copying += self.in_bytes[r.config]
+                    p = self.plans[r.config]
+                    ch = torch.empty((p.F, p.W, p.N, p.L), dtype=torch.complex64, device=dev)
+                    out = torch.empty(p.out_shape, dtype=torch.complex64, device=dev)
+                    with torch.cuda.stream(self.s_in):
+                        ch.copy_(self.host[r.config][r.data], non_blocking=True)
+                        ev = torch.cuda.Event()
+                        ev.record(self.s_in)
+                    staged.append(_Live(r, need, ch, out, ev))
+                    in_flight += need
+                    progressed = True
+                while staged and staged[0].ready.query():
+                    lv = staged.popleft()
+                    copying -= self.in_bytes[lv.req.config]
+                    ready.append(lv)
+                    progressed = True
+                # an iteration over the ready requests
+                if ready and len(running) < self.depth:
+                    plans, chs, outs, begin, end, finishing = [], [], [], [], [], []
+                    # a short iteration while small requests are pending (their
+                    # latency), a long one when only large frames are (fewer,
+                    # fuller launches)
+                    small = any(self._small(lv.req.config) for lv in ready) or \
+                        any(self._small(lv.req.config) for lv in staged) or \
+                        (bool(waiting) and self._small(waiting[0][3].config))
+                    budget = self.slice_ms if small else self.slice_ms * self.long_slices
+                    items = 0
+                    # shortest remaining work first (then arrival): small
+                    # frames pass large ones, and the large frame nearest
+                    # completion gets the rest of the budget
+                    ready.sort(key=lambda lv: ((self.positions[lv.req.config] - lv.next_pos) *
+                                               self.ms_per_pos[lv.req.config], lv.req.arrival))
+                    for lv in ready:
+                        if plans and (budget <= 0 or len(plans) == self.max_acqs):
+                            break  # (the oldest ready request always gets a slice)
+                        k = lv.req.config
+                        left = self.positions[k] - lv.next_pos
+                        take = min(left, max(1, int(budget / max(self.ms_per_pos[k], 1e-9))))
+                        per_pos = 8 * self.plans[k].F  # work items per position (at most)
+                        if items + take * per_pos > self.max_items:
+                            take = (self.max_items - items) // per_pos
+                            if take <= 0:
+                                break
+                        items += take * per_pos
+                        plans.append(self.plans[k])
+                        chs.append(lv.ch)
+                        outs.append(lv.out)
+                        begin.append(lv.next_pos)
+                        end.append(lv.next_pos + take)
+                        lv.next_pos += take
+                        budget -= self.ms_per_pos[k] * take
+                        if lv.next_pos >= self.positions[k]:
+                            finishing.append(lv)
+                    self.batcher.submit(plans, chs, outs, begin, end, self.s_run)
+                    ev = torch.cuda.Event()
+                    ev.record(self.s_run)
+                    running.append((ev, finishing))
+                    ready = [lv for lv in ready if lv.next_pos < self.positions[lv.req.config]]
+                    iters += 1
+                    progressed = True
+                # finished iterations: copy their completed requests out
+                while running and running[0][0].query():
+                    ev, finishing = running.popleft()
+                    for lv in finishing:
+                        self.s_out.wait_event(ev)
+                        with torch.cuda.stream(self.s_out):
+                            lv.host = torch.empty(lv.out.shape, dtype=lv.out.dtype, pin_memory=True)
+                            lv.host.copy_(lv.out, non_blocking=True)
+                            lv.done = torch.cuda.Event()
+                            lv.done.record(self.s_out)
+                        draining.append(lv)
+                    progressed = True
+                while draining and draining[0].done.query():
+                    lv = draining.popleft()
+                    tdone = time.perf_counter() - t0
+                    lat[lv.req.rid] = tdone - lv.req.arrival
+                    last_done = max(last_done, tdone)
+                    if keep_outputs:
+                        outputs[lv.req.rid] = lv.host.numpy()
+                    in_flight -= lv.bytes
+                    lv.ch = lv.out = lv.host = None
+                    completed += 1
+                    progressed = True
+                if not progressed:
+                    time.sleep(20e-6)
+        cfgs = np.array([r.config for r in trace])
+        return ServeResult(lat, cfgs, last_done - trace[0].arrival, iters, outputs)
+
+    def plain_loop(self, trace: list[Request], keep_outputs: bool = False) -> ServeResult:
+        """One request at a time: copy in, one launch of its plan, copy out, wait."""
+        dev = self.device
+        n = len(trace)
+        lat = np.zeros(n)
+        outputs = {}
+        bufs = {}
+        s = self.s_run
+        t0 = time.perf_counter()
+        last_done = 0.0
+        with torch.cuda.device(dev):
+            for k, p in enumerate(self.plans):
+                bufs[k] = (torch.empty((p.F, p.W, p.N, p.L), dtype=torch.complex64, device=dev),
+                           torch.empty(p.out_shape, dtype=torch.complex64, device=dev),
+                           torch.empty(p.out_shape, dtype=torch.complex64, pin_memory=True))
+            for r in trace:
+                while time.perf_counter() - t0 < r.arrival:
+                    time.sleep(20e-6)
+                ch, out, host = bufs[r.config]
+                with torch.cuda.stream(s):
+                    ch.copy_(self.host[r.config][r.data], non_blocking=True)
+                    self.plans[r.config].run(ch, out, stream=s)
+                    host.copy_(out, non_blocking=True)
+                s.synchronize()
+                tdone = time.perf_counter() - t0
+                lat[r.rid] = tdone - r.arrival
+                last_done = tdone
+                if keep_outputs:
+                    outputs[r.rid] = host.numpy().copy()
+        cfgs = np.array([r.config for r in trace])
+        return ServeResult(lat, cfgs, last_done - trace[0].arrival, 0, outputs)
+
+    def close(self):
+        self.batcher.close()
