@@ -1,0 +1,99 @@
+"""The midprocess.das facade (ustb_amd.das.DAS) and the device Hilbert step.
+
+- hilbert: the analytic signal of das.m:211-214 / das.py:144-145 against
+  scipy.signal.hilbert (the oracle's), odd and even lengths;
+- DAS.go(): IQ data in the reference layout [time, channel, wave, frame],
+  delays and apodization on the device, the reference phase of das.m:306-309,
+  output [pixel, channel, wave, frame], against the FP64 oracle on the
+  oracle's geometry (das_c's wd and FP32 index, as in test_gpu_geom.py);
+  RF data through the Hilbert step; the plan cache;
+- DAS._beamform: the Python reference's static contract (das.py:203-261)
+  against the oracle's restatement of it.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.cases import channel_noise, geom_cases, matrix_problem, oracle_matrices
+from ustb_amd.das import DAS
+from ustb_amd.enums import Dimension
+from ustb_amd.ops import hilbert
+from ustb_amd.uff import ChannelData
+
+pytestmark = pytest.mark.gpu
+CASES = geom_cases()
+TOL = 1e-5
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+@pytest.mark.parametrize("L", [777, 1024])
+def test_hilbert_matches_scipy(cuda, L):
+    rng = np.random.default_rng(L)
+    x = rng.standard_normal((2, 3, 5, L)).astype(np.float32)
+    got = hilbert(torch.from_numpy(x).to(cuda)).cpu().numpy()
+    ref = O.hilbert(x.astype(np.float64), axis=-1)
+    # the real part comes back through the FFT round trip, as in MATLAB's and
+    # scipy's hilbert (ifft(fft(x) .* h)): equal to x up to FP32 rounding
+    assert _rel(got.real, x) <= TOL
+    assert _rel(got, ref) <= TOL
+
+
+def _das(case, data, dim=Dimension.both):
+    d = DAS()
+    d.channel_data = ChannelData(probe=case["probe"], sequence=case["seq"], sampling_frequency=case["fs"],
+                                 modulation_frequency=case["fd"], data=data)
+    d.scan = case["scan"]
+    d.receive_apodization = case["rx"]
+    d.transmit_apodization = case["tx"]
+    d.dimension = dim
+    d.spherical_transmit_delay_model = case["model"]
+    return d
+
+
+@pytest.mark.parametrize("name,dim", [("cpwc_tukey", Dimension.both), ("dw_sector", Dimension.both),
+                                      ("cpwc_tukey", Dimension.transmit)])
+def test_go_iq_matches_oracle(cuda, name, dim):
+    case = CASES[name]
+    ch = channel_noise(case, F=2, seed=5)                     # (F, W, N, L)
+    d = _das(case, np.ascontiguousarray(ch.transpose(3, 2, 1, 0)), dim)
+    got = d.go().data.cpu().numpy()                            # (P, N', W', F)
+    ta, ra, td, rd = oracle_matrices(case)
+    ref = O.beamform_fp64(ch, case["fs"], 0.0, ta, ra, td, rd, case["fd"], int(dim),
+                          w0=O.dasc_wd(case["fd"]), index="fp32")
+    ref = ref * O.reference_phase(case["scan"].reference_distance, case["fd"], 1540.0)[None, None, None, :]
+    assert got.shape == ref.transpose(3, 2, 1, 0).shape
+    assert _rel(got, ref.transpose(3, 2, 1, 0)) <= TOL
+    # a second go() reuses the cached plan and gives the same image
+    again = d.go().data.cpu().numpy()
+    assert len(d._plans) == 1 and np.array_equal(again, got)
+    assert d.elapsed_time is not None and d.elapsed_time > 0
+
+
+def test_go_rf_goes_through_hilbert(cuda):
+    case = CASES["pw_rf_hanning"]
+    rng = np.random.default_rng(17)
+    F, W, N, L = 1, len(case["seq"]), case["probe"].N_elements, case["L"]
+    rf = rng.standard_normal((F, W, N, L)).astype(np.float32)
+    d = _das(case, np.ascontiguousarray(rf.transpose(3, 2, 1, 0)))
+    got = d.go().data.cpu().numpy()
+    analytic = O.hilbert(rf.astype(np.float64), axis=-1).astype(np.complex64)
+    ta, ra, td, rd = oracle_matrices(case)
+    ref = O.beamform_fp64(analytic, case["fs"], 0.0, ta, ra, td, rd, 0.0, 3, index="fp32")
+    assert _rel(got, ref.transpose(3, 2, 1, 0)) <= TOL
+
+
+@pytest.mark.parametrize("dim", [0, 1, 2, 3])
+def test_beamform_static_matches_python_reference(cuda, dim):
+    pb = matrix_problem(seed=21 + dim, P=300, F=2)
+    ch = np.ascontiguousarray(pb["ch"].transpose(3, 2, 1, 0))  # (L, N, W, F)
+    t = np.arange(ch.shape[0], dtype=np.float32)
+    w0 = np.float32(2.0 * np.pi * np.float32(0.05))
+    args = (ch, t, pb["tx_apo"].T, pb["rx_apo"].T, pb["tx_delay"].T, pb["rx_delay"].T, w0, dim)
+    got = DAS._beamform(*args, N_pixels=300, N_channels=ch.shape[1], N_waves=ch.shape[2], N_frames=ch.shape[3])
+    ref = O.python_beamform(*args)
+    assert got.shape == ref.shape
+    assert _rel(got, ref) <= TOL
