@@ -164,7 +164,8 @@ def main():
     achieved_gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
     copy = copy_probe(torch, dev)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"das_both_rxtab_{args.config.lower()}_traffic.txt")
+    kname = "das_tx_rxtab" if int(acq.dimension) == 2 else "das_both_rxtab"
+    tpath = os.path.join(ROOT, "profiles", f"{kname}_{args.config.lower()}_traffic.txt")
     if os.path.exists(tpath):
         for line in open(tpath):
             if line.startswith("traffic_bytes_per_launch"):
@@ -249,7 +250,7 @@ def run_batch(args, torch, dev):
     # tile shape: the default (work-weighted) and neighbours, timed on the batch
     shapes = {}
     default = (b.shape["wave_chunk"], b.shape["split"])
-    for wc, ws in dict.fromkeys([default, (4, 8), (5, 5), (8, 4), (4, 4)]):
+    for wc, ws in dict.fromkeys([default, (8, 4), (4, 8), (5, 5), (6, 4), (10, 4)]):
         b.set_tiles(wc, ws)
         b.launch()
         shapes[(wc, ws)] = timed(b.launch, 1)[0]
@@ -260,6 +261,12 @@ def run_batch(args, torch, dev):
         b.replay()
     batch_ms, batch_wall = timed(b.replay, args.steps)
     plain_outs = [torch.empty_like(o) for o in outs]
+    # the plain loop runs every plan with its own autotuned tiles
+    from ustb_amd import autotune
+    for k in sorted(set(k for k, _ in wl.acquisitions)):
+        i = next(i for i, (kk, _) in enumerate(wl.acquisitions) if kk == k)
+        best_k, _ = autotune.tune(plans[k], chs[i], reps=1)
+        log(f"plain loop tiles for {wl.configs[k].name}: {best_k}")
 
     def plain():
         for p, c, o in zip(bplans, chs, plain_outs):

@@ -243,10 +243,12 @@ int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
  *   1 = receive state shared in LDS by the wave-splits of a 64-pixel tile
  *       (dimension both, IQ, separable phase, one frame);
  *   2 = resident receive table and per-tile wave/channel lists (dimension
- *       both, separable or RF phase, any frame count, receive table within half
- *       the device memory, finite nonzero weights and delays; the default
- *       where it applies);
- *   3 = variant 2 over cubic-interpolation coefficients: the plan holds a
+ *       both or transmit, separable or RF phase, any frame count, receive
+ *       table within half the device memory, finite nonzero weights and
+ *       delays; the default where it applies). For dimension transmit
+ *       set_tiles' arguments are frames per block (4 or 8) and channels
+ *       per block (4 or 8; one wave64 each);
+ *   3 = variant 2 (dimension both) over cubic-interpolation coefficients: the plan holds a
  *       buffer of 32 bytes per channel sample that every run rewrites, so
  *       runs of one variant-3 plan on different streams must not overlap.
  * USTBX_E_UNSUPPORTED where the variant does not apply. */
@@ -278,8 +280,9 @@ int ustbx_batch_set_io(ustbx_batch* batch, const void* const* ch, void* const* o
                        ustbx_status* st);
 /* Tile shape of the persistent kernel (0 = keep): waves per split (4, 5, 6,
  * 8, 10, 12, 16) and wave-splits per 64-pixel tile (4, 5 or 8). The default
- * takes 8 splits and covers in one round the wave count below which half of
- * the batch's work lies. Drops a captured graph. */
+ * follows the wave count below which half of the batch's work lies: up to 32
+ * waves 4 splits of 8, else 8 splits covering it in one round. Drops a
+ * captured graph. */
 int ustbx_batch_set_tiles(ustbx_batch* batch, int32_t wave_chunk, int32_t split, ustbx_status* st);
 /* Enqueues the batch on stream: a counter-reset kernel and the persistent
  * kernel; allocates nothing and can be captured in a hipGraph. Launches of

@@ -212,13 +212,50 @@ def test_receive_table_variant(cuda, name, interp):
     plan.close()
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("interp", [Interpolation.linear, Interpolation.cubic])
+def test_transmit_table_variant(cuda, name, interp):
+    """Dimension transmit from the resident tables (frames gathered together):
+    9 frames, so the second chunk of 8 holds one frame; against variant 0 and
+    the FP64 oracle, over its frame/channel-split shapes."""
+    case = CASES[name]
+    plan = _plan(case, dim=2, F=9, separable_phase=True, interpolation=int(interp))
+    applies = plan.has_variant(2)
+    assert applies == (not _nonfinite_weight(case))
+    if not applies:
+        plan.close()
+        return
+    assert plan.variant[0] == 2
+    ch = channel_noise(case, F=9, seed=23)
+    chd = torch.from_numpy(ch).to(cuda)
+    outs = []
+    for fc, cs in ((8, 4), (4, 8), (4, 4), (8, 8)):
+        plan.set_tiles(fc, cs)
+        outs.append(plan.run(chd).cpu().numpy())
+    for o in outs[1:]:
+        assert _rel(o, outs[0]) <= TOL
+    plan.set_variant(0)
+    v0 = plan.run(chd).cpu().numpy()
+    assert _rel(outs[0], v0) <= TOL
+    if interp == Interpolation.linear:
+        ta, ra, td, rd = oracle_matrices(case)
+        ref = O.beamform_fp64(ch, case["fs"], 0.0, ta, ra, td, rd, case["fd"], 2, w0=O.dasc_wd(case["fd"]),
+                              index="fp32")
+        assert _rel(outs[0], ref) <= TOL
+    plan.close()
+
+
 def test_receive_table_rejected_modes(cuda):
-    """Variant 2 covers dimension both only, and not das_c's summation order."""
+    """Variants 2/3 cover dimension both and transmit (3: both only), and not
+    das_c's summation order."""
     case = CASES["cpwc_tukey"]
-    for dim in (0, 1, 2):
+    for dim in (0, 1):
         plan = _plan(case, dim=dim, separable_phase=True)
         assert not plan.has_variant(2)
         plan.close()
+    plan = _plan(case, dim=2, separable_phase=True, interpolation=int(Interpolation.cubic))
+    assert plan.has_variant(2) and not plan.has_variant(3)
+    plan.close()
     plan = _plan(case, dim=3, dasc_order=True)
     assert not plan.has_variant(2)
     assert plan.variant[0] == 0

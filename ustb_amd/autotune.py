@@ -23,6 +23,7 @@ VARIANT0_EXACT = [(0, 8, 1), (0, 8, 4)]
 VARIANT1 = [(1, wc, ws) for ws in (4, 8) for wc in (8, 10, 12, 16)]
 VARIANT2 = [(2, wc, ws) for ws in (4, 5, 8) for wc in (4, 5, 6, 8, 10, 12, 16)]
 VARIANT3 = [(3, wc, ws) for ws in (4, 8) for wc in (6, 10, 16)]
+VARIANT2_TX = [(2, fc, cs) for fc in (4, 8) for cs in (4, 8)]   # frames per block, channels per block
 
 _cache: dict = {}
 _lock = threading.Lock()
@@ -40,6 +41,8 @@ def candidates(plan):
         c = [(0, 8, 1)] if not plan.separable else [(0, 8, 1), (0, 16, 1)]
     if plan.dim == L.DIM_BOTH and plan.separable and plan.F == 1 and plan.fd != 0.0:
         c += VARIANT1
+    if plan.dim == L.DIM_TX and plan.has_variant(2):
+        c += VARIANT2_TX
     if plan.dim == L.DIM_BOTH and plan.has_variant(2):
         c += VARIANT2
         if plan.interp == L.INTERP_CUBIC and plan.has_variant(3):

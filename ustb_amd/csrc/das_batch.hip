@@ -216,10 +216,10 @@ int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batc
     const ustbx_plan* pl = plans[i];
     if (!pl) return set_status(st, USTBX_E_ARG, "NULL plan %d", i);
     if (pl->device != device) return set_status(st, USTBX_E_ARG, "plan %d belongs to another device", i);
-    if (pl->rxtab_state != 1)
+    if (pl->rxtab_state != 1 || pl->G.dim != 3)
       return set_status(st, USTBX_E_UNSUPPORTED,
-                        "plan %d has no receive-table state (select variant 2 first; it needs dimension both, "
-                        "the separable or RF phase and finite weights)",
+                        "plan %d has no dimension-both receive-table state (select variant 2 first; it needs "
+                        "dimension both, the separable or RF phase and finite weights)",
                         i);
     if (interp >= 0 && pl->G.interp != interp)
       return set_status(st, USTBX_E_ARG, "the plans of a batch share one interpolation");
@@ -303,8 +303,13 @@ int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batc
       }
     }
   }
-  const int ws = 8, need = (std::max(nw_med, 1) + 7) / 8;
-  const int wc = need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : 16;
+  int ws = 8, wc = 8;
+  if (nw_med <= 32) {
+    ws = 4;  // as the one-plan default: 4 splits of 8 waves
+  } else {
+    const int need = (nw_med + 7) / 8;
+    wc = need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : 16;
+  }
   const int rc = set_shape(b, wc, ws, st);
   if (rc != USTBX_OK) {
     ustbx_batch_destroy(b);
@@ -408,7 +413,7 @@ struct ustbx_batcher {
 extern "C" {
 
 int32_t ustbx_plan_positions(const ustbx_plan* plan) {
-  if (!plan || plan->rxtab_state != 1) return -1;
+  if (!plan || plan->rxtab_state != 1 || plan->G.dim != 3) return -1;
   const int tiles = (int)plan->h_order.size();
   return (tiles + kQueues - 1) / kQueues;
 }
@@ -469,8 +474,9 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
   int64_t n_items = 0;
   for (int i = 0; i < count; ++i) {
     const ustbx_plan* pl = plans[i];
-    if (!pl || pl->rxtab_state != 1)
-      return set_status(st, USTBX_E_UNSUPPORTED, "acquisition %d: plan without receive-table state", i);
+    if (!pl || pl->rxtab_state != 1 || pl->G.dim != 3)
+      return set_status(st, USTBX_E_UNSUPPORTED, "acquisition %d: plan without dimension-both receive-table state",
+                        i);
     if (pl->device != b->device) return set_status(st, USTBX_E_ARG, "acquisition %d: plan of another device", i);
     if ((pl->G.interp == USTBX_INTERP_CUBIC) != (b->sm == SM_CUBIC))
       return set_status(st, USTBX_E_ARG, "acquisition %d: interpolation differs from the batcher's", i);

@@ -48,9 +48,27 @@ constexpr double kRxTableMaxFraction = 0.5;
 
 bool rxtab_wc_ok(int wc) { return wc == 4 || wc == 5 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
 
-// 8 wave-splits; the smallest waves-per-split covering the busiest tile in
-// one round (else 16 per round).
+// Up to 32 waves: 4 splits of 8 waves (a whole round with enough terms per
+// receive row); more: 8 splits with the smallest waves-per-split covering
+// the busiest tile in one round (else 16 per round).
+void rxtab_default_tiles(int max_nw, int& wc, int& ws);
+
+// Dimension transmit: 8 frames per block (4 when fewer), 4 channels per block.
+void rxtab_defaults(const GeomLaunch& G, int max_nw, int& wc, int& ws) {
+  if (G.dim == 2) {
+    wc = G.F >= 8 ? 8 : 4;
+    ws = 4;
+    return;
+  }
+  rxtab_default_tiles(max_nw, wc, ws);
+}
+
 void rxtab_default_tiles(int max_nw, int& wc, int& ws) {
+  if (max_nw <= 32) {
+    ws = 4;
+    wc = 8;
+    return;
+  }
   ws = 8;
   const int need = (max_nw + 7) / 8;
   wc = need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : 16;
@@ -330,8 +348,10 @@ std::vector<int> band_major_order(const DevScan& sc, int tiles) {
 
 bool rxtab_applies(const ustbx_plan* pl) {
   const GeomLaunch& G = pl->G;
-  if (G.dim != 3 || pl->dasc_order || !(G.separable || G.wd == 0.0f)) return false;
+  if (!(G.dim == 3 || G.dim == 2) || pl->dasc_order || !(G.separable || G.wd == 0.0f)) return false;
   if (G.edge == USTBX_EDGE_PYTHON && G.interp == USTBX_INTERP_LINEAR) return false;
+  // dimension transmit gathers up to 8 frames through one buffer resource
+  if (G.dim == 2 && (int64_t)G.L * G.N * G.W * 8 * 8 > 0x7FFFFE00) return false;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
   if ((size_t)G.N * (size_t)G.P * sizeof(float4) > (size_t)(kRxTableMaxFraction * (double)total_b)) return false;
@@ -341,7 +361,7 @@ bool rxtab_applies(const ustbx_plan* pl) {
 // Variant 3 adds a coefficient buffer of 32 bytes per channel sample (cubic).
 bool coef_applies(const ustbx_plan* pl) {
   const GeomLaunch& G = pl->G;
-  return G.interp == USTBX_INTERP_CUBIC && (int64_t)G.L * G.N * G.W * 32 <= 0x7FFFFE00;
+  return G.dim == 3 && G.interp == USTBX_INTERP_CUBIC && (int64_t)G.L * G.N * G.W * 32 <= 0x7FFFFE00;
 }
 
 // Builds the variant-2 state once; returns false (and frees it) when the
@@ -754,7 +774,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   pl->dasc_order = dasc_order;
   if (rxtab_build(pl, e)) {
     G.variant = 2;
-    rxtab_default_tiles(pl->max_nw, G.wave_chunk, G.split);
+    rxtab_defaults(G, pl->max_nw, G.wave_chunk, G.split);
   } else if (e != hipSuccess) {
     (void)hipFree(pl->tx_table);
     (void)hipFree(pl->dev);
@@ -776,13 +796,13 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) 
     if (!rxtab_build(plan, e)) {
       if (e != hipSuccess) return hip_fail(st, e, "receive table");
       return set_status(st, USTBX_E_UNSUPPORTED,
-                        "the receive-table kernel needs dimension both, the separable (or RF) phase, a receive "
+                        "the receive-table kernels need dimension both or transmit, the separable (or RF) phase, a receive "
                         "table within half the device memory and finite nonzero weights and delays");
     }
     if (variant == 3 && !coef_build(plan, e)) {
       if (e != hipSuccess) return hip_fail(st, e, "coefficient buffer");
       return set_status(st, USTBX_E_UNSUPPORTED,
-                        "the coefficient variant needs cubic interpolation and 32 bytes per sample within 2 GiB "
+                        "the coefficient variant needs dimension both, cubic interpolation and 32 bytes per sample within 2 GiB "
                         "per frame");
     }
   }
@@ -791,7 +811,7 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) 
   plan->G.variant = variant;
   if (same_family) return ok(st);
   if (variant >= 2) {
-    rxtab_default_tiles(plan->max_nw, plan->G.wave_chunk, plan->G.split);
+    rxtab_defaults(plan->G, plan->max_nw, plan->G.wave_chunk, plan->G.split);
   } else if (variant == 1) {
     shared_rx_default_tiles(plan->G.W, plan->G.wave_chunk, plan->G.split);
   } else {
@@ -803,6 +823,14 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) 
 
 int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split, ustbx_status* st) {
   if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (plan->G.variant >= 2 && plan->G.dim == 2) {
+    if (wave_chunk && wave_chunk != 4 && wave_chunk != 8)
+      return set_status(st, USTBX_E_ARG, "frames per block must be 4 or 8");
+    if (split && split != 4 && split != 8) return set_status(st, USTBX_E_ARG, "channels per block must be 4 or 8");
+    if (wave_chunk) plan->G.wave_chunk = wave_chunk;
+    if (split) plan->G.split = split;
+    return ok(st);
+  }
   if (plan->G.variant >= 2) {
     if (wave_chunk && !rxtab_wc_ok(wave_chunk))
       return set_status(st, USTBX_E_ARG, "waves per split must be 4, 5, 6, 8, 10, 12 or 16");

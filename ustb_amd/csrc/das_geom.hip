@@ -8,6 +8,7 @@ namespace ustbx {
 hipError_t launch_das_geom(const GeomLaunch& G, hipStream_t stream) {
   const int ph = (G.wd == 0.0f) ? PH_RF : (G.separable ? PH_SEP : PH_DASC);
   if (G.variant >= 2 && G.dim == MODE_BOTH) return launch_both_rxtab(G, stream);
+  if (G.variant == 2 && G.dim == MODE_TX) return launch_tx_rxtab(G, stream);
   if (G.variant == 1 && G.dim == MODE_BOTH && ph == PH_SEP && G.F == 1)
     return launch_both_shared(G, stream);
   switch (G.dim) {

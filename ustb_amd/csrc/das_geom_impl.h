@@ -530,6 +530,7 @@ hipError_t launch_geom_t(const GeomLaunch& G, hipStream_t stream) {
 // per-mode dispatchers (one translation unit each)
 hipError_t launch_both_shared(const GeomLaunch& G, hipStream_t s);
 hipError_t launch_both_rxtab(const GeomLaunch& G, hipStream_t s);
+hipError_t launch_tx_rxtab(const GeomLaunch& G, hipStream_t s);
 hipError_t launch_geom_both(const GeomLaunch& G, int ph, int wc, int split, hipStream_t s);
 hipError_t launch_geom_tx(const GeomLaunch& G, int ph, int wc, int split, hipStream_t s);
 hipError_t launch_geom_rx_none(const GeomLaunch& G, int ph, int wc, hipStream_t s);
