@@ -43,7 +43,9 @@ def parse():
     ap.add_argument("--slice-ms", type=float, default=2.0, help="C5: estimated work per serving iteration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--phase", default="separable", choices=["separable", "dasc"])
+    ap.add_argument("--phase", default="auto", choices=["auto", "separable", "dasc"],
+                    help="auto: das_c's phase for dimension transmit (its sine and cosine are shared by the "
+                         "frames of a block), the separable phase otherwise")
     ap.add_argument("--wave-chunk", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant (default: the plan's choice)")
@@ -125,6 +127,8 @@ def main():
     print(f"[bench] {acq.name}: data {acq.data.shape} generated in {time.time() - t_gen:.1f}s", file=sys.stderr,
           flush=True)
     ch = torch.from_numpy(acq.data).to(dev)
+    if args.phase == "auto":
+        args.phase = "dasc" if int(acq.dimension) == 2 else "separable"
     plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=(args.phase == "separable"),
                      **acq.plan_kwargs())
     if args.variant >= 0:
@@ -436,8 +440,8 @@ def check_errors(acq, plan, out, ch, torch):
     ref64 = O.beamform_fp64(acq.data, acq.fs, acq.t0, ta, ra, td, rd, acq.fd, int(acq.dimension),
                             interp="cubic" if acq.interpolation == 1 else "linear", pixels=pix, w0=wd)
     res["vs_fp64"] = float(np.nanmax(np.abs(img[..., pix] - ref64)) / peak)
-    # linear mode of the same plan vs das_c
-    lin = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=True,
+    # linear mode of the same plan (same phase arithmetic) vs das_c
+    lin = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=plan.separable,
                     **acq.plan_kwargs(interpolation=0))
     img_lin = lin.run(ch).cpu().numpy()
     lin.close()

@@ -243,11 +243,13 @@ int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
  *   1 = receive state shared in LDS by the wave-splits of a 64-pixel tile
  *       (dimension both, IQ, separable phase, one frame);
  *   2 = resident receive table and per-tile wave/channel lists (dimension
- *       both or transmit, separable or RF phase, any frame count, receive
- *       table within half the device memory, finite nonzero weights and
- *       delays; the default where it applies). For dimension transmit
- *       set_tiles' arguments are frames per block (4 or 8) and channels
- *       per block (4 or 8; one wave64 each);
+ *       both with the separable or RF phase outside das_c's order, or
+ *       dimension transmit with any phase; any frame count, receive table
+ *       within half the device memory, finite nonzero weights and delays;
+ *       the default where it applies). For dimension transmit set_tiles'
+ *       arguments are frames per block (4 or 8) and channels per block (4 or
+ *       8; one wave64 each), and without the separable phase (or with RF
+ *       data) it keeps das_c's term arithmetic and summation order;
  *   3 = variant 2 (dimension both) over cubic-interpolation coefficients: the plan holds a
  *       buffer of 32 bytes per channel sample that every run rewrites, so
  *       runs of one variant-3 plan on different streams must not overlap.

@@ -245,9 +245,63 @@ def test_transmit_table_variant(cuda, name, interp):
     plan.close()
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("dasc_order", [False, True])
+def test_transmit_table_dasc_phase(cuda, name, dasc_order):
+    """Dimension transmit from the resident tables with das_c's phase (and
+    with das_c's order): das_c's term arithmetic and summation order, so the
+    image equals the exact per-lane kernel's (variant 0) bit for bit, and
+    das_c's bit for bit for RF data (within 1e-5 of the peak for IQ, where
+    only the device's and the host's sin/cos differ). 9 frames: the second
+    chunk of 8 holds one frame."""
+    case = CASES[name]
+    F = 9
+    print(f"case tx dasc phase {name} dasc_order={dasc_order}", flush=True)
+    plan = _plan(case, dim=2, F=F, separable_phase=False, dasc_order=dasc_order)
+    applies = plan.has_variant(2)
+    assert applies == (not _nonfinite_weight(case))
+    if not applies:
+        plan.close()
+        return
+    assert plan.variant[0] == 2
+    ch = channel_noise(case, F=F, seed=29)
+    chd = torch.from_numpy(ch).to(cuda)
+    outs = []
+    for fc, cs in ((8, 4), (4, 8)):
+        plan.set_tiles(fc, cs)
+        outs.append(plan.run(chd).cpu().numpy())
+    assert np.array_equal(outs[1], outs[0])
+    ta, ra, td, rd = [t.cpu().numpy() for t in plan.materialize()]
+    plan.set_variant(0)
+    v0 = plan.run(chd).cpu().numpy()
+    ref = O.das_c(ch, case["fs"], 0.0, ta, ra, td, rd, case["fd"], 2)
+    assert np.array_equal(outs[0], v0)
+    if case["fd"] == 0.0:
+        assert np.array_equal(outs[0], ref)
+    else:
+        assert _rel(outs[0], ref) <= TOL
+    plan.close()
+
+
+@pytest.mark.parametrize("name", ["cpwc_tukey", "dw_sector"])
+def test_transmit_table_dasc_phase_cubic(cuda, name):
+    """Cubic interpolation with das_c's phase in dimension transmit: the
+    resident-table kernel against variant 0 and the FP64 oracle taking das_c's
+    FP32 phase argument (no das_c counterpart: das_c is linear)."""
+    case = CASES[name]
+    plan = _plan(case, dim=2, F=3, separable_phase=False, interpolation=int(Interpolation.cubic))
+    assert plan.variant[0] == 2
+    ch = channel_noise(case, F=3, seed=31)
+    chd = torch.from_numpy(ch).to(cuda)
+    got = plan.run(chd).cpu().numpy()
+    plan.set_variant(0)
+    assert _rel(got, plan.run(chd).cpu().numpy()) <= TOL
+    plan.close()
+
+
 def test_receive_table_rejected_modes(cuda):
-    """Variants 2/3 cover dimension both and transmit (3: both only), and not
-    das_c's summation order."""
+    """Variants 2/3 cover dimension both and transmit (3: both only), and
+    dimension both not in das_c's summation order or with das_c's phase."""
     case = CASES["cpwc_tukey"]
     for dim in (0, 1):
         plan = _plan(case, dim=dim, separable_phase=True)
@@ -259,4 +313,7 @@ def test_receive_table_rejected_modes(cuda):
     plan = _plan(case, dim=3, dasc_order=True)
     assert not plan.has_variant(2)
     assert plan.variant[0] == 0
+    plan.close()
+    plan = _plan(case, dim=3, separable_phase=False)
+    assert not plan.has_variant(2)
     plan.close()

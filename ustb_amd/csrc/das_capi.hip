@@ -53,11 +53,12 @@ bool rxtab_wc_ok(int wc) { return wc == 4 || wc == 5 || wc == 6 || wc == 8 || wc
 // the busiest tile in one round (else 16 per round).
 void rxtab_default_tiles(int max_nw, int& wc, int& ws);
 
-// Dimension transmit: 8 frames per block (4 when fewer), 4 channels per block.
+// Dimension transmit: 8 frames per block (4 when fewer), 8 channels per block
+// (4 when fewer).
 void rxtab_defaults(const GeomLaunch& G, int max_nw, int& wc, int& ws) {
-  if (G.dim == 2) {
+  if (G.dim == 2) {  // frames per block, channels per block (measured best on 32 frames x 64 channels)
     wc = G.F >= 8 ? 8 : 4;
-    ws = 4;
+    ws = G.N >= 8 ? 8 : 4;
     return;
   }
   rxtab_default_tiles(max_nw, wc, ws);
@@ -322,21 +323,23 @@ int ustbx_das_run_host(const ustbx_das_desc* d, const void* ch, const float* tx_
 
 namespace {
 
-// Tile order of the receive-table kernel. Blocks go to the 8 XCDs round-robin
-// and each XCD takes a contiguous range of tile indices (the XCD remap), i.e.
-// whole scan lines (depth-fastest pixel order) with every depth, so the XCDs'
-// work stays balanced. Within its range an XCD visits the tiles depth band by
-// depth band across its lines: blocks resident together then cover a compact
-// patch of the image, whose delays span a short window of every trace, and
-// those windows stay in the XCD's L2. Identity where the scan has no
-// line-of-depths layout in whole tiles.
-std::vector<int> band_major_order(const DevScan& sc, int tiles) {
+// Tile order of the receive-table kernels. Dimension both (shares = 8):
+// blocks go to the 8 XCDs round-robin and each XCD takes a contiguous range of
+// tile indices (the XCD remap), i.e. whole scan lines (depth-fastest pixel
+// order) with every depth, so the XCDs' work stays balanced. Within its range
+// an XCD visits the tiles depth band by depth band across its lines: blocks
+// resident together then cover a compact patch of the image, whose delays span
+// a short window of every trace, and those windows stay in the XCD's L2.
+// Dimension transmit (shares = 1): an XCD owns whole (channel group, frame
+// chunk) slabs, each visited depth band by depth band across all lines.
+// Identity where the scan has no line-of-depths layout in whole tiles.
+std::vector<int> band_major_order(const DevScan& sc, int tiles, int shares) {
   std::vector<int> order(tiles);
   for (int t = 0; t < tiles; ++t) order[t] = t;
-  if (!(sc.kind == SCAN_LINEAR || sc.kind == SCAN_SECTOR) || sc.n0 % 64 != 0 || tiles < 8) return order;
+  if (!(sc.kind == SCAN_LINEAR || sc.kind == SCAN_SECTOR) || sc.n0 % 64 != 0 || tiles < shares) return order;
   const int tpl = sc.n0 / 64;  // tiles per line
-  const int q = tiles >> 3, r = tiles & 7;
-  for (int k = 0; k < 8; ++k) {
+  const int q = tiles / shares, r = tiles % shares;
+  for (int k = 0; k < shares; ++k) {
     const int s0 = k * q + std::min(k, r);
     const int cnt = q + (k < r ? 1 : 0);
     std::stable_sort(order.begin() + s0, order.begin() + s0 + cnt, [tpl](int a, int b) {
@@ -348,7 +351,10 @@ std::vector<int> band_major_order(const DevScan& sc, int tiles) {
 
 bool rxtab_applies(const ustbx_plan* pl) {
   const GeomLaunch& G = pl->G;
-  if (!(G.dim == 3 || G.dim == 2) || pl->dasc_order || !(G.separable || G.wd == 0.0f)) return false;
+  if (!(G.dim == 3 || G.dim == 2)) return false;
+  // dimension both needs the separable (or RF) phase outside das_c's order;
+  // the transmit kernel has das_c's term arithmetic and summation order too
+  if (G.dim == 3 && (pl->dasc_order || !(G.separable || G.wd == 0.0f))) return false;
   if (G.edge == USTBX_EDGE_PYTHON && G.interp == USTBX_INTERP_LINEAR) return false;
   // dimension transmit gathers up to 8 frames through one buffer resource
   if (G.dim == 2 && (int64_t)G.L * G.N * G.W * 8 * 8 > 0x7FFFFE00) return false;
@@ -416,7 +422,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
     (void)hipFree(m);
     return false;
   }
-  std::vector<int> order = band_major_order(G.scan, (int)tiles);
+  std::vector<int> order = band_major_order(G.scan, (int)tiles, G.dim == 2 ? 1 : 8);
   err = hipMemcpy(ord, order.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     (void)hipFree(m);
@@ -796,8 +802,9 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) 
     if (!rxtab_build(plan, e)) {
       if (e != hipSuccess) return hip_fail(st, e, "receive table");
       return set_status(st, USTBX_E_UNSUPPORTED,
-                        "the receive-table kernels need dimension both or transmit, the separable (or RF) phase, a receive "
-                        "table within half the device memory and finite nonzero weights and delays");
+                        "the receive-table kernels need dimension transmit, or dimension both with the separable (or RF) "
+                        "phase outside das_c's order, a receive table within half the device memory and finite nonzero "
+                        "weights and delays");
     }
     if (variant == 3 && !coef_build(plan, e)) {
       if (e != hipSuccess) return hip_fail(st, e, "coefficient buffer");

@@ -3,10 +3,13 @@
 // the per-channel images das_c keeps when it sums over transmits only
 // (das_c.cpp:395-399), with the phase taken as exp(i wd tx) * exp(i wd rx).
 //
-// A block is one 64-pixel tile x CS wave64s, one channel each: grid.z runs
-// over groups of CS channels (slowest), grid.y over chunks of FC frames, so
-// the blocks resident together share one channel group and one frame chunk
-// and their sample windows fit in L2. Every channel's image is written (a
+// A block is one 64-pixel tile x CS wave64s, one channel each. The work is
+// slabs (a group of CS channels x a chunk of FC frames) x tiles, slab-major,
+// and each XCD takes a contiguous share of it (the XCD remap): an XCD owns
+// whole slabs and visits a slab's tiles in the plan's order (depth band by
+// depth band across all scan lines), so the blocks resident on an XCD read
+// one slab's traces over one depth band, a window that fits its L2, and the
+// channel data is fetched about once. Every channel's image is written (a
 // channel with no receive weight on the tile gets zeros, das_c's
 // zero-initialized output). Per (channel, wave) the sample index is formed
 // once (das_c's operation order) and the FC frames' samples are gathered
@@ -14,22 +17,37 @@
 // shared by all frames. Terms outside the window (or with a zero weight) read
 // through an out-of-range offset, as in the dimension-both tile body (same
 // finiteness condition on the plan).
+//
+// Phase (PH): PH_SEP takes the tables' exp(i wd td) and exp(i wd rd) and
+// multiplies the receive factor in after the sum. PH_DASC and PH_RF follow
+// das_c's term arithmetic instead (das_c.cpp:372-399): c_delay = td + rd,
+// c_apo = ta * ra, weight = c_apo * exp(i fl32(wd * c_delay)) (PH_RF: c_apo),
+// das_c's 4-term complex dot, added to the running sum over waves in
+// ascending order. The sine and cosine are taken once per (pixel, channel,
+// wave) and shared by the block's frames. That is das_c's order for this
+// mode, so the result equals das_c's (bit for bit for RF data, linear).
 #include "das_rxtab.h"
 
 namespace ustbx {
 
-template <int SM, int FC, int CS>
-__global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, const int* __restrict__ order) {
+template <int SM, int FC, int CS, int PH>
+__global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, float wd,
+                                                               const int* __restrict__ order) {
   using M = SampleMode<SM>;
   constexpr int TP = 64;
   constexpr int ST = M::kStride;
   const int lane = threadIdx.x & (TP - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
-  const int n = (int)blockIdx.z * CS + s;
+  const int tiles = (A.P + TP - 1) / TP;
+  const int chunks = (F + FC - 1) / FC;
+  const int item = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int slab = item / tiles;
+  const int rank = item - slab * tiles;
+  const int group = slab / chunks;
+  const int n = group * CS + s;
   if (n >= A.N) return;  // no block barrier below
-  int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (order) tile = __builtin_amdgcn_readfirstlane(order[tile]);
-  const int f0 = (int)blockIdx.y * FC;
+  const int tile = order ? __builtin_amdgcn_readfirstlane(order[rank]) : rank;
+  const int f0 = (slab - group * chunks) * FC;
   const int nf = min(FC, F - f0);
   const int p = tile * TP + lane;
   const bool active = p < A.P;
@@ -57,6 +75,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, c
       return;
     }
     const float rd = ron ? rv.x : kFar;
+    const float ra = rv.y;
     const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
     const unsigned nb = (unsigned)(n * lst);
     float2 part[FC];
@@ -79,10 +98,26 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, c
       const bool ton = v.y != 0.0f;
       const float td = ton ? v.x : kFar;
       const float2 T = ton ? make_float2(v.z, v.w) : make_float2(0.0f, 0.0f);
-      float d;
+      float d, c_delay;
       {
 #pragma clang fp contract(off)
-        d = ((td + rd) - t0) * fs;  // das_c.cpp:372,375
+        c_delay = td + rd;            // das_c.cpp:372
+        d = (c_delay - t0) * fs;      // das_c.cpp:375
+      }
+      // das_c's weight for this term (zero where either weight is zero)
+      float wr = 0.0f, wi = 0.0f;
+      if (PH != PH_SEP) {
+#pragma clang fp contract(off)
+        const bool on = ton && ron;
+        const float c_apo = on ? v.y * ra : 0.0f;  // das_c.cpp:373
+        if (PH == PH_DASC) {
+          float sv, cv;
+          sincosf(wd * (on ? c_delay : 0.0f), &sv, &cv);  // das_c.cpp:383-387
+          wr = cv * c_apo;
+          wi = sv * c_apo;
+        } else {
+          wr = c_apo;
+        }
       }
       const bool ok = d >= lo_ok && d < lim;
       const int n0 = (int)d;
@@ -101,6 +136,15 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, c
 #pragma unroll
       for (int f = 0; f < FC; ++f) {
         float2 sm;
+        if (PH != PH_SEP && SM == SM_LINEAR) {
+          Term t;
+          t.a = 1.0f - b;
+          t.b = b;
+          t.wr = wr;
+          t.wi = wi;
+          cadd(part[f], dasc_contrib(t, q[f][0]));  // das_c.cpp:389-399
+          continue;
+        }
         if (SM == SM_LINEAR) {
           const float4 x = q[f][0];
           sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
@@ -109,7 +153,12 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, c
           sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
                            fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
         }
-        cfma(part[f], T, sm);
+        if (PH == PH_SEP) {
+          cfma(part[f], T, sm);
+        } else {
+#pragma clang fp contract(off)
+          cadd(part[f], make_float2(wr * sm.x - wi * sm.y, wr * sm.y + wi * sm.x));
+        }
       }
     }
     if (active) {
@@ -117,7 +166,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, c
 #pragma unroll
       for (int f = 0; f < FC; ++f) {
         if (f < nf) {
-          float2 v = cmul(R, part[f]);
+          float2 v = PH == PH_SEP ? cmul(R, part[f]) : part[f];
           if (A.ref) v = cmul(v, refp);
           A.out[((size_t)(f0 + f) * A.N + n) * PS + p] = v;
         }
@@ -126,26 +175,36 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, c
   }
 }
 
-template <int SM, int FC, int CS>
+template <int SM, int FC, int CS, int PH>
 static hipError_t launch_tx_t(const GeomLaunch& G, hipStream_t stream) {
-  const dim3 grid((unsigned)((G.P + 63) / 64), (unsigned)((G.F + FC - 1) / FC), (unsigned)((G.N + CS - 1) / CS));
-  hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, FC, CS>), grid, dim3(64 * CS), 0, stream, rx_acq(G, SM), G.F,
-                     G.tile_order);
+  const int64_t blocks = (int64_t)((G.P + 63) / 64) * ((G.F + FC - 1) / FC) * ((G.N + CS - 1) / CS);
+  if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, FC, CS, PH>), dim3((unsigned)blocks), dim3(64 * CS), 0, stream,
+                     rx_acq(G, SM), G.F, G.wd, G.tile_order);
   return hipGetLastError();
 }
 
-template <int SM>
+template <int SM, int PH>
 static hipError_t launch_tx_sm(const GeomLaunch& G, hipStream_t stream) {
   const int fc = G.wave_chunk, cs = G.split;
-  if (fc == 4) return cs == 4 ? launch_tx_t<SM, 4, 4>(G, stream) : launch_tx_t<SM, 4, 8>(G, stream);
-  if (fc == 8) return cs == 4 ? launch_tx_t<SM, 8, 4>(G, stream) : launch_tx_t<SM, 8, 8>(G, stream);
+  if (fc == 4) return cs == 4 ? launch_tx_t<SM, 4, 4, PH>(G, stream) : launch_tx_t<SM, 4, 8, PH>(G, stream);
+  if (fc == 8) return cs == 4 ? launch_tx_t<SM, 8, 4, PH>(G, stream) : launch_tx_t<SM, 8, 8, PH>(G, stream);
   return hipErrorInvalidValue;
 }
 
-// Frames per block (wave_chunk: 4 or 8) and channels per block (split: 4 or 8).
+template <int SM>
+static hipError_t launch_tx_ph(const GeomLaunch& G, hipStream_t stream) {
+  if (G.wd == 0.0f) return launch_tx_sm<SM, PH_RF>(G, stream);
+  if (G.separable) return launch_tx_sm<SM, PH_SEP>(G, stream);
+  return launch_tx_sm<SM, PH_DASC>(G, stream);
+}
+
+// Frames per block (wave_chunk: 4 or 8) and channels per block (split: 4 or
+// 8); the phase follows the plan (RF data and das_c's phase: das_c's term
+// arithmetic; otherwise the separable phase).
 hipError_t launch_tx_rxtab(const GeomLaunch& G, hipStream_t stream) {
-  if (G.interp == INTERP_CUBIC) return launch_tx_sm<SM_CUBIC>(G, stream);
-  return launch_tx_sm<SM_LINEAR>(G, stream);
+  if (G.interp == INTERP_CUBIC) return launch_tx_ph<SM_CUBIC>(G, stream);
+  return launch_tx_ph<SM_LINEAR>(G, stream);
 }
 
 }  // namespace ustbx
