@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--phase", default="auto", choices=["auto", "separable", "dasc"],
                     help="auto: das_c's phase for dimension transmit (its sine and cosine are shared by the "
                          "frames of a block), the separable phase otherwise")
+    ap.add_argument("--abi", default="geometry", choices=["geometry", "matrix"],
+                    help="C1-C3: geometry plan (delays and apodization from resident tables) or the das_c "
+                         "matrix contract (ustbx_das_run over das.m's FP32 matrices in HBM, the plain kernel)")
     ap.add_argument("--wave-chunk", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant (default: the plan's choice)")
@@ -121,6 +124,9 @@ def main():
         return
     if args.config == "C5":
         run_serve(args, torch, dev)
+        return
+    if args.abi == "matrix":
+        run_matrix(args, torch, dev)
         return
     t_gen = time.time()
     acq = configs.BUILDERS[args.config]()
@@ -215,6 +221,90 @@ def main():
     print(json.dumps(result))
 
 
+def run_matrix(args, torch, dev):
+    """The das_c / das_cuda contract on device buffers (ustbx_das_run): das.m's
+    FP32 delay and apodization matrices, computed once and resident in HBM,
+    read by the plain kernel on every call. Algorithmic bytes per call: channel
+    data + the four matrices + the image, each once."""
+    from oracle import oracle as O
+    from ustb_amd import configs, ops
+    from ustb_amd.geometry import make_plan
+
+    acq = configs.BUILDERS[args.config]()
+    ch = torch.from_numpy(acq.data).to(dev)
+    plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=False, **acq.plan_kwargs())
+    ta, ra, td, rd = plan.materialize()
+    plan.close()
+    dim = int(acq.dimension)
+    interp = int(acq.interpolation)
+    out = torch.empty(ops.out_shape(dim, acq.P, acq.probe.N_elements, len(acq.sequence), acq.n_frames),
+                      dtype=torch.complex64, device=dev)
+
+    def call():
+        ops.das(ch, acq.fs, acq.t0, ta, ra, td, rd, acq.fd, dim, interp=interp, split=args.split, out=out)
+
+    for _ in range(args.warmup):
+        call()
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        call()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    value = acq.units / (ms_per_step * 1e-3) / 1e9
+    mat_bytes = 4 * (ta.numel() + ra.numel() + td.numel() + rd.numel())
+    alg_bytes = acq.data.nbytes + mat_bytes + out.numel() * 8
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    copy = copy_probe(torch, dev)
+    log(f"matrix ABI {args.steps} steps: {ms_per_step:.4f} ms/step (kernel {kernel_ms:.4f} ms), "
+        f"{achieved:.0f} GB/s of {copy['GBs']:.0f} achievable")
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"das_matrix_{args.config.lower()}_traffic.txt")
+    if os.path.exists(tpath):
+        for line in open(tpath):
+            if line.startswith("traffic_bytes_per_launch"):
+                traffic = float(line.split("=")[1])
+    result = {
+        "metric": "DAS Gsamples/s (pixel x channel x wave)",
+        "value": round(value, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": acq.name, "abi": "matrix (ustbx_das_run, das_c contract)", "pixels": acq.P,
+                   "channels": acq.probe.N_elements, "waves": len(acq.sequence), "frames": acq.n_frames,
+                   "samples": acq.n_samples, "interp": "cubic" if interp == 1 else "linear", "dimension": dim,
+                   "split": args.split or "auto"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel_ms": round(kernel_ms, 4),
+                     "algorithmic_bytes": alg_bytes, "achievable_copy_GBs": copy["GBs"],
+                     "frac_of_achievable": round(achieved / copy["GBs"], 4)},
+    }
+    if not args.no_check:
+        pix = _sample_pixels(acq.P, 2048, seed=5)
+        sel = lambda t: np.ascontiguousarray(t.cpu().numpy()[:, pix])
+        ref = O.das_c(acq.data, acq.fs, acq.t0, sel(ta), sel(ra), sel(td), sel(rd), acq.fd, dim)
+        got = out.cpu().numpy()[..., pix]
+        peak = float(np.nanmax(np.abs(ref)))
+        result["errors"] = {"pixels_sampled": int(pix.size),
+                            "vs_dasc": float(np.nanmax(np.abs(got - ref)) / peak),
+                            "bitwise_vs_dasc": bool(np.array_equal(got, ref, equal_nan=True))}
+        log(f"errors {result['errors']}")
+    print(json.dumps(result))
+
+
 def run_batch(args, torch, dev):
     """C4: the ragged batch in one persistent launch (hipGraph replay) against
     the plain loop of one launch per acquisition on the same buffers."""
@@ -254,7 +344,7 @@ def run_batch(args, torch, dev):
     # tile shape: the default (work-weighted) and neighbours, timed on the batch
     shapes = {}
     default = (b.shape["wave_chunk"], b.shape["split"])
-    for wc, ws in dict.fromkeys([default, (8, 4), (4, 8), (5, 5), (6, 4), (10, 4)]):
+    for wc, ws in dict.fromkeys([default, (8, 4), (16, 4), (16, 8), (8, 8), (4, 8), (5, 5), (6, 4), (10, 4)]):
         b.set_tiles(wc, ws)
         b.launch()
         shapes[(wc, ws)] = timed(b.launch, 1)[0]

@@ -27,6 +27,14 @@ float dasc_wd(float fd) {
 
 int auto_split(int P) { return P >= (1 << 18) ? 1 : 4; }
 
+// Channel groups of the matrix kernel: enough waves to fill the chip, P/64 x
+// split >= 8192 (32 per CU): 8 at 64 K pixels, 1 from 512 K.
+int matrix_auto_split(int P) {
+  int s = 1;
+  while (s < 8 && (int64_t)P * s < (int64_t)8192 * 64) s *= 2;
+  return s;
+}
+
 // The shared-receive-state kernel covers dimension both with the separable
 // phase on one frame.
 bool shared_rx_applies(const GeomLaunch& G) { return G.dim == 3 && G.separable && G.F == 1 && G.wd != 0.0f; }
@@ -48,9 +56,10 @@ constexpr double kRxTableMaxFraction = 0.5;
 
 bool rxtab_wc_ok(int wc) { return wc == 4 || wc == 5 || wc == 6 || wc == 8 || wc == 10 || wc == 12 || wc == 16; }
 
-// Up to 32 waves: 4 splits of 8 waves (a whole round with enough terms per
-// receive row); more: 8 splits with the smallest waves-per-split covering
-// the busiest tile in one round (else 16 per round).
+// Up to 8 waves: 8 splits of 4 waves, most of them in channel groups; up to
+// 32 waves: 4 splits of 8 waves (a whole round with enough terms per receive
+// row); more: 8 splits with the smallest waves-per-split covering the busiest
+// tile in one round (else 16 per round).
 void rxtab_default_tiles(int max_nw, int& wc, int& ws);
 
 // Dimension transmit: 8 frames per block (4 when fewer), 8 channels per block
@@ -65,6 +74,11 @@ void rxtab_defaults(const GeomLaunch& G, int max_nw, int& wc, int& ws) {
 }
 
 void rxtab_default_tiles(int max_nw, int& wc, int& ws) {
+  if (max_nw <= 8) {  // 1-2 wave-splits; the others take shares of the channels
+    ws = 8;
+    wc = 4;
+    return;
+  }
   if (max_nw <= 32) {
     ws = 4;
     wc = 8;
@@ -259,7 +273,7 @@ int ustbx_das_run(const ustbx_das_desc* d, const void* ch, const float* tx_apo,
   m.dim = d->dim;
   m.interp = d->interp;
   m.edge = d->edge;
-  m.split = (d->flags & USTBX_FLAG_DASC_ORDER) ? 1 : (d->split ? d->split : auto_split(d->P));
+  m.split = (d->flags & USTBX_FLAG_DASC_ORDER) ? 1 : (d->split ? d->split : matrix_auto_split(d->P));
   if (m.split != 1 && m.split != 2 && m.split != 4 && m.split != 8)
     return set_status(st, USTBX_E_ARG, "split must be 1, 2, 4 or 8");
   hipError_t e = launch_das_matrix(m, s);

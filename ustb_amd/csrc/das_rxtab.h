@@ -147,13 +147,14 @@ template <int SM, int WC, bool GUARD>
 __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
                                                const float2 (&T)[WC], const int (&wb)[WC], int nj,
                                                const float4* __restrict__ rxt, const int* __restrict__ chl,
-                                               int nc, size_t PS, int p, bool active, int lst, float t0, float fs,
-                                               float lim, float2& acc) {
-  // receive row of the next channel in flight during this one; its index
-  // one further ahead (a scalar load off the critical path)
+                                               int cstride, int nc, size_t PS, int p, bool active, int lst,
+                                               float t0, float fs, float lim, float2& acc) {
+  // channels chl[0], chl[cstride], ... (nc of them); the receive row of the
+  // next one in flight during this one, its index one further ahead (a
+  // scalar load off the critical path)
   float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
   int n = __builtin_amdgcn_readfirstlane(chl[0]);
-  int n_next = nc > 1 ? __builtin_amdgcn_readfirstlane(chl[1]) : 0;
+  int n_next = nc > 1 ? __builtin_amdgcn_readfirstlane(chl[cstride]) : 0;
   if (active) rn = rxt[(size_t)n * PS + p];
   for (int k = 0; k < nc; ++k) {
     const float4 rv = rn;
@@ -161,7 +162,7 @@ __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const 
     if (k + 1 < nc) {
       n = n_next;
       if (active) rn = rxt[(size_t)n * PS + p];
-      if (k + 2 < nc) n_next = __builtin_amdgcn_readfirstlane(chl[k + 2]);
+      if (k + 2 < nc) n_next = __builtin_amdgcn_readfirstlane(chl[(k + 2) * cstride]);
     }
     const bool ron = rv.y != 0.0f;
     const float rd = ron ? rv.x : kFar;
@@ -193,10 +194,22 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
       make_rsrc((const char*)A.data + (size_t)f * A.frame_bytes, A.frame_bytes);
   const float lim = (float)(SM == SM_LINEAR ? A.L - 1 : A.L - 2);
 
+  // The splits share out the tile's waves round-robin, WC per split and round.
+  // When the waves need fewer splits than WS in one round (wneed), the block
+  // forms cg = WS / wneed channel groups of wneed splits: group g takes the
+  // channels chl[g], chl[g + cg], ... and every wave (one round). All of it is
+  // wave-uniform; the reduction below sums the splits' partials in order.
+  const int wneed = (nw + WC - 1) / WC;
+  const int cg = (wneed > 0 && wneed < WS) ? WS / wneed : 1;
+  const int wsp = cg > 1 ? wneed : WS;  // splits per channel group
+  const int g = s / wsp;
+  const int sw = s - g * wsp;           // split within the group
+  const int ncg = g < cg ? (nc - g + cg - 1) / cg : 0;
+
   float2 acc = make_float2(0.0f, 0.0f);
-  for (int k0 = 0; nc > 0 && k0 < nw; k0 += WS * WC) {
+  for (int k0 = 0; ncg > 0 && k0 < nw; k0 += wsp * WC) {
     // number of this split's waves in the round (wave-uniform)
-    const int nj = min(WC, max(0, (nw - k0 - s + WS - 1) / WS));
+    const int nj = min(WC, max(0, (nw - k0 - sw + wsp - 1) / wsp));
     if (nj == 0) break;
     float td[WC];
     float2 T[WC];
@@ -207,7 +220,7 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
       T[j] = make_float2(0.0f, 0.0f);
       wb[j] = 0;
       if (j < nj) {
-        const int w = __builtin_amdgcn_readfirstlane(wl[k0 + s + WS * j]);
+        const int w = __builtin_amdgcn_readfirstlane(wl[k0 + sw + wsp * j]);
         wb[j] = w * wst;
         if (active) {
           const float4 v = txt[(size_t)w * PS + p];
@@ -219,9 +232,11 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
       }
     }
     if (nj == WC)
-      rxtab_channels<SM, WC, false>(rs, td, T, wb, nj, A.rx, chl, nc, PS, p, active, lst, A.t0, A.fs, lim, acc);
+      rxtab_channels<SM, WC, false>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst, A.t0, A.fs, lim,
+                                    acc);
     else
-      rxtab_channels<SM, WC, true>(rs, td, T, wb, nj, A.rx, chl, nc, PS, p, active, lst, A.t0, A.fs, lim, acc);
+      rxtab_channels<SM, WC, true>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst, A.t0, A.fs, lim,
+                                   acc);
   }
   red[s][lane] = acc;
   __syncthreads();
