@@ -144,7 +144,8 @@ const char* ustbx_status_id(int32_t code);
 
 /* spherical_transmit_delay_model (spherical_transmit_delay_model.m:32-37) */
 #define USTBX_TXMODEL_SPHERICAL 1
-#define USTBX_TXMODEL_UNIFIED 2 /* not on the device: pass tx_delay_matrix */
+#define USTBX_TXMODEL_UNIFIED 2 /* computed once per plan (tools.calculate_unified_delay_model;
+                                   2-D linear and sector scans) */
 #define USTBX_TXMODEL_HYBRID 3
 #define USTBX_TXMODEL_BLENDED 4
 

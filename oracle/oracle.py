@@ -12,7 +12,8 @@ Contents
 - ``python_beamform``: restatement of the Python reference's
   DAS._beamform (python/src/ustb/midprocess/das.py:203-261) in NumPy FP32.
 - geometry in FP64 rounded to FP32 where das.m applies single():
-  ``rx_delay`` (das.m:103-106), ``tx_delay`` (das.m:110-206),
+  ``rx_delay`` (das.m:103-106), ``tx_delay`` (das.m:110-206) with
+  ``unified_delay`` (calculate_unified_delay_model.m:14-100),
   ``window`` (apodization.m:172-186), ``rx_apod`` (apodization.m:323-468),
   ``tx_apod`` (apodization.m:249-320, 472-555), pixel grids
   (linear_scan.m:64-82, sector_scan.m:73-121), element positions
@@ -321,11 +322,84 @@ def plane_delay(px, py, pz, az, el):
     return ((pz * ca) * ce + (px * sa) * ce) + py * se
 
 
+def unified_delay(tx_delay_col, mask, x_matrix, z_matrix, src_x, src_z, src_distance, dirtest):
+    """tools.calculate_unified_delay_model (calculate_unified_delay_model.m:14-100),
+    FP64. tx_delay_col: the signed source distance per pixel, (-1)^(z<src.z)
+    * |src - x| (das.m:120); mask: the wave's boxcar transmit mask; x_matrix,
+    z_matrix: pixel coordinates. All (N_depth, N_lines), depth fastest in the
+    scan's pixel order. Returns the (N_depth, N_lines) delay distance, the
+    source distance added.
+
+    The branch order and the quick fix are the reference's: the mask's last row
+    is set true before the per-line loop (lines 34-37), so the first branch
+    (line 50) always holds and the two elseif branches (lines 74-82) and their
+    fills (lines 85-90) never run; they are kept as written."""
+    tx_delay = np.asarray(tx_delay_col, dtype=np.float64)
+    mask = np.array(mask, dtype=bool)
+    n_depth, n_lines = tx_delay.shape
+    # lines 34-37: quick fix where the beam does not cover the bottom row
+    botinds = ~mask[-1, :]
+    maxval = np.max(tx_delay[-1, :] * mask[-1, :])
+    masked_delays = mask * tx_delay
+    mask[-1, :] = True
+    masked_delays[-1, botinds] = maxval
+    interpolated = np.zeros_like(tx_delay)
+    mask_before = np.zeros(tx_delay.shape, dtype=bool)
+    mask_after = np.zeros(tx_delay.shape, dtype=bool)
+    last_before_idx = None
+    first_after_idx = None
+    for x in range(n_lines):
+        z_idx_focus = int(np.argmin(np.abs(z_matrix[:, x] - src_z)))  # line 49
+        if np.sum(mask[z_idx_focus:, x] > 0):
+            ray = masked_delays[:, x]
+            pos = ray.copy()                      # lines 52-55
+            pos[ray > 0] = 0
+            pos[pos == 0] = -np.inf
+            neg = ray.copy()                      # lines 56-58
+            neg[ray < 0] = 0
+            neg[neg == 0] = np.inf
+            idx_a = int(np.argmax(pos))           # lines 60-61 (first index of the max)
+            idx_b = int(np.argmin(neg))
+            dx = x_matrix[idx_b, x] - x_matrix[idx_a, x]
+            dz = z_matrix[idx_b, x] - z_matrix[idx_a, x]
+            nrm = math.sqrt(dx * dx + dz * dz)    # norm(pos_b - pos_a)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                w1 = np.sqrt((z_matrix[idx_a, x] - z_matrix[:, x]) ** 2) / nrm   # lines 68-71
+                w1 = w1 / w1[idx_b]
+                w2 = np.sqrt((z_matrix[idx_b, x] - z_matrix[:, x]) ** 2) / nrm
+                w2 = w2 / w2[idx_a]
+                interpolated[:, x] = w1 * tx_delay[idx_b, x] + w2 * tx_delay[idx_a, x]   # line 72
+        elif np.sum(masked_delays[z_idx_focus:, x] == 0) and (x_matrix[z_idx_focus, x] - src_x) * dirtest < 0:
+            last_before_idx = x
+            mask_before[:, x] = True
+        elif np.sum(masked_delays[z_idx_focus:, x] == 0) and (x_matrix[z_idx_focus, x] - src_x) * dirtest > 0:
+            if first_after_idx is None:
+                first_after_idx = x
+            mask_after[:, x] = True
+    if mask_before.any():                         # lines 85-90
+        interpolated[:, mask_before[0]] = interpolated[:, [last_before_idx + 1]]
+    if mask_after.any():
+        interpolated[:, mask_after[0]] = interpolated[:, [first_after_idx - 1]]
+    interpolated[mask] = tx_delay[mask]           # line 93
+    interpolated[np.isinf(interpolated)] = 0      # line 94
+    return interpolated + src_distance            # line 96
+
+
 def tx_delay(px, py, pz, waves, scan_kind="linear", model=MODEL_HYBRID, pw_margin=1e-3,
-             blending_power=0.5):
+             blending_power=0.5, scan_shape=None, tx_f_number=(1.0, 1.0), dirtest=1.0):
     """Transmit delay (das.m:110-206). waves: dicts with wavefront, src (xyz),
-    distance, azimuth, elevation, sound_speed, delay. Returns (W, P) float32."""
+    distance, azimuth, elevation, sound_speed, delay. Returns (W, P) float32.
+    The unified model needs scan_shape = (N_depth, N_lines) of a 2-D linear or
+    sector scan and the transmit apodization's f-number (das.m:134-152)."""
     out = np.zeros((len(waves), px.size), dtype=np.float32)
+    unified_mask = None
+    if model == MODEL_UNIFIED:
+        if scan_kind not in ("linear", "sector") or scan_shape is None:
+            raise ValueError("Only linear scan and sector scan in 2D is supported for the unified spherical "
+                             "transmit delay model.")
+        # das.m:136-149: a boxcar transmit apodization with no minimum aperture
+        m = tx_apod(px, py, pz, waves, WIN_BOXCAR, f_number=tx_f_number, minimum_aperture=(0.0, 0.0))
+        unified_mask = np.nan_to_num(m) != 0
     for w, wv in enumerate(waves):
         sx, sy, sz = wv["src"]
         dist = wv["distance"]
@@ -356,8 +430,11 @@ def tx_delay(px, py, pz, waves, scan_kind="linear", model=MODEL_HYBRID, pw_margi
                     nd = np.minimum(np.abs(dist - rr) / dist, 1.0)
                     npw = nd ** blending_power
                     v = (d + dist) * npw + pd * (1.0 - npw)
-            else:
-                raise ValueError("unified model is not restated")
+            else:  # MODEL_UNIFIED (das.m:133-152)
+                nd, nl = scan_shape
+                shape = lambda a: np.reshape(a, (nd, nl), order="F")
+                v = unified_delay(shape(d), shape(unified_mask[w]), shape(px), shape(pz), sx, sz, dist,
+                                  dirtest).ravel(order="F")
         out[w] = (v / wv["sound_speed"] - wv["delay"]).astype(np.float32)
     return out
 

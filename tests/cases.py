@@ -77,6 +77,15 @@ def geom_cases():
                                     model=4, scan_kind="linear")
     cases["focused_spherical"] = dict(cases["focused_blended"], model=1)
     cases["focused_hybrid"] = dict(cases["focused_blended"], model=3)
+    cases["focused_unified"] = dict(cases["focused_blended"], model=2)
+    # phased array, focused steered transmits, sector scan, unified model
+    prb = LinearArray(16, 0.2e-3)
+    ang = np.deg2rad(np.linspace(-30, 30, 5))
+    seq = [Wave.spherical(20e-3 * math.sin(a), 0.0, 20e-3 * math.cos(a), sound_speed=c) for a in ang]
+    scan = SectorScan(np.deg2rad(np.linspace(-35, 35, 21)), np.linspace(2e-3, 40e-3, 37))
+    cases["focused_unified_sector"] = dict(probe=prb, scan=scan, seq=seq, rx=Apodization(Window.hanning, f_number=1.4),
+                                           tx=Apodization(Window.none, f_number=1.5), fs=10e6, fd=2.5e6, L=600,
+                                           model=2, scan_kind="sector")
     # 3-D: matrix array, diverging waves, 3-D linear scan (C4c in small)
     prb = MatrixArray(6, 5, 0.3e-3)
     seq = [Wave.spherical(x, y, -8e-3, sound_speed=c, origin=(x, y, -8e-3)) for x in (-2e-3, 2e-3)
@@ -103,7 +112,8 @@ def oracle_matrices(case, lens_delay=0.0, pw_margin=1e-3, blending_power=0.5):
     tx = case["tx"]
     ta = O.tx_apod(px, py, pz, waves, int(tx.window), tx.f_number, tx.minimum_aperture, tx.maximum_aperture,
                    tx.tilt, tx.grating_lobe_angle)
-    td = O.tx_delay(px, py, pz, waves, case["scan_kind"], case["model"], pw_margin, blending_power)
+    td = O.tx_delay(px, py, pz, waves, case["scan_kind"], case["model"], pw_margin, blending_power,
+                    scan_shape=scan.shape[:2], tx_f_number=tx.f_number)
     return ta, ra, td, rd
 
 

@@ -67,6 +67,80 @@ def test_focused_models():
     assert np.allclose(hyb, sph, rtol=1e-6)  # on axis the plane patch equals the spherical delay
 
 
+def _unified_inputs(name):
+    """A focused case's FP64 inputs to the unified model: the signed source
+    distance (closed form, das.m:120), the boxcar mask das.m builds
+    (das.m:136-150, through the oracle's pinned transmit apodization) and the
+    pixel grids, each (N_depth, N_lines, W)."""
+    from tests.cases import geom_cases
+
+    case = geom_cases()[name]
+    scan, seq = case["scan"], case["seq"]
+    nd, nl = scan.shape[:2]
+    waves = [wave_dict(w) for w in seq]
+    m = O.tx_apod(scan.x, scan.y, scan.z, waves, O.WIN_BOXCAR, f_number=case["tx"].f_number,
+                  minimum_aperture=(0.0, 0.0))
+    mask = np.nan_to_num(m) != 0
+    grid = lambda a: np.reshape(a, (nd, nl), order="F")
+    out = []
+    for w, wv in enumerate(waves):
+        s = np.asarray(wv["src"])
+        d = np.where(scan.z < s[2], -1.0, 1.0) * np.sqrt((s[0] - scan.x) ** 2 + (s[1] - scan.y) ** 2 +
+                                                        (s[2] - scan.z) ** 2)
+        out.append((grid(d), grid(mask[w]), grid(scan.x), grid(scan.z), wv))
+    return case, out
+
+
+@pytest.mark.parametrize("name", ["focused_unified", "focused_unified_sector"])
+def test_unified_model_is_spherical_inside_the_cone(name):
+    """Inside the boxcar cone (and on the bottom row, which the reference's
+    quick fix counts as inside) the unified model is the conventional virtual
+    source delay |s - x| + s.distance with the sign of das.m:120."""
+    _, per_wave = _unified_inputs(name)
+    for d, mask, xm, zm, wv in per_wave:
+        u = O.unified_delay(d, mask, xm, zm, wv["src"][0], wv["src"][2], wv["distance"], 1.0)
+        inside = mask.copy()
+        inside[-1, :] = True
+        assert inside.sum() < inside.size  # some pixels lie outside
+        np.testing.assert_array_equal(u[inside], d[inside] + wv["distance"])
+
+
+@pytest.mark.parametrize("name", ["focused_unified", "focused_unified_sector"])
+def test_unified_model_interpolates_across_the_focus(name):
+    """Outside the cone, on a line whose cone holds pixels before and after
+    the focus, the delay is the straight line in depth through the cone's
+    last pixel before the focus (the largest negative delay) and its first
+    pixel after it (numpy.interp between them, Nguyen & Prager's model)."""
+    _, per_wave = _unified_inputs(name)
+    checked = 0
+    for d, mask, xm, zm, wv in per_wave:
+        u = O.unified_delay(d, mask, xm, zm, wv["src"][0], wv["src"][2], wv["distance"], 1.0)
+        for x in range(d.shape[1]):
+            col, m = d[:, x], mask[:, x]
+            if not (m[-1] and np.any(m & (col < 0)) and np.any(m & (col > 0))):
+                continue
+            neg = np.where(m & (col < 0), col, -np.inf)
+            pos = np.where(m & (col > 0), col, np.inf)
+            a, b = int(np.argmax(neg)), int(np.argmin(pos))
+            za, zb = zm[a, x], zm[b, x]
+            between = (~m) & (zm[:, x] > min(za, zb)) & (zm[:, x] < max(za, zb))
+            if not between.any():
+                continue
+            lo, hi = (a, b) if za < zb else (b, a)
+            ref = np.interp(zm[between, x], [zm[lo, x], zm[hi, x]], [col[lo], col[hi]]) + wv["distance"]
+            np.testing.assert_allclose(u[between, x], ref, rtol=1e-12, atol=1e-15)
+            checked += int(between.sum())
+    assert checked > 0
+
+
+def test_unified_model_scan_support():
+    """das.m:148-149: only 2-D linear and sector scans."""
+    px = py = pz = np.zeros(4)
+    w = wave_dict(Wave.spherical(0.0, 0.0, 10e-3))
+    with pytest.raises(ValueError, match="Only linear scan and sector scan"):
+        O.tx_delay(px, py, pz, [w], "xyz", O.MODEL_UNIFIED, scan_shape=(2, 2))
+
+
 @pytest.mark.parametrize("win,r,v", [
     (O.WIN_TUKEY25, 0.0, 1.0), (O.WIN_TUKEY25, 0.375, 1.0), (O.WIN_TUKEY25, 0.4375, 0.5),
     (O.WIN_TUKEY25, 0.5, 0.0), (O.WIN_TUKEY50, 0.375, 0.5), (O.WIN_HANNING, 0.0, 1.0),

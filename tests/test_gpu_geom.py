@@ -299,6 +299,19 @@ def test_transmit_table_dasc_phase_cubic(cuda, name):
     plan.close()
 
 
+def test_unified_model_needs_a_2d_scan(cuda):
+    """das.m:148-149: the unified model covers 2-D linear and sector scans."""
+    from ustb_amd import _lib as L
+
+    case = CASES["matrix3d_dw"]
+    seq = [type(w).spherical(0.0, 0.0, 10e-3) for w in case["seq"][:1]]
+    with pytest.raises(L.USTBXError, match="Only linear scan and sector scan"):
+        make_plan(case["probe"], case["scan"], seq, n_samples=case["L"], n_frames=1,
+                  sampling_frequency=case["fs"], initial_time=0.0, modulation_frequency=case["fd"],
+                  sound_speed=1540.0, receive_apodization=case["rx"], transmit_apodization=case["tx"],
+                  dimension=3, tx_model=2)
+
+
 def test_receive_table_rejected_modes(cuda):
     """Variants 2/3 cover dimension both and transmit (3: both only), and
     dimension both not in das_c's summation order or with das_c's phase."""

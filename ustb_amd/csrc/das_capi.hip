@@ -614,6 +614,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     return set_status(st, USTBX_E_UNSUPPORTED, "Unknown apodization type!");
 
   std::vector<DevWave> waves(W);
+  bool has_unified = false;
   for (int w = 0; w < W; ++w) {
     const ustbx_wave& s = g->waves[w];
     DevWave& d = waves[w];
@@ -650,32 +651,25 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
             else
               d.txk = sc.kind == USTBX_SCAN_LINEAR ? TXK_BLENDED_LIN : TXK_BLENDED_SEC;
             break;
+          case USTBX_TXMODEL_UNIFIED:
+            if (!((sc.kind == USTBX_SCAN_LINEAR || sc.kind == USTBX_SCAN_SECTOR) && sc.n2 == 1))
+              return set_status(st, USTBX_E_UNSUPPORTED,
+                                "Only linear scan and sector scan in 2D is supported for the unified spherical "
+                                "transmit delay model.");
+            d.txk = TXK_UNIFIED;
+            has_unified = true;
+            break;
           default:
-            return set_status(st, USTBX_E_UNSUPPORTED,
-                              "unified transmit delay model: pass tx_delay_matrix");
+            return set_status(st, USTBX_E_ARG, "unknown spherical transmit delay model");
         }
       }
     } else {
       return set_status(st, USTBX_E_ARG, "Unknown wavefront. Check available options at uff.wavefront");
     }
-    // apodization kind
-    if (ta.matrix) {
-      d.apk = TXAP_MATRIX;
-    } else if (ta.vector) {
-      d.apk = TXAP_CONST;
-      d.apo_const = ta.vector[w];
-    } else if (ta.window == USTBX_WIN_NONE) {
-      d.apk = TXAP_CONST;
-      d.apo_const = 1.0f;
-    } else if (s.wavefront == USTBX_WAVE_PLANE || isinf(s.src_distance)) {
-      // plane wave: constant per wave (apodization.m:481-487)
-      const double tt = tan(s.azimuth - ta.tilt[0]);
-      const double tp = tan(s.elevation - ta.tilt[1]);
-      const double rt = fabs(ta.f_number[0] * tt), rp = fabs(ta.f_number[1] * tp);
-      d.apk = TXAP_CONST;
-      d.apo_const = (float)(win_eval(ta.window, rt) * win_eval(ta.window, rp));
-    } else {
-      d.apk = TXAP_INCIDENCE;
+    // incidence geometry of a wave with a finite source (apodization.m:489-552),
+    // used by the transmit apodization and by the unified model's mask
+    const bool plane_like = s.wavefront == USTBX_WAVE_PLANE || isinf(s.src_distance);
+    if (!plane_like) {
       double OS[3] = {s.src[0] - s.origin[0], s.src[1] - s.origin[1], s.src[2] - s.origin[2]};
       const double nrm = sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2]);
       if (nrm <= 2.220446049250313e-16) {  // STA (apodization.m:506-518), flat probes
@@ -694,6 +688,25 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
       d.yu0 = yu[0]; d.yu1 = yu[1]; d.yu2 = yu[2];
       d.os_az = atan2(OS[0], OS[2]);
       d.os_el = atan2(OS[1], OS[2]);
+    }
+    // apodization kind
+    if (ta.matrix) {
+      d.apk = TXAP_MATRIX;
+    } else if (ta.vector) {
+      d.apk = TXAP_CONST;
+      d.apo_const = ta.vector[w];
+    } else if (ta.window == USTBX_WIN_NONE) {
+      d.apk = TXAP_CONST;
+      d.apo_const = 1.0f;
+    } else if (plane_like) {
+      // plane wave: constant per wave (apodization.m:481-487)
+      const double tt = tan(s.azimuth - ta.tilt[0]);
+      const double tp = tan(s.elevation - ta.tilt[1]);
+      const double rt = fabs(ta.f_number[0] * tt), rp = fabs(ta.f_number[1] * tp);
+      d.apk = TXAP_CONST;
+      d.apo_const = (float)(win_eval(ta.window, rt) * win_eval(ta.window, rp));
+    } else {
+      d.apk = TXAP_INCIDENCE;
     }
   }
   const size_t off_waves = ar.add(waves.data(), sizeof(DevWave) * W);
@@ -773,10 +786,41 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     cull = g->elem_x[n] > g->elem_x[n - 1] && g->elem_y[n] == g->elem_y[0] && g->elem_z[n] == g->elem_z[0];
   G.rx_cull = cull ? 1 : 0;
 
+  // unified-model delays (das.m:133-152) into the plan's [W][P] matrix
+  if (has_unified) {
+    pl->unified_bytes = (size_t)W * (size_t)P * sizeof(float);
+    e = hipMalloc(&pl->unified_tx, pl->unified_bytes);
+    if (e == hipSuccess) {
+      // das.m:136-149: boxcar, no minimum aperture, the transmit apodization's
+      // f-number, uff.apodization's other defaults
+      DevTxApod mk;
+      memset(&mk, 0, sizeof(mk));
+      mk.window = USTBX_WIN_BOXCAR;
+      mk.f0 = ta.f_number[0];
+      mk.f1 = ta.f_number[1];
+      mk.min0 = 0.0 * ta.f_number[0];
+      mk.min1 = 0.0 * ta.f_number[1];
+      mk.max0 = 10.0 * ta.f_number[0];
+      mk.max1 = 10.0 * ta.f_number[1];
+      mk.gl0 = 0.5 * M_PI;
+      mk.gl1 = 0.5 * M_PI;
+      G.tx_delay_mat = pl->unified_tx;
+      e = launch_geom_unified_tx(G, mk, pl->unified_tx, nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    if (e != hipSuccess) {
+      (void)hipFree(pl->unified_tx);
+      (void)hipFree(pl->dev);
+      delete pl;
+      return hip_fail(st, e, "unified transmit delay model");
+    }
+  }
+
   // resident transmit table
   pl->tx_table_bytes = (size_t)W * (size_t)P * sizeof(float4);
   e = hipMalloc(&pl->tx_table, pl->tx_table_bytes);
   if (e != hipSuccess) {
+    (void)hipFree(pl->unified_tx);
     (void)hipFree(pl->dev);
     delete pl;
     return hip_fail(st, e, "hipMalloc (transmit table)");
@@ -786,6 +830,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     (void)hipFree(pl->tx_table);
+    (void)hipFree(pl->unified_tx);
     (void)hipFree(pl->dev);
     delete pl;
     return hip_fail(st, e, "transmit table");
@@ -797,6 +842,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     rxtab_defaults(G, pl->max_nw, G.wave_chunk, G.split);
   } else if (e != hipSuccess) {
     (void)hipFree(pl->tx_table);
+    (void)hipFree(pl->unified_tx);
     (void)hipFree(pl->dev);
     delete pl;
     return hip_fail(st, e, "receive table");
@@ -928,7 +974,9 @@ int ustbx_plan_active_terms(const ustbx_plan* plan, int64_t* terms, ustbx_status
 int64_t ustbx_plan_pixels(const ustbx_plan* plan) { return plan ? plan->G.P : 0; }
 
 int64_t ustbx_plan_device_bytes(const ustbx_plan* plan) {
-  return plan ? (int64_t)(plan->dev_bytes + plan->tx_table_bytes + plan->rxtab_bytes + plan->coef_bytes) : 0;
+  return plan ? (int64_t)(plan->dev_bytes + plan->tx_table_bytes + plan->unified_bytes + plan->rxtab_bytes +
+                          plan->coef_bytes)
+              : 0;
 }
 
 int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st) {
@@ -952,6 +1000,7 @@ void ustbx_plan_destroy(ustbx_plan* plan) {
   (void)hipFree(plan->G.coef);
   (void)hipFree(plan->rxtab);
   (void)hipFree(plan->tx_table);
+  (void)hipFree(plan->unified_tx);
   (void)hipFree(plan->dev);
   delete plan;
 }

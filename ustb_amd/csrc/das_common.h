@@ -406,7 +406,8 @@ enum {
   TXK_HYBRID_SEC = 5,   // same with the sector-scan plane patch
   TXK_BLENDED_LIN = 6,
   TXK_BLENDED_SEC = 7,
-  TXK_MATRIX = 8        // explicit [W][P] matrix (unified model)
+  TXK_MATRIX = 8,       // explicit [W][P] matrix given by the caller
+  TXK_UNIFIED = 9       // unified model, precomputed into the plan's [W][P] matrix
 };
 enum { TXAP_CONST = 0, TXAP_INCIDENCE = 1, TXAP_MATRIX = 2 };
 
@@ -453,6 +454,7 @@ __device__ __forceinline__ float tx_delay_f32(const DevWave& w, double x,
       v = 0.0;
       break;
     case TXK_MATRIX:
+    case TXK_UNIFIED:
       return mat[(size_t)wi * P + p];
     default: {
       const double dx = w.sx - x, dy = w.sy - y, dz = w.sz - z;

@@ -78,6 +78,8 @@ struct GeomLaunch {
 
 // Builds the transmit table of a plan (one lane per (wave, pixel)).
 hipError_t launch_geom_tx_table(const GeomLaunch& g, float4* table, hipStream_t stream);
+hipError_t launch_geom_unified_tx(const GeomLaunch& G, const DevTxApod& mask_apod, float* out,
+                                  hipStream_t stream);
 
 // Builds the variant-2 tables; tile_bad[t] = 1 where a nonzero weight or its
 // delay is not finite.

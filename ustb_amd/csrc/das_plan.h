@@ -14,6 +14,8 @@ struct ustbx_plan {
   size_t dev_bytes;
   float4* tx_table;      // resident [W][P] transmit table
   size_t tx_table_bytes;
+  float* unified_tx;     // [W][P] transmit delays of unified-model waves (else null)
+  size_t unified_bytes;
   bool dasc_order;
   // variant 2 (receive table): 0 not built, 1 built, -1 not applicable
   int rxtab_state;

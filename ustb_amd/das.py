@@ -6,10 +6,10 @@ Mirrors the Python reference's ``ustb.midprocess.DAS``
 data, and the static ``_beamform`` with the reference's signature.
 
 ``go()`` computes delays and apodization on the device (a resident geometry
-plan, cached per configuration) and fuses the IQ reference phase
-(das.m:306-309) into the kernel epilogue; configurations the device geometry
-does not cover (scanline or unified-model transmit state) are given to the
-same kernels as explicit matrices computed on the host.
+plan, cached per configuration, including the unified transmit delay model)
+and fuses the IQ reference phase (das.m:306-309) into the kernel epilogue.
+Scanline apodization has no device form: the plan raises
+USTBX_E_UNSUPPORTED for it.
 """
 from __future__ import annotations
 
