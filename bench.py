@@ -49,6 +49,9 @@ def parse():
     ap.add_argument("--abi", default="geometry", choices=["geometry", "matrix"],
                     help="C1-C3: geometry plan (delays and apodization from resident tables) or the das_c "
                          "matrix contract (ustbx_das_run over das.m's FP32 matrices in HBM, the plain kernel)")
+    ap.add_argument("--postprocess", default="none", choices=["none", "cf"],
+                    help="C3: das with the coherence factor over the receive channels, the fused kernel "
+                         "against the per-channel images followed by the coherence-factor kernel")
     ap.add_argument("--wave-chunk", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant (default: the plan's choice)")
@@ -127,6 +130,9 @@ def main():
         return
     if args.abi == "matrix":
         run_matrix(args, torch, dev)
+        return
+    if args.postprocess == "cf":
+        run_cf(args, torch, dev)
         return
     t_gen = time.time()
     acq = configs.BUILDERS[args.config]()
@@ -217,6 +223,89 @@ def main():
         result["errors"] = check_errors(acq, plan, out, ch, torch)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(acq, plan, torch)
+    plan.close()
+    print(json.dumps(result))
+
+
+def run_cf(args, torch, dev):
+    """das (dimension transmit) + postprocess.coherence_factor over the
+    receive channels: the fused kernel (ustbx_plan_run_cf, no per-channel
+    images in HBM) against the plan's transmit kernel writing the per-channel
+    images followed by the coherence-factor kernel. Same units as the das
+    line (pixel x channel x wave x frame)."""
+    from ustb_amd import _lib as L
+    from ustb_amd import configs, ops
+    from ustb_amd.geometry import make_plan
+
+    acq = configs.BUILDERS[args.config]()
+    if int(acq.dimension) != 2:
+        raise SystemExit("--postprocess cf needs a dimension-transmit workload (C3)")
+    ch = torch.from_numpy(acq.data).to(dev)
+    if args.phase == "auto":
+        args.phase = "dasc"
+    plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=(args.phase == "separable"),
+                     **acq.plan_kwargs())
+    if args.wave_chunk or args.split:
+        plan.set_tiles(args.wave_chunk, args.split)
+    M = plan.active_elements(L.DIM_RX)
+    per_channel = torch.empty(plan.out_shape, dtype=torch.complex64, device=dev)
+    image = torch.empty((plan.F, plan.P), dtype=torch.complex64, device=dev)
+    cf = torch.empty((plan.F, plan.P), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def fused():
+        plan.run_cf(ch, M, L.CF_MATLAB, image, cf)
+
+    def unfused():
+        plan.run(ch, per_channel)
+        return ops.coherence_factor(per_channel, L.DIM_RX, M, L.CF_MATLAB)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps, (time.perf_counter() - t0) * 1e3 / args.steps
+
+    fused_ms, fused_wall = timed(fused)
+    unfused_ms, unfused_wall = timed(unfused)
+    img_u, cf_u = unfused()
+    fused()
+    torch.cuda.synchronize()
+    peak = float(img_u.abs().max())
+    err = float((image - img_u[:, 0, 0]).abs().max()) / max(peak, 1e-30)
+    cf_err = float((cf - cf_u[:, 0, 0]).abs().max())
+    log(f"das + coherence factor: fused {fused_ms:.3f} ms, unfused {unfused_ms:.3f} ms, image {err:.2e}, "
+        f"cf {cf_err:.2e}")
+    result = {
+        "metric": "DAS Gsamples/s (pixel x channel x wave)",
+        "value": round(acq.units / (fused_wall * 1e-3) / 1e9, 3),
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fused_wall, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": acq.name + " + coherence factor over the receive channels",
+                   "phase": args.phase, "postprocess": "coherence_factor (MATLAB flavour, M from the receive "
+                                                       "apodization above 0.16)"},
+        "fused": {"kernel_ms": round(fused_ms, 4), "output_bytes": image.numel() * 8 + cf.numel() * 4,
+                  "workspace_bytes": int(plan._cf_workspace.numel())},
+        "unfused": {"kernel_ms": round(unfused_ms, 4), "per_channel_bytes": per_channel.numel() * 8,
+                    "Gsamples_s": round(acq.units / (unfused_wall * 1e-3) / 1e9, 3)},
+        "fused_speedup": round(unfused_ms / fused_ms, 3),
+        "errors": {"fused_vs_unfused_image": err, "fused_vs_unfused_cf_abs": cf_err},
+    }
     plan.close()
     print(json.dumps(result))
 

@@ -350,6 +350,59 @@ int32_t ustbx_abi_version(void);
 /* Number of HIP devices visible (0 when none); never fails. */
 int32_t ustbx_device_count(void);
 
+/* ---------------------------------------------------------------------------
+ * 5. Postprocess on beamformed data: device buffers [F][W][N][P] complex64
+ *    (MATLAB [P, N, W, F]); dim names the summed dimension (USTBX_DIM_RX,
+ *    USTBX_DIM_TX or USTBX_DIM_BOTH).
+ * ------------------------------------------------------------------------- */
+#define USTBX_CF_MATLAB 0 /* postprocess.coherence_factor (coherence_factor.m:97-143) */
+#define USTBX_CF_PYTHON 1 /* python/src/ustb/postprocess/coherence_factor.py:19-50 */
+
+/* postprocess.coherent_compounding (coherent_compounding.m:41-73). Without a
+ * window (window_size 0) out is [F][W'][N'][P], the summed axes of size 1.
+ * With one (dimension both or transmit) waves and frames are merged, wave
+ * fastest, and out[t] sums entries t .. t+window_size-1 (movsum with
+ * 'Endpoints' 'discard'): out is [T][1][N'][P], T = W*F - window_size + 1;
+ * receive ignores the window. window_size > W is USTBX_E_ARG, the reference's
+ * assertion. */
+int ustbx_coherent_compounding(const void* in, void* out, int32_t P, int32_t N, int32_t W, int32_t F,
+                               int32_t dim, int32_t window_size, void* stream, ustbx_status* st);
+/* Active elements per pixel for the coherence factor (coherence_factor.m:
+ * 82-131): M[p] = number of summed elements whose apodization exceeds
+ * criterium (the reference's active_element_criterium, 0.16 by default); both
+ * counts the products tx_apo * rx_apo. tx_apo [W][P] and rx_apo [N][P] are
+ * device FP32 (NULL = ones), compared in FP64; M is device [P]. */
+int ustbx_active_elements(const float* tx_apo, const float* rx_apo, int32_t P, int32_t N, int32_t W,
+                          int32_t dim, double criterium, float* M, void* stream, ustbx_status* st);
+/* The Mallart-Fink coherence factor: image = CF * coherent sum, cf = CF
+ * (NULL: not written), both [F][W'][N'][P] with the summed axes of size 1.
+ * USTBX_CF_MATLAB: CF = |S|^2 / sum|s|^2 * (1/M), M [P] active elements
+ * (NULL: every summed element); USTBX_CF_PYTHON: |S|^2 / (sum|s|^2 * M +
+ * 1e-30) with M the number of summed elements; NaN -> 0 in both. The
+ * reference's dimension fallbacks (coherence_factor.m:62-79) are the
+ * caller's (ustb_amd.postprocess.CoherenceFactor applies them). */
+int ustbx_coherence_factor(const void* in, void* image, float* cf, const float* M, int32_t P, int32_t N,
+                           int32_t W, int32_t F, int32_t dim, int32_t flavour, void* stream, ustbx_status* st);
+/* das in dimension transmit with the coherence factor over the receive
+ * channels in the kernel's epilogue (no reference counterpart: das.m writes
+ * the per-channel images and postprocess.coherence_factor reads them back).
+ * Each block of the transmit kernel reduces its channels' images, in channel
+ * order, to per-pixel sums S and sum |s|^2 in `workspace` (ceil(N / channels
+ * per block) x F x P float4, ustbx_plan_cf_workspace_bytes), a second pass
+ * adds the groups in order and applies the factor: image [F][P] complex64,
+ * cf [F][P] float (NULL: not written); M and flavour as
+ * ustbx_coherence_factor. Equal to ustbx_plan_run followed by
+ * ustbx_coherence_factor(dim receive) up to the FP32 rounding of the grouped
+ * sums. Needs a dimension-transmit plan on variant 2 and at least two
+ * channels. Calls sharing a workspace must not overlap. */
+int64_t ustbx_plan_cf_workspace_bytes(const ustbx_plan* plan);
+int ustbx_plan_run_cf(const ustbx_plan* plan, const void* ch, void* image, float* cf, const float* M,
+                      int32_t flavour, void* workspace, int64_t workspace_bytes, void* stream, ustbx_status* st);
+/* M[p] from a plan's FP32 apodization (as ustbx_plan_materialize gives it),
+ * for ustbx_plan_run_cf and ustbx_coherence_factor. */
+int ustbx_plan_active_elements(const ustbx_plan* plan, int32_t dim, double criterium, float* M, void* stream,
+                               ustbx_status* st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,9 @@ Contents
   (linear_scan.m:64-82, sector_scan.m:73-121), element positions
   (linear_array.m:56-61, matrix_array.m:60-70), ``reference_phase``
   (das.m:306-309).
+- postprocess in FP64: ``coherent_compounding`` (coherent_compounding.m:41-73),
+  ``coherence_factor`` with ``coherence_dimension`` and ``active_elements``
+  (coherence_factor.m:58-151; python/src/ustb/postprocess/coherence_factor.py:19-50).
 
 Array layout: MATLAB column-major [L,N,W,F] is held as NumPy C-order
 (F, W, N, L); [P,N] as (N, P); [P,W] as (W, P); outputs (F, W', N', P).
@@ -561,3 +564,94 @@ def hilbert(x, axis=-1):
     from scipy.signal import hilbert as _h
 
     return _h(x, axis=axis)
+
+
+# ---------------------------------------------------------------------------
+# Postprocess on beamformed data, layout (F, W, N, P) = MATLAB [P, N, W, F]
+# ---------------------------------------------------------------------------
+
+def coherent_compounding(data, dim=DIM_BOTH, window_size=None):
+    """postprocess.coherent_compounding (coherent_compounding.m:41-73), FP64.
+
+    Without a window: the sum over channels and waves (both), waves
+    (transmit) or channels (receive). With one, frames and waves are merged
+    (wave fastest) and a moving sum of window_size consecutive entries with
+    the incomplete ends discarded (movsum 'Endpoints','discard') runs along
+    them: both after summing channels, transmit per channel; receive ignores
+    the window. Returns (F', W', N', P)."""
+    x = np.asarray(data).astype(np.complex128)
+    F, W, N, P = x.shape
+    if window_size is None:
+        if dim == DIM_BOTH:
+            return x.sum(axis=(1, 2), keepdims=True)
+        if dim == DIM_TX:
+            return x.sum(axis=1, keepdims=True)
+        if dim == DIM_RX:
+            return x.sum(axis=2, keepdims=True)
+        raise ValueError("Unknown dimension mode; check HELP dimension")
+    if window_size > W:
+        raise ValueError("Cannot have a window size greater that the available number of transmit events.")
+    merged = x.reshape(F * W, N, P)                  # reshape to [P, N, 1, W*F] (line 58)
+    if dim == DIM_RX:
+        return x.sum(axis=2, keepdims=True)
+    if dim == DIM_BOTH:
+        merged = merged.sum(axis=1, keepdims=True)
+    elif dim != DIM_TX:
+        raise ValueError("Unknown dimension mode; check HELP dimension")
+    T = F * W - window_size + 1
+    out = np.stack([merged[t:t + window_size].sum(axis=0) for t in range(T)])
+    return out[:, None, :, :]                        # [P, N', 1, T] -> (T, 1, N', P)
+
+
+def coherence_dimension(dim, N, W):
+    """The dimension coherence_factor.m actually uses (lines 62-79): dimension
+    both falls back to transmit (one channel) or receive (one wave); errors
+    as the reference raises them."""
+    if dim == DIM_RX and N < 2:
+        raise ValueError("Not enough channels to compute factor")
+    if dim == DIM_TX and W < 2:
+        raise ValueError("Not enough waves to compute factor")
+    if dim == DIM_BOTH:
+        if N < 2 and W > 1:
+            return DIM_TX
+        if W < 2 and N > 1:
+            return DIM_RX
+        if W < 2 and N < 2:
+            raise ValueError("Not enough waves and channels to compute factor")
+    return dim
+
+
+def active_elements(dim, tx_apod, rx_apod, criterium=0.16):
+    """M per pixel (coherence_factor.m:82-131): the number of summed elements
+    whose apodization exceeds the criterium; both uses the product of the
+    transmit and receive apodization. tx_apod (W, P), rx_apod (N, P), FP64."""
+    if dim == DIM_BOTH:
+        prod = np.asarray(tx_apod, np.float64)[:, None, :] * np.asarray(rx_apod, np.float64)[None, :, :]
+        return (prod > criterium).sum(axis=(0, 1)).astype(np.float64)
+    if dim == DIM_TX:
+        return (np.asarray(tx_apod, np.float64) > criterium).sum(axis=0).astype(np.float64)
+    return (np.asarray(rx_apod, np.float64) > criterium).sum(axis=0).astype(np.float64)
+
+
+def coherence_factor(data, dim=DIM_BOTH, M=None, flavour="matlab"):
+    """Mallart-Fink coherence factor. flavour "matlab": postprocess.
+    coherence_factor (coherence_factor.m:97-143): CF = |sum|^2 / sum|.|^2 *
+    (1/M), NaN -> 0, M per pixel (None: every summed element active).
+    flavour "python": the Python reference's CoherenceFactor
+    (postprocess/coherence_factor.py:19-50): CF = |sum|^2 / (sum|.|^2 * M +
+    1e-30) with M the number of summed elements, NaN -> 0. FP64. Returns
+    (image = CF * coherent sum, CF), each (F, W', N', P)."""
+    x = np.asarray(data).astype(np.complex128)
+    F, W, N, P = x.shape
+    axes = {DIM_BOTH: (1, 2), DIM_TX: (1,), DIM_RX: (2,)}[dim]
+    count = {DIM_BOTH: N * W, DIM_TX: W, DIM_RX: N}[dim]
+    coherent = x.sum(axis=axes, keepdims=True)
+    incoherent = (np.abs(x) ** 2).sum(axis=axes, keepdims=True)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if flavour == "python":
+            cf = np.abs(coherent) ** 2 / (incoherent * float(count) + 1e-30)
+        else:
+            Mp = np.full(P, float(count)) if M is None else np.asarray(M, np.float64)
+            cf = (np.abs(coherent) ** 2 / incoherent) * (1.0 / Mp)[None, None, None, :]
+    cf[np.isnan(cf)] = 0
+    return cf * coherent, cf

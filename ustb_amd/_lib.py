@@ -28,6 +28,7 @@ DIM_NONE, DIM_RX, DIM_TX, DIM_BOTH = 0, 1, 2, 3
 INTERP_LINEAR, INTERP_CUBIC = 0, 1
 EDGE_DASC, EDGE_PYTHON = 0, 1
 FLAG_DASC_ORDER, FLAG_SEPARABLE_PHASE, FLAG_REF_PHASE = 1, 2, 4
+CF_MATLAB, CF_PYTHON = 0, 1
 SCAN_LINEAR, SCAN_SECTOR, SCAN_XYZ = 0, 1, 2
 WAVE_PLANE, WAVE_SPHERICAL, WAVE_PHOTOACOUSTIC = 0, 1, 2
 TXMODEL_SPHERICAL, TXMODEL_UNIFIED, TXMODEL_HYBRID, TXMODEL_BLENDED = 1, 2, 3, 4
@@ -130,6 +131,17 @@ EXPORTS = {
     "ustbx_synth_echoes": (ctypes.c_int, [ctypes.POINTER(SynthDesc), c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p]),
     "ustbx_hilbert": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_coherent_compounding": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32,
+                                                  c_int32, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_active_elements": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32,
+                                             ctypes.c_double, c_void_p, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_coherence_factor": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_int32, c_int32,
+                                              c_int32, c_int32, c_int32, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_plan_cf_workspace_bytes": (c_int64, [c_void_p]),
+    "ustbx_plan_run_cf": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
+                                         c_int64, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_plan_active_elements": (ctypes.c_int, [c_void_p, c_int32, ctypes.c_double, c_void_p, c_void_p,
+                                                  ctypes.POINTER(Status)]),
 }
 
 _lib = None

@@ -947,6 +947,45 @@ int ustbx_plan_run(const ustbx_plan* plan, const void* ch, void* out, void* stre
   return ok(st);
 }
 
+int64_t ustbx_plan_cf_workspace_bytes(const ustbx_plan* plan) {
+  if (!plan || plan->G.dim != USTBX_DIM_TX || plan->G.variant != 2) return 0;
+  return (int64_t)tx_cf_workspace_bytes(plan->G);
+}
+
+int ustbx_plan_run_cf(const ustbx_plan* plan, const void* ch, void* image, float* cf, const float* M,
+                      int32_t flavour, void* workspace, int64_t workspace_bytes, void* stream, ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (!ch || !image) return set_status(st, USTBX_E_ARG, "NULL channel data or image");
+  if (flavour != USTBX_CF_MATLAB && flavour != USTBX_CF_PYTHON)
+    return set_status(st, USTBX_E_ARG, "flavour must be USTBX_CF_MATLAB or USTBX_CF_PYTHON");
+  const GeomLaunch& P0 = plan->G;
+  if (P0.dim != USTBX_DIM_TX || P0.variant != 2)
+    return set_status(st, USTBX_E_UNSUPPORTED,
+                      "the fused coherence factor needs a dimension-transmit plan on the receive-table kernel "
+                      "(variant 2)");
+  if (P0.N < 2) return set_status(st, USTBX_E_ARG, "Not enough channels to compute factor");
+  const size_t need = tx_cf_workspace_bytes(P0);
+  if (!workspace || workspace_bytes < 0 || (size_t)workspace_bytes < need)
+    return set_status(st, USTBX_E_ARG, "workspace of %zu bytes needed (ustbx_plan_cf_workspace_bytes)", need);
+  GeomLaunch G = P0;
+  G.ch = ch;
+  hipError_t e = launch_tx_cf(G, reinterpret_cast<float4*>(workspace), reinterpret_cast<float2*>(image), cf, M,
+                              flavour, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(st, e, "das_tx_cf kernels launch");
+  return ok(st);
+}
+
+int ustbx_plan_active_elements(const ustbx_plan* plan, int32_t dim, double criterium, float* M, void* stream,
+                               ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (!(dim == USTBX_DIM_RX || dim == USTBX_DIM_TX || dim == USTBX_DIM_BOTH))
+    return set_status(st, USTBX_E_ARG, "Unknown dimension mode; check HELP dimension");
+  if (!M) return set_status(st, USTBX_E_ARG, "NULL output");
+  hipError_t e = launch_geom_active_elements(plan->G, dim, criterium, M, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(st, e, "geom_active_elements_kernel launch");
+  return ok(st);
+}
+
 int ustbx_plan_materialize(const ustbx_plan* plan, float* tx_apo, float* rx_apo,
                            float* tx_delay, float* rx_delay, void* stream, ustbx_status* st) {
   if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");

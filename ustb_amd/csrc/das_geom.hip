@@ -41,6 +41,40 @@ __global__ __launch_bounds__(kBlock) void geom_materialize_kernel(
   }
 }
 
+// Active elements of the coherence factor per pixel (coherence_factor.m:
+// 82-131) from the plan's FP32 apodization: receive weights above the
+// criterium (dim receive), transmit weights (transmit), or their products
+// (both), compared in FP64.
+__global__ __launch_bounds__(kBlock) void geom_active_elements_kernel(GeomLaunch G, int dim, double crit,
+                                                                      float* __restrict__ M) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= G.P) return;
+  double px, py, pz, rref;
+  pixel_xyz(G.scan, p, px, py, pz, rref);
+  int cnt = 0;
+  if (dim == MODE_TX) {
+    for (int w = 0; w < G.W; ++w) cnt += ((double)tx_apod_f32(G.waves[w], G.tx, px, py, pz, w, p, G.P) > crit) ? 1 : 0;
+  } else {
+    for (int n = 0; n < G.N; ++n) {
+      const double ra = (double)rx_apod_f32(G.rx, px, py, pz, G.ex[n], G.ey[n], G.ez[n], n, p, G.P);
+      if (dim == MODE_RX) {
+        cnt += (ra > crit) ? 1 : 0;
+      } else {
+        for (int w = 0; w < G.W; ++w)
+          cnt += ((double)tx_apod_f32(G.waves[w], G.tx, px, py, pz, w, p, G.P) * ra > crit) ? 1 : 0;
+      }
+    }
+  }
+  M[p] = (float)cnt;
+}
+
+hipError_t launch_geom_active_elements(const GeomLaunch& G, int dim, double crit, float* M, hipStream_t stream) {
+  if (G.P <= 0) return hipSuccess;
+  hipLaunchKernelGGL(geom_active_elements_kernel, dim3((unsigned)((G.P + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     stream, G, dim, crit, M);
+  return hipGetLastError();
+}
+
 // Transmit table of a plan: {td, ta, ta*cos(wd td), ta*sin(wd td)} per
 // (wave, pixel); grid.y = wave.
 __global__ __launch_bounds__(kBlock) void geom_tx_table_kernel(GeomLaunch G,

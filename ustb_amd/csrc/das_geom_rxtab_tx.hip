@@ -26,29 +26,21 @@
 // ascending order. The sine and cosine are taken once per (pixel, channel,
 // wave) and shared by the block's frames. That is das_c's order for this
 // mode, so the result equals das_c's (bit for bit for RF data, linear).
+#include "das_cf.h"
 #include "das_rxtab.h"
 
 namespace ustbx {
 
-template <int SM, int FC, int CS, int PH>
-__global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, float wd,
-                                                               const int* __restrict__ order) {
+// One channel's images of FC frames f0 .. f0+nf-1 on one 64-pixel tile (a
+// wave64, lane = pixel): v[f] is what das_c writes to out[f][n][p] (times the
+// reference phase where the plan has it), zeros for a channel with no receive
+// weight on the tile. Wave-uniform: tile, n, f0, nf.
+template <int SM, int FC, int PH>
+__device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, int n, int f0, int nf, int lane,
+                                           float2 (&v)[FC]) {
   using M = SampleMode<SM>;
   constexpr int TP = 64;
   constexpr int ST = M::kStride;
-  const int lane = threadIdx.x & (TP - 1);
-  const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
-  const int tiles = (A.P + TP - 1) / TP;
-  const int chunks = (F + FC - 1) / FC;
-  const int item = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  const int slab = item / tiles;
-  const int rank = item - slab * tiles;
-  const int group = slab / chunks;
-  const int n = group * CS + s;
-  if (n >= A.N) return;  // no block barrier below
-  const int tile = order ? __builtin_amdgcn_readfirstlane(order[rank]) : rank;
-  const int f0 = (slab - group * chunks) * FC;
-  const int nf = min(FC, F - f0);
   const int p = tile * TP + lane;
   const bool active = p < A.P;
   const size_t PS = (size_t)A.P;
@@ -65,113 +57,206 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, f
   constexpr float lo_ok = 1.0f;
   const float t0 = A.t0, fs = A.fs;
 
-  {
-    float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (active) rv = A.rx[(size_t)n * PS + p];
-    const bool ron = rv.y != 0.0f;
-    if (!__any(ron)) {  // no receive weight on the tile: zeros
-      if (active)
-        for (int f = 0; f < nf; ++f) A.out[((size_t)(f0 + f) * A.N + n) * PS + p] = make_float2(0.0f, 0.0f);
-      return;
+#pragma unroll
+  for (int f = 0; f < FC; ++f) v[f] = make_float2(0.0f, 0.0f);
+  float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (active) rv = A.rx[(size_t)n * PS + p];
+  const bool ron = rv.y != 0.0f;
+  if (!__any(ron)) return;  // no receive weight on the tile: zeros
+  const float rd = ron ? rv.x : kFar;
+  const float ra = rv.y;
+  const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
+  const unsigned nb = (unsigned)(n * lst);
+  float2 part[FC];
+#pragma unroll
+  for (int f = 0; f < FC; ++f) part[f] = make_float2(0.0f, 0.0f);
+  // the next wave's transmit row is in flight during this wave's gathers,
+  // its index one further ahead
+  int w = nw > 0 ? __builtin_amdgcn_readfirstlane(wl[0]) : 0;
+  int w_next = nw > 1 ? __builtin_amdgcn_readfirstlane(wl[1]) : 0;
+  float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (active && nw > 0) vn = A.tx[(size_t)w * PS + p];
+  for (int i = 0; i < nw; ++i) {
+    const float4 tv = vn;
+    const int wc = w;
+    if (i + 1 < nw) {
+      w = w_next;
+      if (active) vn = A.tx[(size_t)w * PS + p];
+      if (i + 2 < nw) w_next = __builtin_amdgcn_readfirstlane(wl[i + 2]);
     }
-    const float rd = ron ? rv.x : kFar;
-    const float ra = rv.y;
-    const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
-    const unsigned nb = (unsigned)(n * lst);
-    float2 part[FC];
-#pragma unroll
-    for (int f = 0; f < FC; ++f) part[f] = make_float2(0.0f, 0.0f);
-    // the next wave's transmit row is in flight during this wave's gathers,
-    // its index one further ahead
-    int w = nw > 0 ? __builtin_amdgcn_readfirstlane(wl[0]) : 0;
-    int w_next = nw > 1 ? __builtin_amdgcn_readfirstlane(wl[1]) : 0;
-    float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (active && nw > 0) vn = A.tx[(size_t)w * PS + p];
-    for (int i = 0; i < nw; ++i) {
-      const float4 v = vn;
-      const int wc = w;
-      if (i + 1 < nw) {
-        w = w_next;
-        if (active) vn = A.tx[(size_t)w * PS + p];
-        if (i + 2 < nw) w_next = __builtin_amdgcn_readfirstlane(wl[i + 2]);
-      }
-      const bool ton = v.y != 0.0f;
-      const float td = ton ? v.x : kFar;
-      const float2 T = ton ? make_float2(v.z, v.w) : make_float2(0.0f, 0.0f);
-      float d, c_delay;
-      {
+    const bool ton = tv.y != 0.0f;
+    const float td = ton ? tv.x : kFar;
+    const float2 T = ton ? make_float2(tv.z, tv.w) : make_float2(0.0f, 0.0f);
+    float d, c_delay;
+    {
 #pragma clang fp contract(off)
-        c_delay = td + rd;            // das_c.cpp:372
-        d = (c_delay - t0) * fs;      // das_c.cpp:375
-      }
-      // das_c's weight for this term (zero where either weight is zero)
-      float wr = 0.0f, wi = 0.0f;
-      if (PH != PH_SEP) {
+      c_delay = td + rd;            // das_c.cpp:372
+      d = (c_delay - t0) * fs;      // das_c.cpp:375
+    }
+    // das_c's weight for this term (zero where either weight is zero)
+    float wr = 0.0f, wi = 0.0f;
+    if (PH != PH_SEP) {
 #pragma clang fp contract(off)
-        const bool on = ton && ron;
-        const float c_apo = on ? v.y * ra : 0.0f;  // das_c.cpp:373
-        if (PH == PH_DASC) {
-          float sv, cv;
-          sincosf(wd * (on ? c_delay : 0.0f), &sv, &cv);  // das_c.cpp:383-387
-          wr = cv * c_apo;
-          wi = sv * c_apo;
-        } else {
-          wr = c_apo;
-        }
-      }
-      const bool ok = d >= lo_ok && d < lim;
-      const int n0 = (int)d;
-      const float b = __builtin_amdgcn_fractf(d);
-      const unsigned off = ok ? (unsigned)(n0 * ST) + nb + (unsigned)(wc * wst) : (unsigned)kOOB;
-      float4 q[FC][M::kLoads];
-#pragma unroll
-      for (int f = 0; f < FC; ++f) {
-        const unsigned o = off + (unsigned)(f * fb);
-        q[f][0] = buf_load16(rs, (int)(o + (unsigned)M::kLo));
-        if (M::kLoads == 2) q[f][M::kLoads - 1] = buf_load16(rs, (int)(o + (unsigned)M::kHi));
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
-      if (SM == SM_CUBIC) keys_weights_horner(b, c0, c1, c2, c3);
-#pragma unroll
-      for (int f = 0; f < FC; ++f) {
-        float2 sm;
-        if (PH != PH_SEP && SM == SM_LINEAR) {
-          Term t;
-          t.a = 1.0f - b;
-          t.b = b;
-          t.wr = wr;
-          t.wi = wi;
-          cadd(part[f], dasc_contrib(t, q[f][0]));  // das_c.cpp:389-399
-          continue;
-        }
-        if (SM == SM_LINEAR) {
-          const float4 x = q[f][0];
-          sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
-        } else {
-          const float4 lo = q[f][0], hi = q[f][M::kLoads - 1];
-          sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
-                           fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
-        }
-        if (PH == PH_SEP) {
-          cfma(part[f], T, sm);
-        } else {
-#pragma clang fp contract(off)
-          cadd(part[f], make_float2(wr * sm.x - wi * sm.y, wr * sm.y + wi * sm.x));
-        }
+      const bool on = ton && ron;
+      const float c_apo = on ? tv.y * ra : 0.0f;  // das_c.cpp:373
+      if (PH == PH_DASC) {
+        float sv, cv;
+        sincosf(wd * (on ? c_delay : 0.0f), &sv, &cv);  // das_c.cpp:383-387
+        wr = cv * c_apo;
+        wi = sv * c_apo;
+      } else {
+        wr = c_apo;
       }
     }
-    if (active) {
-      const float2 refp = A.ref ? A.ref[p] : make_float2(1.0f, 0.0f);
+    const bool ok = d >= lo_ok && d < lim;
+    const int n0 = (int)d;
+    const float b = __builtin_amdgcn_fractf(d);
+    const unsigned off = ok ? (unsigned)(n0 * ST) + nb + (unsigned)(wc * wst) : (unsigned)kOOB;
+    float4 q[FC][M::kLoads];
 #pragma unroll
-      for (int f = 0; f < FC; ++f) {
-        if (f < nf) {
-          float2 v = PH == PH_SEP ? cmul(R, part[f]) : part[f];
-          if (A.ref) v = cmul(v, refp);
-          A.out[((size_t)(f0 + f) * A.N + n) * PS + p] = v;
-        }
+    for (int f = 0; f < FC; ++f) {
+      const unsigned o = off + (unsigned)(f * fb);
+      q[f][0] = buf_load16(rs, (int)(o + (unsigned)M::kLo));
+      if (M::kLoads == 2) q[f][M::kLoads - 1] = buf_load16(rs, (int)(o + (unsigned)M::kHi));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
+    if (SM == SM_CUBIC) keys_weights_horner(b, c0, c1, c2, c3);
+#pragma unroll
+    for (int f = 0; f < FC; ++f) {
+      float2 sm;
+      if (PH != PH_SEP && SM == SM_LINEAR) {
+        Term t;
+        t.a = 1.0f - b;
+        t.b = b;
+        t.wr = wr;
+        t.wi = wi;
+        cadd(part[f], dasc_contrib(t, q[f][0]));  // das_c.cpp:389-399
+        continue;
+      }
+      if (SM == SM_LINEAR) {
+        const float4 x = q[f][0];
+        sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
+      } else {
+        const float4 lo = q[f][0], hi = q[f][M::kLoads - 1];
+        sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
+                         fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
+      }
+      if (PH == PH_SEP) {
+        cfma(part[f], T, sm);
+      } else {
+#pragma clang fp contract(off)
+        cadd(part[f], make_float2(wr * sm.x - wi * sm.y, wr * sm.y + wi * sm.x));
       }
     }
+  }
+  const float2 refp = (active && A.ref) ? A.ref[p] : make_float2(1.0f, 0.0f);
+#pragma unroll
+  for (int f = 0; f < FC; ++f) {
+    float2 x = PH == PH_SEP ? cmul(R, part[f]) : part[f];
+    if (A.ref) x = cmul(x, refp);
+    v[f] = x;
+  }
+}
+
+template <int SM, int FC, int CS, int PH>
+__global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, float wd,
+                                                               const int* __restrict__ order) {
+  constexpr int TP = 64;
+  const int lane = threadIdx.x & (TP - 1);
+  const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
+  const int tiles = (A.P + TP - 1) / TP;
+  const int chunks = (F + FC - 1) / FC;
+  const int item = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int slab = item / tiles;
+  const int rank = item - slab * tiles;
+  const int group = slab / chunks;
+  const int n = group * CS + s;
+  if (n >= A.N) return;  // no block barrier
+  const int tile = order ? __builtin_amdgcn_readfirstlane(order[rank]) : rank;
+  const int f0 = (slab - group * chunks) * FC;
+  const int nf = min(FC, F - f0);
+  float2 v[FC];
+  tx_channel<SM, FC, PH>(A, wd, tile, n, f0, nf, lane, v);
+  const int p = tile * TP + lane;
+  if (p < A.P) {
+    const size_t PS = (size_t)A.P;
+#pragma unroll
+    for (int f = 0; f < FC; ++f)
+      if (f < nf) A.out[((size_t)(f0 + f) * A.N + n) * PS + p] = v[f];
+  }
+}
+
+// Dimension transmit with the coherence factor over the receive channels,
+// first stage: the transmit kernel's blocks and work order (one wave per
+// channel, XCDs owning channel-group x frame-chunk slabs), but instead of the
+// per-channel images each block stores, per frame and pixel, the sums over
+// its CS channels in channel order: S_g = sum v, E_g = sum |v|^2
+// (coherence_factor.m:126-127), through LDS. part is [G][F][P] float4
+// {S.x, S.y, E, 0}, G = ceil(N / CS).
+template <int SM, int FC, int CS, int PH>
+__global__ __launch_bounds__(64 * CS) void das_tx_cf_part_kernel(RxAcq A, int F, float wd,
+                                                                 const int* __restrict__ order,
+                                                                 float4* __restrict__ part) {
+  constexpr int TP = 64;
+  __shared__ float2 red[CS][TP];
+  const int lane = threadIdx.x & (TP - 1);
+  const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
+  const int tiles = (A.P + TP - 1) / TP;
+  const int chunks = (F + FC - 1) / FC;
+  const int item = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int slab = item / tiles;
+  const int rank = item - slab * tiles;
+  const int group = slab / chunks;
+  const int n = group * CS + s;
+  const int tile = order ? __builtin_amdgcn_readfirstlane(order[rank]) : rank;
+  const int f0 = (slab - group * chunks) * FC;
+  const int nf = min(FC, F - f0);
+  float2 v[FC];
+  if (n < A.N) {
+    tx_channel<SM, FC, PH>(A, wd, tile, n, f0, nf, lane, v);
+  } else {
+#pragma unroll
+    for (int f = 0; f < FC; ++f) v[f] = make_float2(0.0f, 0.0f);
+  }
+  const int p = tile * TP + lane;
+  const size_t PS = (size_t)A.P;
+#pragma unroll
+  for (int f = 0; f < FC; ++f) {
+    red[s][lane] = v[f];
+    __syncthreads();
+    if (s == 0 && p < A.P && f < nf) {
+      float2 S = make_float2(0.0f, 0.0f);
+      float E = 0.0f;
+#pragma unroll
+      for (int k = 0; k < CS; ++k) cf_accumulate(S, E, red[k][lane]);
+      part[((size_t)group * F + f0 + f) * PS + p] = make_float4(S.x, S.y, E, 0.0f);
+    }
+    __syncthreads();
+  }
+}
+
+// Second stage: S = sum over the groups in order, E likewise, then the
+// coherence factor; image and cf [F][P].
+__global__ __launch_bounds__(256) void das_tx_cf_finish_kernel(const float4* __restrict__ part, int G, int F, int P,
+                                                              int N, float2* __restrict__ image,
+                                                              float* __restrict__ cfo, const float* __restrict__ M,
+                                                              int flavour) {
+  const size_t total = (size_t)F * P;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256) {
+    float2 S = make_float2(0.0f, 0.0f);
+    float E = 0.0f;
+    for (int g = 0; g < G; ++g) {
+      const float4 q = part[(size_t)g * total + i];
+      S.x = S.x + q.x;
+      S.y = S.y + q.y;
+      E = E + q.z;
+    }
+    const size_t p = i % (size_t)P;
+    const float m = (flavour == USTBX_CF_PYTHON || !M) ? (float)N : M[p];
+    const float cf = coherence_value(S, E, m, flavour);
+    image[i] = make_float2(cf * S.x, cf * S.y);
+    if (cfo) cfo[i] = cf;
   }
 }
 
@@ -197,6 +282,52 @@ static hipError_t launch_tx_ph(const GeomLaunch& G, hipStream_t stream) {
   if (G.wd == 0.0f) return launch_tx_sm<SM, PH_RF>(G, stream);
   if (G.separable) return launch_tx_sm<SM, PH_SEP>(G, stream);
   return launch_tx_sm<SM, PH_DASC>(G, stream);
+}
+
+template <int SM, int FC, int CS, int PH>
+static hipError_t launch_cf_t(const GeomLaunch& G, float4* part, hipStream_t stream) {
+  const int64_t blocks = (int64_t)((G.P + 63) / 64) * ((G.F + FC - 1) / FC) * ((G.N + CS - 1) / CS);
+  if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, FC, CS, PH>), dim3((unsigned)blocks), dim3(64 * CS), 0, stream,
+                     rx_acq(G, SM), G.F, G.wd, G.tile_order, part);
+  return hipGetLastError();
+}
+
+template <int SM, int PH>
+static hipError_t launch_cf_sh(const GeomLaunch& G, float4* part, hipStream_t stream) {
+  const int fc = G.wave_chunk, cs = G.split;
+  if (fc == 4) return cs == 4 ? launch_cf_t<SM, 4, 4, PH>(G, part, stream) : launch_cf_t<SM, 4, 8, PH>(G, part, stream);
+  if (fc == 8) return cs == 4 ? launch_cf_t<SM, 8, 4, PH>(G, part, stream) : launch_cf_t<SM, 8, 8, PH>(G, part, stream);
+  return hipErrorInvalidValue;
+}
+
+template <int SM>
+static hipError_t launch_cf_ph(const GeomLaunch& G, float4* part, hipStream_t stream) {
+  if (G.wd == 0.0f) return launch_cf_sh<SM, PH_RF>(G, part, stream);
+  if (G.separable) return launch_cf_sh<SM, PH_SEP>(G, part, stream);
+  return launch_cf_sh<SM, PH_DASC>(G, part, stream);
+}
+
+// Bytes of the partial sums launch_tx_cf needs: ceil(N / channels per block)
+// x F x P float4.
+size_t tx_cf_workspace_bytes(const GeomLaunch& G) {
+  const int cs = G.split > 0 ? G.split : 1;
+  return (size_t)((G.N + cs - 1) / cs) * (size_t)G.F * (size_t)G.P * sizeof(float4);
+}
+
+// The fused coherence-factor form: partial sums per channel group (the
+// plan's frames and channels per block) into part, then the finishing pass.
+hipError_t launch_tx_cf(const GeomLaunch& G, float4* part, float2* image, float* cf, const float* M, int flavour,
+                        hipStream_t stream) {
+  hipError_t e = G.interp == INTERP_CUBIC ? launch_cf_ph<SM_CUBIC>(G, part, stream)
+                                          : launch_cf_ph<SM_LINEAR>(G, part, stream);
+  if (e != hipSuccess) return e;
+  const int groups = (G.N + G.split - 1) / G.split;
+  size_t blocks = ((size_t)G.F * G.P + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(das_tx_cf_finish_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, part, groups, G.F, G.P,
+                     G.N, image, cf, M, flavour);
+  return hipGetLastError();
 }
 
 // Frames per block (wave_chunk: 4 or 8) and channels per block (split: 4 or

@@ -80,6 +80,10 @@ struct GeomLaunch {
 hipError_t launch_geom_tx_table(const GeomLaunch& g, float4* table, hipStream_t stream);
 hipError_t launch_geom_unified_tx(const GeomLaunch& G, const DevTxApod& mask_apod, float* out,
                                   hipStream_t stream);
+hipError_t launch_geom_active_elements(const GeomLaunch& G, int dim, double crit, float* M, hipStream_t stream);
+hipError_t launch_tx_cf(const GeomLaunch& G, float4* part, float2* image, float* cf, const float* M, int flavour,
+                        hipStream_t stream);
+size_t tx_cf_workspace_bytes(const GeomLaunch& G);
 
 // Builds the variant-2 tables; tile_bad[t] = 1 where a nonzero weight or its
 // delay is not finite.

@@ -113,7 +113,7 @@ class DAS:
         self.elapsed_time = time.perf_counter() - t0
         # (F, W', N', P) -> reference layout [pixel, channel, wave, frame]
         data = out.permute(3, 2, 1, 0)
-        self.beamformed_data = BeamformedData(scan=self.scan, data=data)
+        self.beamformed_data = BeamformedData(scan=self.scan, data=data, probe=cd.probe, sequence=cd.sequence)
         return self.beamformed_data
 
     # ------------------------------------------------------------------

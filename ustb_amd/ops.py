@@ -68,6 +68,68 @@ def das(ch: torch.Tensor, fs: float, t0: float, tx_apo: torch.Tensor, rx_apo: to
     return out
 
 
+def _summed_shape(dim: int, F: int, W: int, N: int, P: int) -> tuple:
+    return {L.DIM_BOTH: (F, 1, 1, P), L.DIM_TX: (F, 1, N, P), L.DIM_RX: (F, W, 1, P)}[dim]
+
+
+def coherent_compounding(x: torch.Tensor, dim: int, window_size: int = 0) -> torch.Tensor:
+    """postprocess.coherent_compounding on a device tensor (F, W, N, P)
+    complex64; returns (F', W', N', P)."""
+    _require_cuda(x)
+    if x.dtype != torch.complex64 or x.dim() != 4:
+        raise ValueError("beamformed data must be a (F, W, N, P) complex64 tensor")
+    x = x.contiguous()
+    F, W, N, P = x.shape
+    if window_size and dim != L.DIM_RX:
+        shape = (F * W - window_size + 1, 1, 1 if dim == L.DIM_BOTH else N, P)
+    else:
+        shape = _summed_shape(dim, F, W, N, P)
+    out = torch.empty(tuple(max(0, s) for s in shape), dtype=torch.complex64, device=x.device)
+    st = L.Status()
+    L.check(L.lib().ustbx_coherent_compounding(x.data_ptr(), out.data_ptr(), P, N, W, F, int(dim),
+                                               int(window_size), _stream_ptr(x.device), ctypes.byref(st)), st)
+    return out
+
+
+def active_elements(tx_apo: torch.Tensor | None, rx_apo: torch.Tensor | None, dim: int, P: int, N: int, W: int,
+                    criterium: float = 0.16) -> torch.Tensor:
+    """M[p] of the coherence factor from FP32 apodization matrices ([W, P],
+    [N, P]; None = ones)."""
+    ref = tx_apo if tx_apo is not None else rx_apo
+    dev = ref.device if ref is not None else torch.device("cuda", torch.cuda.current_device())
+    for t in (tx_apo, rx_apo):
+        if t is not None:
+            _require_cuda(t)
+            if t.dtype != torch.float32 or not t.is_contiguous():
+                raise ValueError("apodization must be contiguous float32")
+    M = torch.empty(P, dtype=torch.float32, device=dev)
+    st = L.Status()
+    L.check(L.lib().ustbx_active_elements(tx_apo.data_ptr() if tx_apo is not None else None,
+                                          rx_apo.data_ptr() if rx_apo is not None else None, P, N, W, int(dim),
+                                          float(criterium), M.data_ptr(), _stream_ptr(dev), ctypes.byref(st)), st)
+    return M
+
+
+def coherence_factor(x: torch.Tensor, dim: int, M: torch.Tensor | None = None, flavour: int = L.CF_MATLAB):
+    """Mallart-Fink coherence factor of a device tensor (F, W, N, P)
+    complex64: (image, cf), each (F, W', N', P)."""
+    _require_cuda(x)
+    if x.dtype != torch.complex64 or x.dim() != 4:
+        raise ValueError("beamformed data must be a (F, W, N, P) complex64 tensor")
+    x = x.contiguous()
+    F, W, N, P = x.shape
+    shape = _summed_shape(dim, F, W, N, P)
+    image = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    cf = torch.empty(shape, dtype=torch.float32, device=x.device)
+    if M is not None and (M.dtype != torch.float32 or M.numel() != P or not M.is_contiguous()):
+        raise ValueError(f"M must be a contiguous float32 tensor of {P} elements")
+    st = L.Status()
+    L.check(L.lib().ustbx_coherence_factor(x.data_ptr(), image.data_ptr(), cf.data_ptr(),
+                                           M.data_ptr() if M is not None else None, P, N, W, F, int(dim),
+                                           int(flavour), _stream_ptr(x.device), ctypes.byref(st)), st)
+    return image, cf
+
+
 def hilbert(rf: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
     """Analytic signal along the last axis (MATLAB/scipy hilbert) of a
     float32 device tensor; returns complex64 of the same shape."""
@@ -165,6 +227,46 @@ class GeometryPlan:
         L.check(L.lib().ustbx_plan_run(self.handle, ch.data_ptr(), out.data_ptr(), sp,
                                        ctypes.byref(st)), st)
         return out
+
+    def run_cf(self, ch: torch.Tensor, M: torch.Tensor | None = None, flavour: int = L.CF_MATLAB,
+               image: torch.Tensor | None = None, cf: torch.Tensor | None = None,
+               stream: torch.cuda.Stream | None = None):
+        """Dimension transmit with the coherence factor over the receive
+        channels in the kernel's epilogue: (image [F, P] complex64, cf [F, P]
+        float32), equal to run() followed by coherence_factor(dim receive) up
+        to the FP32 rounding of the per-group sums. The partial-sum workspace
+        is kept with the plan (calls on one plan must not overlap)."""
+        _require_cuda(ch)
+        if ch.dtype != torch.complex64 or tuple(ch.shape) != (self.F, self.W, self.N, self.L):
+            raise ValueError(f"channel data must be complex64 {(self.F, self.W, self.N, self.L)}")
+        if image is None:
+            image = torch.empty((self.F, self.P), dtype=torch.complex64, device=ch.device)
+        if cf is None:
+            cf = torch.empty((self.F, self.P), dtype=torch.float32, device=ch.device)
+        for t, dt in ((image, torch.complex64), (cf, torch.float32)):
+            if tuple(t.shape) != (self.F, self.P) or t.dtype != dt or not t.is_contiguous():
+                raise ValueError(f"image / cf must be contiguous ({self.F}, {self.P}) complex64 / float32")
+        if M is not None and (M.dtype != torch.float32 or M.numel() != self.P or not M.is_contiguous()):
+            raise ValueError(f"M must be a contiguous float32 tensor of {self.P} elements")
+        need = int(L.lib().ustbx_plan_cf_workspace_bytes(self.handle))
+        ws = getattr(self, "_cf_workspace", None)
+        if ws is None or ws.numel() < need or ws.device != ch.device:
+            ws = torch.empty(max(need, 1), dtype=torch.uint8, device=ch.device)
+            self._cf_workspace = ws
+        sp = stream.cuda_stream if stream is not None else _stream_ptr(ch.device)
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_run_cf(self.handle, ch.data_ptr(), image.data_ptr(), cf.data_ptr(),
+                                          M.data_ptr() if M is not None else None, int(flavour), ws.data_ptr(),
+                                          ws.numel(), sp, ctypes.byref(st)), st)
+        return image, cf
+
+    def active_elements(self, dim: int, criterium: float = 0.16) -> torch.Tensor:
+        """M[p] of the coherence factor from the plan's FP32 apodization."""
+        M = torch.empty(self.P, dtype=torch.float32, device=self.device)
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_active_elements(self.handle, int(dim), float(criterium), M.data_ptr(),
+                                                   _stream_ptr(self.device), ctypes.byref(st)), st)
+        return M
 
     def active_terms(self) -> int:
         """(pixel, channel, wave, frame) terms with nonzero tx and rx weight."""

@@ -268,3 +268,5 @@ class BeamformedData:
 
     scan: Scan
     data: object
+    probe: object = None      # optional properties (beamformed_data.m:47-54)
+    sequence: object = None
