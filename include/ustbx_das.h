@@ -403,6 +403,24 @@ int ustbx_plan_run_cf(const ustbx_plan* plan, const void* ch, void* image, float
 int ustbx_plan_active_elements(const ustbx_plan* plan, int32_t dim, double criterium, float* M, void* stream,
                                ustbx_status* st);
 
+/* ---------------------------------------------------------------------------
+ * 6. RF -> IQ front end (preprocess.fast_demodulation).
+ * ------------------------------------------------------------------------- */
+#define USTBX_FIR_CAUSAL 0     /* MATLAB filter(b, 1, x), zero initial state (fast_demodulation.m:149) */
+#define USTBX_FIR_ZERO_PHASE 1 /* scipy filtfilt(b, 1, x) with its defaults (fast_demodulation.py:66) */
+
+/* Down-mix, low-pass and decimate RF traces: rf [traces][L] float32 (device;
+ * the channel data [F][W][N][L] is traces = F*W*N), iq [traces][ceil(L/down)]
+ * complex64 (device). iq[j] is sample j*down of the filtered signal
+ * rf(n) * exp(-1j 2 pi fmod (t0 + n/fs)) (fast_demodulation.m:136, 149, 171).
+ * b: the FIR taps (host, FP64; the caller designs them: fast_demodulation.m
+ * :267-275 or fast_demodulation.py:61-64), at most 767 causal / 384
+ * zero-phase; zero-phase needs L > 3 * ntaps, as filtfilt does. The caller
+ * updates the time origin (MATLAB: t0 - (ntaps-1)/2/fs) and the sampling
+ * frequency (fs / down). No allocation; asynchronous on stream. */
+int ustbx_demodulate(const float* rf, void* iq, int32_t L, int64_t traces, double fs, double t0, double fmod,
+                     const double* b, int32_t ntaps, int32_t down, int32_t mode, void* stream, ustbx_status* st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,10 @@ Contents
 - postprocess in FP64: ``coherent_compounding`` (coherent_compounding.m:41-73),
   ``coherence_factor`` with ``coherence_dimension`` and ``active_elements``
   (coherence_factor.m:58-151; python/src/ustb/postprocess/coherence_factor.py:19-50).
+- RF -> IQ in FP64: ``fast_demodulation`` with ``kaiser_lowpass`` and
+  ``fir_causal`` (fast_demodulation.m:58-183, 267-275),
+  ``python_fast_demodulation`` with ``fir_zero_phase``
+  (python/src/ustb/preprocess/fast_demodulation.py:24-85).
 
 Array layout: MATLAB column-major [L,N,W,F] is held as NumPy C-order
 (F, W, N, L); [P,N] as (N, P); [P,W] as (W, P); outputs (F, W', N', P).
@@ -655,3 +659,117 @@ def coherence_factor(data, dim=DIM_BOTH, M=None, flavour="matlab"):
             cf = (np.abs(coherent) ** 2 / incoherent) * (1.0 / Mp)[None, None, None, :]
     cf[np.isnan(cf)] = 0
     return cf * coherent, cf
+
+
+# ---------------------------------------------------------------------------
+# RF -> IQ front end: preprocess.fast_demodulation, layout (F, W, N, L)
+# ---------------------------------------------------------------------------
+
+def kaiser_lowpass(f_pass, f_stop, fs, dev=1e-3):
+    """The low-pass FIR of fast_demodulation.m:267-275: kaiserord([f_pass
+    f_stop], [1 0], [dev dev], fs) and fir1(N, Wn, 'low', kaiser(N+1, beta),
+    'noscale'), restated from Kaiser's published design formulas (MATLAB
+    Signal Processing Toolbox, not in the reference): A = -20 log10(dev);
+    beta = 0.1102 (A - 8.7) for A > 50, 0.5842 (A - 21)^0.4 + 0.07886 (A - 21)
+    for 21 <= A <= 50, else 0; order N = ceil((A - 7.95) / (2.285 dw)), dw =
+    2 pi (f_stop - f_pass) / fs; Wn = (f_pass + f_stop) / fs (normalised to
+    Nyquist); taps h[k] = Wn sinc(Wn (k - N/2)) I0(beta sqrt(1 - (2k/N -
+    1)^2)) / I0(beta), k = 0..N, unscaled. FP64."""
+    A = -20.0 * math.log10(dev)
+    if A > 50:
+        beta = 0.1102 * (A - 8.7)
+    elif A >= 21:
+        beta = 0.5842 * (A - 21.0) ** 0.4 + 0.07886 * (A - 21.0)
+    else:
+        beta = 0.0
+    dw = 2.0 * math.pi * (f_stop - f_pass) / fs
+    N = int(math.ceil((A - 7.95) / (2.285 * dw)))
+    Wn = (f_pass + f_stop) / fs
+    k = np.arange(N + 1, dtype=np.float64)
+    h = Wn * np.sinc(Wn * (k - N / 2.0))
+    w = np.i0(beta * np.sqrt(np.clip(1.0 - (2.0 * k / N - 1.0) ** 2, 0.0, None))) / np.i0(beta)
+    return h * w
+
+
+def fir_causal(b, x):
+    """filter(b, 1, x, [], dim) with zero initial state along the last axis:
+    y[n] = sum_k b[k] x[n - k], x[n < 0] = 0."""
+    x = np.asarray(x)
+    y = np.zeros(x.shape, dtype=np.result_type(x, np.float64))
+    L = x.shape[-1]
+    for k, bk in enumerate(np.asarray(b, dtype=np.float64)):
+        if k < L:
+            y[..., k:] += bk * x[..., :L - k]
+    return y
+
+
+def fir_zero_phase(b, x):
+    """scipy.signal.filtfilt(b, 1, x) along the last axis with its defaults
+    (padtype 'odd', padlen 3 * len(b), steady-state initial conditions),
+    restated for a FIR: odd extension by padlen samples at both ends, a
+    forward pass whose input before the start holds the first extended
+    sample, a backward pass likewise from the end, then the padding cut."""
+    x = np.asarray(x)
+    K = len(b)
+    pad = 3 * K
+    L = x.shape[-1]
+    if L <= pad:
+        raise ValueError("The length of the input vector x must be greater than padlen")
+    left = 2.0 * x[..., :1] - x[..., pad:0:-1]
+    right = 2.0 * x[..., -1:] - x[..., -2:-pad - 2:-1]
+    e = np.concatenate([left, x, right], axis=-1)
+    Lp = e.shape[-1]
+    bb = np.asarray(b, dtype=np.float64)
+    y = np.zeros(e.shape, dtype=np.result_type(e, np.float64))
+    for k, bk in enumerate(bb):          # forward: e[n < 0] = e[0]
+        idx = np.maximum(np.arange(Lp) - k, 0)
+        y += bk * e[..., idx]
+    z = np.zeros_like(y)
+    for k, bk in enumerate(bb):          # backward: y[n >= Lp] = y[Lp - 1]
+        idx = np.minimum(np.arange(Lp) + k, Lp - 1)
+        z += bk * y[..., idx]
+    return z[..., pad:pad + L]
+
+
+def fast_demodulation(rf, fs, t0, fmod, fdown=None, lowpass=(0.5, 1.0)):
+    """preprocess.fast_demodulation (fast_demodulation.m:58-183), FP64: down-mix
+    by exp(-1j 2 pi fmod t), t = t0 + (0:L-1)/fs (line 136), the Kaiser
+    low-pass with zero initial state (line 149), decimation by Ndown =
+    floor(fs / fdown) from the first sample (line 171). fdown defaults to
+    2 fmod (line 81). Returns (iq, fs / Ndown, t0 - (len(b) - 1) / 2 / fs, b)
+    (lines 90-91, 177-179)."""
+    if fdown is None:
+        fdown = 2.0 * fmod
+    Ndown = int(math.floor(fs / fdown))
+    b = kaiser_lowpass(lowpass[0] * fmod, lowpass[1] * fmod, fs)
+    delay = (len(b) - 1) / 2.0 / fs
+    L = np.asarray(rf).shape[-1]
+    t = t0 + np.arange(L, dtype=np.float64) / fs
+    x = np.asarray(rf, dtype=np.float64) * np.exp(-1j * 2.0 * np.pi * fmod * t)
+    y = fir_causal(b, x)
+    return y[..., ::Ndown], fs / Ndown, t0 - delay, b
+
+
+def python_fast_demodulation(rf, fs, t0, fc, fdown=None, lowpass=(0.5, 1.0)):
+    """The Python reference's FastDemodulation.go
+    (python/src/ustb/preprocess/fast_demodulation.py:24-85), FP64: the same
+    down-mix, scipy.signal.firwin(numtaps, fc * lowpass[0], fs=fs) with
+    numtaps = min(65, L // 3 * 2 - 1) (at least 3, odd), zero-phase
+    filtfilt, decimation by floor(fs / fdown) (default floor(fs / (4 fc)));
+    t0 is kept. Returns (iq, fs_out, t0, b)."""
+    from scipy.signal import firwin
+
+    L = np.asarray(rf).shape[-1]
+    t = t0 + np.arange(L, dtype=np.float64) / fs
+    x = np.asarray(rf, dtype=np.float64) * np.exp(-1j * 2.0 * np.pi * fc * t)
+    numtaps = min(65, L // 3 * 2 - 1)
+    if numtaps < 3:
+        numtaps = 3
+    if numtaps % 2 == 0:
+        numtaps += 1
+    b = firwin(numtaps, fc * lowpass[0], fs=fs)
+    y = fir_zero_phase(b, x)
+    dec = max(1, int(math.floor(fs / fdown))) if fdown is not None else max(1, int(math.floor(fs / (4 * fc))))
+    if dec > 1:
+        return y[..., ::dec], fs / dec, t0, b
+    return y, fs, t0, b

@@ -29,6 +29,7 @@ INTERP_LINEAR, INTERP_CUBIC = 0, 1
 EDGE_DASC, EDGE_PYTHON = 0, 1
 FLAG_DASC_ORDER, FLAG_SEPARABLE_PHASE, FLAG_REF_PHASE = 1, 2, 4
 CF_MATLAB, CF_PYTHON = 0, 1
+FIR_CAUSAL, FIR_ZERO_PHASE = 0, 1
 SCAN_LINEAR, SCAN_SECTOR, SCAN_XYZ = 0, 1, 2
 WAVE_PLANE, WAVE_SPHERICAL, WAVE_PHOTOACOUSTIC = 0, 1, 2
 TXMODEL_SPHERICAL, TXMODEL_UNIFIED, TXMODEL_HYBRID, TXMODEL_BLENDED = 1, 2, 3, 4
@@ -142,6 +143,9 @@ EXPORTS = {
                                          c_int64, c_void_p, ctypes.POINTER(Status)]),
     "ustbx_plan_active_elements": (ctypes.c_int, [c_void_p, c_int32, ctypes.c_double, c_void_p, c_void_p,
                                                   ctypes.POINTER(Status)]),
+    "ustbx_demodulate": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, c_void_p, c_int32, c_int32, c_int32, c_void_p,
+                                        ctypes.POINTER(Status)]),
 }
 
 _lib = None

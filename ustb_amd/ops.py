@@ -12,6 +12,8 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
+
 import torch
 
 from . import _lib as L
@@ -128,6 +130,26 @@ def coherence_factor(x: torch.Tensor, dim: int, M: torch.Tensor | None = None, f
                                            M.data_ptr() if M is not None else None, P, N, W, F, int(dim),
                                            int(flavour), _stream_ptr(x.device), ctypes.byref(st)), st)
     return image, cf
+
+
+def demodulate(rf: torch.Tensor, fs: float, t0: float, fmod: float, b, down: int,
+               mode: int = L.FIR_CAUSAL) -> torch.Tensor:
+    """RF -> IQ (down-mix, FIR with taps b, keep every down-th sample) of a
+    float32 device tensor whose last axis is time; returns complex64 with the
+    last axis ceil(L / down)."""
+    _require_cuda(rf)
+    if rf.dtype != torch.float32:
+        raise TypeError("RF data must be float32")
+    rf = rf.contiguous()
+    Ls = rf.shape[-1]
+    traces = rf.numel() // max(Ls, 1)
+    taps = np.ascontiguousarray(np.asarray(b, dtype=np.float64))
+    out = torch.empty(tuple(rf.shape[:-1]) + ((Ls + down - 1) // down,), dtype=torch.complex64, device=rf.device)
+    st = L.Status()
+    L.check(L.lib().ustbx_demodulate(rf.data_ptr(), out.data_ptr(), Ls, traces, float(fs), float(t0), float(fmod),
+                                     taps.ctypes.data, int(taps.size), int(down), int(mode),
+                                     _stream_ptr(rf.device), ctypes.byref(st)), st)
+    return out
 
 
 def hilbert(rf: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
