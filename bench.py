@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--frames", type=int, default=600, help="C5: requests served (the trace's first ones)")
     ap.add_argument("--rate", type=float, default=2000.0, help="C5: Poisson arrival rate, frames/s")
     ap.add_argument("--slice-ms", type=float, default=2.0, help="C5: estimated work per serving iteration")
+    ap.add_argument("--long-slices", type=int, default=8,
+                    help="C5: iteration length, in slices, while no small frame is pending")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--phase", default="auto", choices=["auto", "separable", "dasc"],
@@ -543,7 +545,8 @@ def run_serve(args, torch, dev):
     plans = [make_plan(c.probe, c.scan, c.sequence, separable_phase=True, **c.plan_kwargs()) for c in cfgs]
     torch.cuda.synchronize()
     log(f"C5: {len(cfgs)} configurations, data and plans in {time.time() - t:.1f}s")
-    srv = Server(plans, datasets, hbm_budget=64 << 30, slice_ms=args.slice_ms, tiles=(8, 4))
+    srv = Server(plans, datasets, hbm_budget=64 << 30, slice_ms=args.slice_ms, tiles=(8, 4),
+                 long_slices=args.long_slices)
     log(f"calibrated ms per position {[round(x, 4) for x in srv.ms_per_pos]}, positions {srv.positions}")
     names = [c.name for c in cfgs]
     trace = poisson_trace(args.frames, rate=args.rate, n_configs=len(cfgs), n_data=len(datasets[0]), seed=5)
@@ -584,7 +587,7 @@ def run_serve(args, torch, dev):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"C5 serving, Poisson {args.rate:g} frames/s, first {len(trace)} requests of the "
-                               f"trace, uniform over {names}", "hbm_budget_GiB": 64, "slice_ms": args.slice_ms,
+                               f"trace, uniform over {names}", "hbm_budget_GiB": 64, "slice_ms": args.slice_ms, "long_slices": args.long_slices,
                    "waves_per_split": srv.tiles[0], "splits": srv.tiles[1],
                    "units_per_frame": {n: c.units for n, c in zip(names, cfgs)}},
         "serving": res.summary(names),

@@ -403,11 +403,12 @@ struct ustbx_batcher {
   int nblocks;
   int max_acqs;
   int64_t max_items;
-  char* dev;      // queue offsets | descriptors | items | counters
-  char* staging;  // pinned host copy of the first three
+  char* dev;         // queue offsets | descriptors | items | counters
+  char* staging[2];  // pinned host copies of the first three, used in turn
   size_t b_qoff, b_acq;
-  hipEvent_t staged;  // the last submission's uploads have left staging
-  bool pending;
+  hipEvent_t staged[2];  // that buffer's last uploads have left it
+  bool pending[2];
+  int cur;               // the buffer the next submission fills
 };
 
 extern "C" {
@@ -444,8 +445,10 @@ int ustbx_batcher_create(int32_t interp, int32_t max_acqs, int64_t max_items, in
   b->b_acq = al(sizeof(RxAcq) * (size_t)max_acqs);
   const size_t b_items = al(sizeof(int4) * (size_t)max_items), b_ctr = al(sizeof(int) * kQueues);
   if (e == hipSuccess) e = hipMalloc(&b->dev, b->b_qoff + b->b_acq + b_items + b_ctr);
-  if (e == hipSuccess) e = hipHostMalloc(&b->staging, b->b_qoff + b->b_acq + b_items, hipHostMallocDefault);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+  for (int i = 0; i < 2; ++i) {
+    if (e == hipSuccess) e = hipHostMalloc(&b->staging[i], b->b_qoff + b->b_acq + b_items, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     ustbx_batcher_destroy(b);
     return hip_fail(st, e, "batcher setup");
@@ -462,14 +465,19 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
   if (count > b->max_acqs) return set_status(st, USTBX_E_CAPACITY, "more acquisitions than the batcher holds");
   if (!plans || !ch || !out || !pos_begin || !pos_end) return set_status(st, USTBX_E_ARG, "NULL argument array");
   hipError_t e = hipSuccess;
-  if (b->pending) {
-    e = hipEventSynchronize(b->staged);  // staging is free again
+  // two staging buffers in turn: this one's last uploads were queued two
+  // submissions ago, so with at most one submission in flight the host does
+  // not wait here
+  const int c = b->cur;
+  if (b->pending[c]) {
+    e = hipEventSynchronize(b->staged[c]);  // this staging buffer is free again
     if (e != hipSuccess) return hip_fail(st, e, "batcher staging");
-    b->pending = false;
+    b->pending[c] = false;
   }
-  int* qoff = reinterpret_cast<int*>(b->staging);
-  RxAcq* acqs = reinterpret_cast<RxAcq*>(b->staging + b->b_qoff);
-  int4* items = reinterpret_cast<int4*>(b->staging + b->b_qoff + b->b_acq);
+  char* stg = b->staging[c];
+  int* qoff = reinterpret_cast<int*>(stg);
+  RxAcq* acqs = reinterpret_cast<RxAcq*>(stg + b->b_qoff);
+  int4* items = reinterpret_cast<int4*>(stg + b->b_qoff + b->b_acq);
   // count first, so that capacity errors leave nothing half-written
   int64_t n_items = 0;
   for (int i = 0; i < count; ++i) {
@@ -509,12 +517,13 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
     acqs[i].out = reinterpret_cast<float2*>(out[i]);
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  e = hipMemcpyAsync(b->dev, b->staging, b->b_qoff + sizeof(RxAcq) * count, hipMemcpyHostToDevice, s);
+  e = hipMemcpyAsync(b->dev, stg, b->b_qoff + sizeof(RxAcq) * count, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && at > 0)
     e = hipMemcpyAsync(b->dev + b->b_qoff + b->b_acq, items, sizeof(int4) * at, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipEventRecord(b->staged, s);
+  if (e == hipSuccess) e = hipEventRecord(b->staged[c], s);
   if (e != hipSuccess) return hip_fail(st, e, "batcher upload");
-  b->pending = true;
+  b->pending[c] = true;
+  b->cur = 1 - c;
   if (at == 0) return ok(st);
   int* d_ctr = reinterpret_cast<int*>(b->dev + b->b_qoff + b->b_acq + ((sizeof(int4) * b->max_items + 255) & ~size_t(255)));
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, d_ctr);
@@ -531,9 +540,11 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
 
 void ustbx_batcher_destroy(ustbx_batcher* b) {
   if (!b) return;
-  if (b->pending) (void)hipEventSynchronize(b->staged);
-  if (b->staged) (void)hipEventDestroy(b->staged);
-  if (b->staging) (void)hipHostFree(b->staging);
+  for (int i = 0; i < 2; ++i) {
+    if (b->pending[i]) (void)hipEventSynchronize(b->staged[i]);
+    if (b->staged[i]) (void)hipEventDestroy(b->staged[i]);
+    if (b->staging[i]) (void)hipHostFree(b->staging[i]);
+  }
   (void)hipFree(b->dev);
   delete b;
 }

@@ -4,8 +4,8 @@ Requests (one frame each, of one of several probe configurations) arrive on
 a trace. The server keeps every configuration's plan (geometry, delays and
 apodization tables) resident, admits a request when its input and output
 buffers fit in the HBM budget (smallest input first, with a bounded amount
-of input queued on the copy stream), copies its channel data in on a copy
-stream,
+of input and at most one large frame queued on the copy stream), copies its
+channel data in on a copy stream,
 and runs iterations of the batcher on a compute stream: each iteration takes
 the ready requests in order of least remaining work (then arrival) and gives
 each a slice of its tiles until about `slice_ms` of estimated work is packed
@@ -191,6 +191,7 @@ class Server:
         waiting: list = []  # heap of (input bytes, arrival, rid, request)
         staged: deque[_Live] = deque()
         copying = 0  # input bytes queued on the copy stream
+        copying_large = 0  # of them, frames too large for one short iteration
         ready: list[_Live] = []
         running: deque[tuple[torch.cuda.Event, list[_Live]]] = deque()
         draining: deque[_Live] = deque()
@@ -210,8 +211,9 @@ class Server:
                     heapq.heappush(waiting, (self.in_bytes[r.config], r.arrival, r.rid, r))
                     nxt += 1
                 # admission: smallest input first, under the HBM budget, with
-                # at most copy_window bytes queued on the copy stream (so a
-                # small frame's copy never waits behind many large ones)
+                # at most copy_window bytes and one large frame's input queued
+                # on the copy stream (so a small frame's copy waits behind at
+                # most one large copy)
                 while waiting:
                     r = waiting[0][3]
                     need = self.in_bytes[r.config] + self.out_bytes[r.config]
@@ -219,8 +221,12 @@ class Server:
                         break
                     if copying + self.in_bytes[r.config] > self.copy_window and copying > 0:
                         break
+                    large = not self._small(r.config)
+                    if large and copying_large > 0:
+                        break
                     heapq.heappop(waiting)
                     copying += self.in_bytes[r.config]
+                    copying_large += 1 if large else 0
                     p = self.plans[r.config]
                     ch = torch.empty((p.F, p.W, p.N, p.L), dtype=torch.complex64, device=dev)
                     out = torch.empty(p.out_shape, dtype=torch.complex64, device=dev)
@@ -234,6 +240,7 @@ class Server:
                 while staged and staged[0].ready.query():
                     lv = staged.popleft()
                     copying -= self.in_bytes[lv.req.config]
+                    copying_large -= 0 if self._small(lv.req.config) else 1
                     ready.append(lv)
                     progressed = True
                 # an iteration over the ready requests
