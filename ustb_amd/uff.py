@@ -124,6 +124,7 @@ class MatrixArray(Probe):
         y0 = y0 - np.sum(y0) / N_y
         X, Y = np.meshgrid(x0, y0, indexing="ij")
         super().__init__(X.ravel(order="F"), Y.ravel(order="F"))
+        self.N_x, self.N_y, self.pitch_x, self.pitch_y = N_x, N_y, pitch_x, pitch_y
 
 
 class Scan:
