@@ -111,13 +111,16 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
     const bool ok = d >= lo_ok && d < lim;
     const int n0 = (int)d;
     const float b = __builtin_amdgcn_fractf(d);
-    const unsigned off = ok ? (unsigned)(n0 * ST) + nb + (unsigned)(wc * wst) : (unsigned)kOOB;
+    // the first gather's offset folded into the wave-uniform part (n0 >= 1
+    // wherever ok holds, so the sum stays non-negative)
+    const unsigned off =
+        ok ? (unsigned)(n0 * ST) + (nb + (unsigned)(wc * wst) + (unsigned)M::kLo) : (unsigned)kOOB;
     float4 q[FC][M::kLoads];
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
       const unsigned o = off + (unsigned)(f * fb);
-      q[f][0] = buf_load16(rs, (int)(o + (unsigned)M::kLo));
-      if (M::kLoads == 2) q[f][M::kLoads - 1] = buf_load16(rs, (int)(o + (unsigned)M::kHi));
+      q[f][0] = buf_load16(rs, (int)o);
+      if (M::kLoads == 2) q[f][M::kLoads - 1] = buf_load16(rs, (int)(o + (unsigned)(M::kHi - M::kLo)));
     }
     __builtin_amdgcn_sched_barrier(0);
     float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;

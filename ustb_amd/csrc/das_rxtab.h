@@ -103,9 +103,11 @@ __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const
         const bool ok = d >= lo_ok && d < lim;
         const int n0 = (int)d;
         bb[i] = __builtin_amdgcn_fractf(d);
-        const int off = ok ? n0 * M::kStride + (nb + wb[j]) : kOOB;
-        q[i][0] = buf_load16(rs, off + M::kLo);
-        if (M::kLoads == 2) q[i][M::kLoads - 1] = buf_load16(rs, off + M::kHi);
+        // the first gather's offset folded into the wave-uniform base, so both
+        // gathers address off and off + (kHi - kLo) (an immediate offset)
+        const int off = ok ? n0 * M::kStride + (nb + wb[j] + M::kLo) : kOOB;
+        q[i][0] = buf_load16(rs, off);
+        if (M::kLoads == 2) q[i][M::kLoads - 1] = buf_load16(rs, off + (M::kHi - M::kLo));
       }
     }
     // keep the batch's gathers ahead of their uses (the scheduler would
