@@ -37,7 +37,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; C4: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 3; C4: 1)")
-    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4", "C5"])
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4", "C5", "RTB", "RF2IQ"],
+                    help="RTB: focused transmits (retrospective transmit beamforming), the focused-wave "
+                         "delay models; RF2IQ: the RF front end (fast_demodulation and hilbert) on STA RF "
+                         "of 128 waves x 128 channels x 2048 samples")
+    ap.add_argument("--tx-model", default="hybrid", choices=["spherical", "unified", "hybrid", "blended"],
+                    help="RTB: spherical_transmit_delay_model of the focused waves")
     ap.add_argument("--frames", type=int, default=600, help="C5: requests served (the trace's first ones)")
     ap.add_argument("--rate", type=float, default=2000.0, help="C5: Poisson arrival rate, frames/s")
     ap.add_argument("--slice-ms", type=float, default=2.0, help="C5: estimated work per serving iteration")
@@ -130,6 +135,9 @@ def main():
     if args.config == "C5":
         run_serve(args, torch, dev)
         return
+    if args.config == "RF2IQ":
+        run_frontend(args, torch, dev)
+        return
     if args.abi == "matrix":
         run_matrix(args, torch, dev)
         return
@@ -137,14 +145,18 @@ def main():
         run_cf(args, torch, dev)
         return
     t_gen = time.time()
-    acq = configs.BUILDERS[args.config]()
+    acq = build_acquisition(args)
     print(f"[bench] {acq.name}: data {acq.data.shape} generated in {time.time() - t_gen:.1f}s", file=sys.stderr,
           flush=True)
     ch = torch.from_numpy(acq.data).to(dev)
     if args.phase == "auto":
         args.phase = "dasc" if int(acq.dimension) == 2 else "separable"
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter()
     plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=(args.phase == "separable"),
                      **acq.plan_kwargs())
+    torch.cuda.synchronize()
+    plan_ms = (time.perf_counter() - t_plan) * 1e3
     if args.variant >= 0:
         plan.set_variant(args.variant)
     if args.wave_chunk or args.split:
@@ -207,6 +219,7 @@ def main():
                    "interp": "cubic" if acq.interpolation == 1 else "linear",
                    "dimension": int(acq.dimension), "phase": args.phase,
                    "geometry": "on-device, resident per-configuration tables",
+                   "plan_build_ms": round(plan_ms, 2),
                    "kernel_variant": variant, "waves_per_split": wave_chunk, "splits": split},
         "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": VALU_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / VALU_PEAK_TFLOPS, 4),
@@ -229,6 +242,126 @@ def main():
     print(json.dumps(result))
 
 
+def build_acquisition(args):
+    from ustb_amd import configs
+    from ustb_amd.enums import SphericalTransmitDelayModel
+
+    if args.config == "RTB":
+        return configs.rtb(tx_model=int(SphericalTransmitDelayModel[args.tx_model]))
+    return configs.BUILDERS[args.config]()
+
+
+def _timed(torch, stream, fn, warmup, steps):
+    """(kernel ms from HIP events on the launch stream, wall ms) per step."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, (time.perf_counter() - t0) * 1e3 / steps
+
+
+def run_frontend(args, torch, dev):
+    """The RF front end on the device: preprocess.fast_demodulation
+    (fast_demodulation.m:58-183: down-mix, Kaiser low-pass, decimation by
+    floor(fs / (2 f0)) = 2), the Python reference's zero-phase FastDemodulation
+    (decimation floor(fs / (4 f0)) = 1) and the analytic signal of das.m's RF
+    branch (das.m:211-214, hipFFT), on STA RF of 128 waves x 128 channels x
+    2048 samples at 20.8 MHz (f0 5.2 MHz): seeded noise plus tones at f0.
+    One step = one pass of fast_demodulation over the whole acquisition;
+    units are RF samples. Each is HBM-bound: algorithmic bytes = RF read once
+    (4 B per sample) + output written once (8 B per output sample)."""
+    from oracle import oracle as O
+    from ustb_amd import _lib as L
+    from ustb_amd import ops
+    from ustb_amd.preprocess import matlab_lowpass, python_lowpass
+
+    W, N, Ls, fs, f0, t0 = 128, 128, 2048, 20.8e6, 5.2e6, 0.0
+    rng = np.random.default_rng(7)
+    t = np.arange(Ls) / fs
+    rf = (0.3 * rng.standard_normal((1, W, N, Ls))
+          + np.cos(2 * np.pi * f0 * t + rng.uniform(0, 6, (1, W, N, 1)))).astype(np.float32)
+    x = torch.from_numpy(rf).to(dev)
+    S = rf.size
+    stream = torch.cuda.current_stream(dev)
+    b_m = matlab_lowpass(0.5 * f0, f0, fs)
+    down_m = int(math.floor(fs / (2 * f0)))
+    b_p = python_lowpass(Ls, 0.5 * f0, fs)
+    down_p = int(math.floor(fs / (4 * f0)))
+    iq_m = torch.empty((1, W, N, (Ls + down_m - 1) // down_m), dtype=torch.complex64, device=dev)
+    an = torch.empty((1, W, N, Ls), dtype=torch.complex64, device=dev)
+    copy = copy_probe(torch, dev)
+
+    def line(name, fn, out_elems, taps):
+        k_ms, w_ms = _timed(torch, stream, fn, args.warmup, args.steps)
+        alg = 4 * S + 8 * out_elems
+        gbs = alg / (k_ms * 1e-3) / 1e9
+        log(f"{name}: {k_ms:.4f} ms/step (kernel), {w_ms:.4f} ms wall, {gbs:.0f} GB/s")
+        return {"kernel_ms": round(k_ms, 4), "wall_ms": round(w_ms, 4), "taps": taps,
+                "Gsamples_s": round(S / (w_ms * 1e-3) / 1e9, 3), "algorithmic_bytes": alg,
+                "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "frac_of_achievable": round(gbs / copy["GBs"], 4)}
+
+    demod = line("fast_demodulation (causal Kaiser, /2)",
+                 lambda: ops.demodulate(x, fs, t0, f0, b_m, down_m, L.FIR_CAUSAL), iq_m.numel(), len(b_m))
+    zp = line("FastDemodulation (zero-phase, /1)",
+              lambda: ops.demodulate(x, fs, t0, f0, b_p, down_p, L.FIR_ZERO_PHASE), S, len(b_p))
+    hil = line("hilbert", lambda: ops.hilbert(x, an), S, 0)
+
+    errors = {}
+    if not args.no_check:
+        k = 256                                           # traces checked against the oracle
+        sub = rf.reshape(-1, Ls)[:k]
+        got = ops.demodulate(x, fs, t0, f0, b_m, down_m, L.FIR_CAUSAL).reshape(-1, iq_m.shape[-1])[:k].cpu().numpy()
+        ref = O.fast_demodulation(sub, fs, t0, f0)[0]
+        errors["fast_demodulation_vs_oracle"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        got = ops.demodulate(x, fs, t0, f0, b_p, down_p, L.FIR_ZERO_PHASE).reshape(-1, Ls)[:k].cpu().numpy()
+        ref = O.python_fast_demodulation(sub, fs, t0, f0)[0]
+        errors["python_fast_demodulation_vs_oracle"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        got = ops.hilbert(x, an).reshape(-1, Ls)[:k].cpu().numpy()
+        ref = O.hilbert(sub.astype(np.float64))
+        errors["hilbert_vs_scipy"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        log(f"errors {errors}")
+    result = {
+        "metric": "RF front end Gsamples/s (RF time samples)",
+        "value": demod["Gsamples_s"],
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": demod["wall_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "RF2IQ fast_demodulation of STA RF, 128 waves x 128 channels x 2048 samples, "
+                               "20.8 MHz, f0 5.2 MHz", "rf_samples": S},
+        "roofline": {"bound": "hbm", "achieved": demod["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": demod["frac"], "traffic": None, "kernel_ms": demod["kernel_ms"],
+                     "achievable_copy_GBs": copy["GBs"], "frac_of_achievable": demod["frac_of_achievable"]},
+        "fast_demodulation": demod,
+        "python_fast_demodulation": zp,
+        "hilbert": hil,
+        "errors": errors,
+    }
+    if not args.no_cpu_baseline:
+        sub = rf.reshape(-1, Ls)
+        O.fast_demodulation(sub[:64], fs, t0, f0)
+        t1 = time.perf_counter()
+        O.fast_demodulation(sub, fs, t0, f0)
+        dt = time.perf_counter() - t1
+        result["cpu_baseline"] = {"value": round(sub.size / dt / 1e9, 4), "unit": "Gsamples/s", "cores": 1,
+                                  "kind": "port", "sample": f"the oracle's fast_demodulation (NumPy/SciPy lfilter, "
+                                                            f"FP64) on {sub.shape[0]} of {W * N} traces, {dt:.2f} s"}
+    print(json.dumps(result))
+
+
 def run_cf(args, torch, dev):
     """das (dimension transmit) + postprocess.coherence_factor over the
     receive channels: the fused kernel (ustbx_plan_run_cf, no per-channel
@@ -239,7 +372,7 @@ def run_cf(args, torch, dev):
     from ustb_amd import configs, ops
     from ustb_amd.geometry import make_plan
 
-    acq = configs.BUILDERS[args.config]()
+    acq = build_acquisition(args)
     if int(acq.dimension) != 2:
         raise SystemExit("--postprocess cf needs a dimension-transmit workload (C3)")
     ch = torch.from_numpy(acq.data).to(dev)
@@ -321,7 +454,7 @@ def run_matrix(args, torch, dev):
     from ustb_amd import configs, ops
     from ustb_amd.geometry import make_plan
 
-    acq = configs.BUILDERS[args.config]()
+    acq = build_acquisition(args)
     ch = torch.from_numpy(acq.data).to(dev)
     plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=False, **acq.plan_kwargs())
     ta, ra, td, rd = plan.materialize()

@@ -46,6 +46,11 @@ def geom_cases():
     cases["cpwc_tukey"] = dict(probe=prb, scan=scan, seq=seq, rx=Apodization(Window.tukey25, f_number=1.7),
                                tx=Apodization(Window.tukey25, f_number=1.7), fs=10.4e6, fd=5.2e6, L=512,
                                model=3, scan_kind="linear")
+    # CPWC on a scan that starts at depth 0: apodization.m's windows give NaN
+    # receive weights on the whole first row (x / 0 after the aperture clamp)
+    cases["cpwc_depth0"] = dict(cases["cpwc_tukey"], scan=LinearScan(np.linspace(-5e-3, 5e-3, 24),
+                                                                      np.linspace(0.0, 30e-3, 40)),
+                                rx=Apodization(Window.hanning, f_number=1.7))
     # RF plane wave 0 deg, hanning rx, no tx apodization (C1 in small)
     cases["pw_rf_hanning"] = dict(probe=LinearArray(24, 0.3e-3), scan=LinearScan(np.linspace(-4e-3, 4e-3, 20),
                                                                                    np.linspace(5e-3, 25e-3, 33)),

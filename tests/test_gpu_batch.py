@@ -36,7 +36,7 @@ def _mix(interp, frames=(1, 2, 1)):
         case = CASES[name]
         F = frames[k % len(frames)]
         plan = _plan(case, F=F, interp=interp)
-        if not plan.has_variant(2):
+        if not plan.has_variant(2) or not plan.batchable:
             plan.close()
             continue
         plans.append(plan)
@@ -137,5 +137,11 @@ def test_batch_rejections(cuda):
     with pytest.raises(L.USTBXError, match="buffers first"):
         b.launch()
     b.close()
-    for p in (lin, cub, tx):
+    # NaN receive weights on the scan's first row: the plan runs variant 2
+    # alone, batches refuse it
+    nan_plan = _plan(CASES["cpwc_depth0"])
+    assert nan_plan.variant[0] == 2 and not nan_plan.batchable and lin.batchable
+    with pytest.raises(L.USTBXError, match="non-finite"):
+        Batch([lin, nan_plan])
+    for p in (lin, cub, tx, nan_plan):
         p.close()

@@ -112,3 +112,29 @@ def test_host_entry_matches_device(cuda):
                                     pb["rx_delay"].ctypes.data, out.ctypes.data, 0, ctypes.byref(st))
     L.check(rc, st)
     assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("dim", [0, 3])
+def test_nonfinite_and_huge_delays_and_nan_weights(cuda, dim):
+    """das_c converts d = (tau - t0) * fs with (int): NaN, +-Inf and |d| >= 2^31
+    fail its 0 < n0 < L-1 test and the term is skipped; a NaN weight is nonzero
+    for das_c's `if (apo)` tests, so it propagates into the pixel's sum."""
+    pb = matrix_problem(seed=60 + dim, P=300, N=6, W=3, F=2, L=64, edges=False)
+    bad = np.array([np.nan, np.inf, -np.inf, 3e9, -3e9, 2.2e9, 1e38], np.float32)
+    pb["tx_delay"][:, :len(bad)] = bad
+    pb["rx_delay"][:, 10:10 + len(bad)] = bad
+    pb["rx_apo"][2, 20] = np.nan
+    pb["tx_apo"][1, 21] = np.nan
+    pb["rx_apo"][:, 21] = 1.0
+    ref = O.das_c(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"], pb["rx_delay"], 0.0, dim)
+    assert np.isnan(ref).any() and np.isfinite(ref[..., :len(bad)]).all()
+    print(f"case nonfinite delays dim={dim}", flush=True)
+    got = _run(pb, cuda, 0.0, dim, dasc_order=True)
+    assert np.array_equal(got, ref, equal_nan=True)
+    for split in (2, 8):
+        got = _run(pb, cuda, 0.05, dim, split=split)
+        ref = O.das_c(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"], pb["rx_delay"],
+                      0.05, dim)
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        ok = ~np.isnan(ref)
+        assert float(np.max(np.abs(got[ok] - ref[ok]))) <= TOL * float(np.max(np.abs(ref[ok])))

@@ -33,6 +33,15 @@ C4  ragged batch of 64 acquisitions drawn (seeded) from
     (a) and (c), Tukey-25 f# 1.7 for (b); linear interpolation; dimension
     both. Acquisitions of one configuration share its plan; each draws one of
     a few seeded datasets of its kind.
+RTB retrospective transmit beamforming of focused transmits, the setup of
+    examples/fresnel/linear_array/RTB_linear_array.m: 128-element linear
+    array (0.3 mm), 31 focused waves with sources at x in -15..15 mm,
+    z = 20 mm and origins below them on the array, RF at 50 MHz (f0 5.2 MHz,
+    fractional bandwidth 0.6) beamformed from its analytic signal, 3500
+    samples; 512 x 512 scan (x +-19.2 mm, z 0-45 mm); rx Hamming f# 2.5 and
+    tx Hamming f# 1.5, both with a 3 mm minimum aperture; linear; dimension
+    both; transmit delays by the hybrid model (das.m's default), or another
+    spherical_transmit_delay_model.
 
 ``scale`` shrinks pixels/elements/waves/samples for tests.
 """
@@ -44,7 +53,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import synth
-from .enums import Dimension, Interpolation, Window
+from .enums import Dimension, Interpolation, SphericalTransmitDelayModel, Window
 from .uff import Apodization, LinearArray, LinearScan, MatrixArray, Probe, Scan, SectorScan, Wave
 
 
@@ -64,6 +73,7 @@ class Acquisition:
     interpolation: int
     c: float = 1540.0
     t0: float = 0.0
+    tx_model: int = SphericalTransmitDelayModel.hybrid
     data: np.ndarray | None = None          # (F, W, N, L) complex64
     targets: np.ndarray | None = None       # point targets (K, 3)
     meta: dict = field(default_factory=dict)
@@ -81,7 +91,7 @@ class Acquisition:
         kw = dict(n_samples=self.n_samples, n_frames=self.n_frames, sampling_frequency=self.fs,
                   initial_time=self.t0, modulation_frequency=self.fd, sound_speed=self.c,
                   receive_apodization=self.rx_apod, transmit_apodization=self.tx_apod,
-                  dimension=self.dimension, interpolation=self.interpolation)
+                  dimension=self.dimension, interpolation=self.interpolation, tx_model=int(self.tx_model))
         kw.update(over)
         return kw
 
@@ -306,4 +316,37 @@ def c5(seed: int = 5, scale: float = 1.0, with_data: bool = True, datasets_per_c
     return configs, datasets
 
 
-BUILDERS = {"C1": c1, "C2": c2, "C3": c3}
+def rtb(seed: int = 6, scale: float = 1.0, with_data: bool = True, n_speckle: int = 5000,
+        tx_model: int = SphericalTransmitDelayModel.hybrid) -> Acquisition:
+    rng = np.random.default_rng(seed)
+    N = max(8, int(round(128 * scale)))
+    W = max(3, int(round(31 * scale)))
+    n = max(8, int(round(512 * scale)))
+    L = max(128, int(round(3500 * scale)))
+    fs, f0, c = 50e6, 5.2e6, 1540.0
+    prb = LinearArray(N, 0.3e-3)
+    seq = [Wave.spherical(x, 0.0, 20e-3, sound_speed=c, origin=(x, 0.0, 0.0))
+           for x in np.linspace(-15e-3, 15e-3, W)]
+    scan = LinearScan(np.linspace(-19.2e-3, 19.2e-3, n), np.linspace(0.0, 45e-3, n))
+    ap = (3e-3, 3e-3)
+    name = SphericalTransmitDelayModel(int(tx_model)).name
+    acq = Acquisition(f"RTB focused 31 waves RF, 512x512, linear, both, {name} model", prb, scan, seq, fs, 0.0, L,
+                      1, Apodization(Window.hamming, f_number=2.5, minimum_aperture=ap),
+                      Apodization(Window.hamming, f_number=1.5, minimum_aperture=ap), Dimension.both,
+                      Interpolation.linear, c=c, tx_model=int(tx_model), meta={"f0": f0, "rf": True})
+    if with_data:
+        zmax = min(45e-3, (L / fs) * c / 2 * 0.8)
+        S = max(100, int(round(n_speckle * scale)))
+        spk = np.stack([rng.uniform(-19e-3, 19e-3, S), np.zeros(S), rng.uniform(2e-3, zmax, S)], 1)
+        tgt = np.array([(0.0, 0.0, z) for z in (5e-3, 10e-3, 20e-3, 30e-3, 40e-3)] +
+                       [(10e-3, 0.0, 20e-3), (-10e-3, 0.0, 20e-3)])
+        tgt = tgt[tgt[:, 2] <= zmax]
+        amps = np.concatenate([0.1 * rng.standard_normal(S), np.ones(len(tgt))])
+        rf = synth.echoes(prb, seq, np.concatenate([spk, tgt]), amps, fs=fs, n_samples=L, f0=f0, bandwidth=0.6,
+                          iq=False)
+        acq.data = synth.analytic(rf)          # the RF branch of das.m:211-214
+        acq.targets = tgt
+    return acq
+
+
+BUILDERS = {"C1": c1, "C2": c2, "C3": c3, "RTB": rtb}

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
     if (it < 0) break;  // every queue is empty
     const __attribute__((address_space(4))) int* w = (const __attribute__((address_space(4))) int*)B.items + 4 * it;
     const int acq = w[0], tile = w[1], frame = w[2];
-    rxtab_tile<SM, WC, WS>(load_constant(B.acqs + acq), tile, frame, red);
+    rxtab_tile<SM, WC, WS, false>(load_constant(B.acqs + acq), tile, frame, red);
     __syncthreads();  // all threads are past this item before the next claim
   }
 }
@@ -219,8 +219,13 @@ int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batc
     if (pl->rxtab_state != 1 || pl->G.dim != 3)
       return set_status(st, USTBX_E_UNSUPPORTED,
                         "plan %d has no dimension-both receive-table state (select variant 2 first; it needs "
-                        "dimension both, the separable or RF phase and finite weights)",
+                        "dimension both and the separable or RF phase)",
                         i);
+    if (pl->n_bad > 0)
+      return set_status(st, USTBX_E_UNSUPPORTED,
+                        "plan %d has pixels with a non-finite nonzero weight or delay (%d tiles); batches take plans "
+                        "whose weights and delays are finite",
+                        i, pl->n_bad);
     if (interp >= 0 && pl->G.interp != interp)
       return set_status(st, USTBX_E_ARG, "the plans of a batch share one interpolation");
     interp = pl->G.interp;
@@ -414,7 +419,7 @@ struct ustbx_batcher {
 extern "C" {
 
 int32_t ustbx_plan_positions(const ustbx_plan* plan) {
-  if (!plan || plan->rxtab_state != 1 || plan->G.dim != 3) return -1;
+  if (!plan || plan->rxtab_state != 1 || plan->G.dim != 3 || plan->n_bad > 0) return -1;
   const int tiles = (int)plan->h_order.size();
   return (tiles + kQueues - 1) / kQueues;
 }
@@ -485,6 +490,9 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
     if (!pl || pl->rxtab_state != 1 || pl->G.dim != 3)
       return set_status(st, USTBX_E_UNSUPPORTED, "acquisition %d: plan without dimension-both receive-table state",
                         i);
+    if (pl->n_bad > 0)
+      return set_status(st, USTBX_E_UNSUPPORTED,
+                        "acquisition %d: plan with a non-finite nonzero weight or delay (batches need finite ones)", i);
     if (pl->device != b->device) return set_status(st, USTBX_E_ARG, "acquisition %d: plan of another device", i);
     if ((pl->G.interp == USTBX_INTERP_CUBIC) != (b->sm == SM_CUBIC))
       return set_status(st, USTBX_E_ARG, "acquisition %d: interpolation differs from the batcher's", i);

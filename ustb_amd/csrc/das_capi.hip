@@ -385,7 +385,9 @@ bool coef_applies(const ustbx_plan* pl) {
 }
 
 // Builds the variant-2 state once; returns false (and frees it) when the
-// variant does not apply or a nonzero weight or its delay is not finite.
+// variant does not apply, or for dimension transmit when a nonzero weight or
+// its delay is not finite. Dimension both keeps such tiles last in both
+// visiting orders, for the tile body's SAFE form.
 bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   err = hipSuccess;
   if (pl->rxtab_state != 0) return pl->rxtab_state > 0;
@@ -402,7 +404,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   const size_t b_bad = al(sizeof(int) * tiles);
   const size_t b_ref = G.ref_phase ? al(sizeof(float2) * P) : 0;
   const size_t b_ord = al(sizeof(int) * tiles);
-  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + b_ord;
+  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + 2 * b_ord;
   char* m = nullptr;
   err = hipMalloc(&m, total);
   if (err != hipSuccess) return false;
@@ -420,24 +422,30 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   int* tbad = (int*)take(b_bad);
   float2* ref = b_ref ? (float2*)take(b_ref) : nullptr;
   int* ord = (int*)take(b_ord);
+  int* nat = (int*)take(b_ord);
   err = launch_rxtab_tables(G, rx, ref, tnc, tcl, twl, tnw, tbad, nullptr);
   if (err == hipSuccess) err = hipDeviceSynchronize();
   std::vector<int> nw(tiles), nc(tiles), bad(tiles);
   if (err == hipSuccess) err = hipMemcpy(nw.data(), tnw, sizeof(int) * tiles, hipMemcpyDeviceToHost);
   if (err == hipSuccess) err = hipMemcpy(nc.data(), tnc, sizeof(int) * tiles, hipMemcpyDeviceToHost);
   if (err == hipSuccess) err = hipMemcpy(bad.data(), tbad, sizeof(int) * tiles, hipMemcpyDeviceToHost);
-  bool finite = true;
-  int max_nw = 0;
+  int n_bad = 0, max_nw = 0;
   for (size_t t = 0; t < tiles; ++t) {
-    finite = finite && bad[t] == 0;
+    n_bad += bad[t] != 0;
     max_nw = std::max(max_nw, nw[t]);
   }
-  if (err != hipSuccess || !finite) {
+  if (err != hipSuccess || (n_bad > 0 && G.dim != 3)) {
     (void)hipFree(m);
     return false;
   }
   std::vector<int> order = band_major_order(G.scan, (int)tiles, G.dim == 2 ? 1 : 8);
+  std::vector<int> natural(tiles);
+  for (size_t t = 0; t < tiles; ++t) natural[t] = (int)t;
+  auto finite_first = [&](int t) { return bad[t] == 0; };
+  std::stable_partition(order.begin(), order.end(), finite_first);
+  std::stable_partition(natural.begin(), natural.end(), finite_first);
   err = hipMemcpy(ord, order.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
+  if (err == hipSuccess) err = hipMemcpy(nat, natural.data(), sizeof(int) * tiles, hipMemcpyHostToDevice);
   if (err != hipSuccess) {
     (void)hipFree(m);
     return false;
@@ -445,6 +453,8 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   pl->rxtab = m;
   pl->rxtab_bytes = total;
   pl->tile_order = ord;
+  pl->tile_natural = nat;
+  pl->n_bad = n_bad;
   pl->max_nw = max_nw;
   pl->rxtab_state = 1;
   pl->h_order = std::move(order);
@@ -457,6 +467,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   G.tile_nw = tnw;
   G.ref_table = ref;
   G.tile_order = ord;
+  G.tile_good = (int)tiles - n_bad;
   return true;
 }
 
@@ -862,9 +873,9 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) 
     if (!rxtab_build(plan, e)) {
       if (e != hipSuccess) return hip_fail(st, e, "receive table");
       return set_status(st, USTBX_E_UNSUPPORTED,
-                        "the receive-table kernels need dimension transmit, or dimension both with the separable (or RF) "
-                        "phase outside das_c's order, a receive table within half the device memory and finite nonzero "
-                        "weights and delays");
+                        "the receive-table kernels need dimension both with the separable (or RF) phase outside das_c's "
+                        "order, or dimension transmit with finite nonzero weights and delays, and a receive table within "
+                        "half the device memory");
     }
     if (variant == 3 && !coef_build(plan, e)) {
       if (e != hipSuccess) return hip_fail(st, e, "coefficient buffer");
@@ -1023,7 +1034,9 @@ int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st) {
   if (order != 0 && order != 1) return set_status(st, USTBX_E_ARG, "order must be 0 or 1");
   if (plan->rxtab_state <= 0)
     return set_status(st, USTBX_E_UNSUPPORTED, "the tile order applies to the receive-table variants");
-  plan->G.tile_order = order ? plan->tile_order : nullptr;
+  // tiles that take the SAFE body are last in either order, which then
+  // stays explicit
+  plan->G.tile_order = order ? plan->tile_order : (plan->n_bad > 0 ? plan->tile_natural : nullptr);
   return ok(st);
 }
 

@@ -250,6 +250,10 @@ void das_geom_kernel(GeomLaunch G) {
         float2 val[MODE == MODE_NONE ? WC : 1];
 #pragma unroll
         for (int j = 0; j < (MODE == MODE_NONE ? WC : 1); ++j) val[j] = make_float2(0.0f, 0.0f);
+        // separable phase: whether some term of this channel is in the window
+        // (das_c skips the others before multiplying, so R, which may be NaN,
+        // multiplies the partial sum only then)
+        bool hit = false;
 
         if (ra != 0.0f) {
           const int noff = n * l8;
@@ -287,6 +291,7 @@ void das_geom_kernel(GeomLaunch G) {
                   sample_index<INTERP>(denay, G.L, G.edge, n0, b, valid);
                 }
                 if (valid) {
+                  hit = true;
                   // byte offset of sample n0 of trace (n, w) within a frame
                   const int off = noff + w * ln8 + (n0 << 3);
 #pragma unroll
@@ -321,8 +326,10 @@ void das_geom_kernel(GeomLaunch G) {
 
         // ---- per-channel fold / write ----
         if (MODE == MODE_BOTH && !EXACT) {
+          if (hit) {
 #pragma unroll
-          for (int f = 0; f < FC; ++f) cfma(acc[f], R, part[f]);
+            for (int f = 0; f < FC; ++f) cfma(acc[f], R, part[f]);
+          }
         } else if (MODE == MODE_TX && active) {
 #pragma unroll
           for (int f = 0; f < FC; ++f) {
@@ -330,7 +337,7 @@ void das_geom_kernel(GeomLaunch G) {
               float2* o = out + ((size_t)(f0 + f) * G.N + n) * PS + p;
               float2 v = part[f];
               if (!EXACT) {
-                v = cmul(R, v);
+                if (hit) v = cmul(R, v);
                 if (w0 > 0) {
                   const float2 prev = *o;
                   cadd(v, prev);
@@ -460,6 +467,7 @@ void das_both_shared_rx_kernel(GeomLaunch G) {
           const float rd = rv.x;
           const int noff = (g0 + c) * l8;
           float2 part = make_float2(0.0f, 0.0f);
+          bool hit = false;  // some term of the channel in the window (R may be NaN)
 #pragma unroll
           for (int j = 0; j < WC; ++j) {
             const int w = r0 + s + WS * j;  // waves dealt round-robin over the splits
@@ -470,6 +478,7 @@ void das_both_shared_rx_kernel(GeomLaunch G) {
                 d = ((td[j] + rd) - G.t0) * G.fs;  // das_c.cpp:372,375
               }
               if (d >= lo_ok && d < lim) {
+                hit = true;
                 const int n0 = (int)d;
                 const float b = __builtin_amdgcn_fractf(d);
                 const int off = noff + w * ln8 + (n0 << 3);
@@ -489,7 +498,7 @@ void das_both_shared_rx_kernel(GeomLaunch G) {
               }
             }
           }
-          cfma(acc, make_float2(rv.y, rv.z), part);
+          if (hit) cfma(acc, make_float2(rv.y, rv.z), part);
         }
       }
       __syncthreads();

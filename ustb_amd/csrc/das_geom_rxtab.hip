@@ -108,19 +108,35 @@ __global__ __launch_bounds__(kBlock) void spline_coef_kernel(const float2* __res
 }
 
 // dimension.both from a plan's resident tables: one block per (tile, frame).
-template <int SM, int WC, int WS>
+template <int SM, int WC, int WS, bool SAFE>
 __global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(RxAcq A, const int* __restrict__ order) {
   __shared__ float2 red[WS][64];
   int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (order) tile = __builtin_amdgcn_readfirstlane(order[tile]);  // block-uniform
-  rxtab_tile<SM, WC, WS>(A, tile, (int)blockIdx.y, red);
+  rxtab_tile<SM, WC, WS, SAFE>(A, tile, (int)blockIdx.y, red);
 }
 
+// The tiles of the visiting order's first tile_good entries take the
+// branch-free tile body; the rest (a nonzero weight or its delay not finite)
+// take the SAFE body, in a second launch of one fixed tile shape.
 template <int SM, int WC, int WS>
 static hipError_t launch_rxtab_t(const GeomLaunch& G, hipStream_t stream) {
-  const dim3 grid((unsigned)((G.P + 63) / 64), (unsigned)G.F);
-  hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS>), grid, dim3(64 * WS), 0, stream, rx_acq(G, SM), G.tile_order);
-  return hipGetLastError();
+  const int tiles = (G.P + 63) / 64;
+  const int good = G.tile_good;
+  if (good < tiles && !G.tile_order) return hipErrorInvalidValue;
+  const RxAcq A = rx_acq(G, SM);
+  if (good > 0) {
+    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS, false>), dim3((unsigned)good, (unsigned)G.F),
+                       dim3(64 * WS), 0, stream, A, G.tile_order);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (good < tiles) {
+    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, 8, 4, true>), dim3((unsigned)(tiles - good), (unsigned)G.F),
+                       dim3(64 * 4), 0, stream, A, G.tile_order + good);
+    return hipGetLastError();
+  }
+  return hipSuccess;
 }
 
 template <int SM, int WS>

@@ -55,8 +55,8 @@ struct GeomLaunch {
   int rx_cull;
   // Kernel variant: 0 = per-lane wave chunks (all modes), 1 = receive state
   // shared through LDS across wave-splits (dimension both, separable phase,
-  // one frame), 2 = resident receive table (dimension both, separable phase,
-  // finite weights and delays).
+  // one frame), 2 = resident receive table (dimension both, separable phase;
+  // dimension transmit with finite weights and delays).
   int variant;
   // Variant 2 state. Resident: receive table [N][P] of {rx delay, rx
   // apodization, rx apodization * exp(i wd rx delay)}; per 64-pixel tile the
@@ -72,8 +72,11 @@ struct GeomLaunch {
   const float2* ref_table;
   float4* coef;
   // Variant 2/3 tile visiting order: block b takes tile tile_order[x(b)],
-  // x = the XCD-contiguous remap of b; null = x(b) itself.
+  // x = the XCD-contiguous remap of b; null = x(b) itself. Its first
+  // tile_good entries are the tiles whose nonzero weights and their delays
+  // are all finite; the order is non-null when tile_good < tiles.
   const int* tile_order;
+  int tile_good;
 };
 
 // Builds the transmit table of a plan (one lane per (wave, pixel)).

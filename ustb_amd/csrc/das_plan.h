@@ -24,6 +24,8 @@ struct ustbx_plan {
   size_t rxtab_bytes;
   size_t coef_bytes;     // variant 3's coefficient buffer (G.coef)
   const int* tile_order; // band-major order (G.tile_order when selected)
+  const int* tile_natural;  // tiles in index order, the same partition
+  int n_bad;             // tiles with a non-finite nonzero weight or delay (last in both orders)
   std::vector<int> h_order;     // host copies for batches: tile order,
   std::vector<int64_t> h_cost;  // and terms per tile (waves x channels)
 };

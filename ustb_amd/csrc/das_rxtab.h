@@ -16,8 +16,11 @@
 // through an out-of-range buffer offset, which the hardware returns as zeros.
 // That is exact only when every nonzero weight and its delay are finite
 // (das_c skips a zero-weight or out-of-window term before it multiplies, so
-// a NaN weight on a skipped term must not reach the sum); plans check this
-// when they build their tables and otherwise keep the exact kernels.
+// a NaN weight on a skipped term must not reach the sum). Tiles where a
+// nonzero weight or its delay is not finite (e.g. the NaN apodization of
+// pixels at depth 0) take the SAFE form instead: a term enters the sum only
+// when it is in the window, and a channel's receive weight multiplies its
+// partial sum only when some term of the channel did.
 #pragma once
 #include "das_geom_impl.h"
 
@@ -79,10 +82,10 @@ __device__ __forceinline__ void keys_weights_horner(float b, float& c0, float& c
 // and read zeros.
 constexpr int RB = 5;
 
-template <int SM, int WC, bool GUARD>
+template <int SM, int WC, bool GUARD, bool SAFE>
 __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
                                                 const float2 (&T)[WC], const int (&wb)[WC], int nj, float rd,
-                                                int nb, float t0, float fs, float lim) {
+                                                int nb, float t0, float fs, float lim, bool& any) {
   using M = SampleMode<SM>;
   constexpr float lo_ok = 1.0f;  // das_c's window: 0 < trunc(d)
   float2 part = make_float2(0.0f, 0.0f);
@@ -91,6 +94,7 @@ __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const
     if (GUARD && j0 >= nj) break;
     float4 q[RB][M::kLoads];
     float bb[RB];
+    bool okb[RB];
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
       const int j = j0 + i;
@@ -101,6 +105,7 @@ __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const
           d = ((td[j] + rd) - t0) * fs;  // das_c.cpp:372,375
         }
         const bool ok = d >= lo_ok && d < lim;
+        okb[i] = ok;
         const int n0 = (int)d;
         bb[i] = __builtin_amdgcn_fractf(d);
         // the first gather's offset folded into the wave-uniform base, so both
@@ -133,7 +138,14 @@ __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const
           sm = make_float2(fmaf(fmaf(fmaf(q1.z, b, q1.x), b, q0.z), b, q0.x),
                            fmaf(fmaf(fmaf(q1.w, b, q1.y), b, q0.w), b, q0.y));
         }
-        cfma(part, T[j], sm);
+        if (SAFE) {
+          float2 t = part;
+          cfma(t, T[j], sm);
+          part = okb[i] ? t : part;
+          any = any || okb[i];
+        } else {
+          cfma(part, T[j], sm);
+        }
       }
     }
     // one batch in flight per wave: the batch's sum is complete here (an empty
@@ -145,7 +157,7 @@ __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const
   return part;
 }
 
-template <int SM, int WC, bool GUARD>
+template <int SM, int WC, bool GUARD, bool SAFE>
 __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
                                                const float2 (&T)[WC], const int (&wb)[WC], int nj,
                                                const float4* __restrict__ rxt, const int* __restrict__ chl,
@@ -169,14 +181,21 @@ __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const 
     const bool ron = rv.y != 0.0f;
     const float rd = ron ? rv.x : kFar;
     const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
-    const float2 part = rxtab_channel<SM, WC, GUARD>(rs, td, T, wb, nj, rd, nb, t0, fs, lim);
-    cfma(acc, R, part);
+    bool any = false;
+    const float2 part = rxtab_channel<SM, WC, GUARD, SAFE>(rs, td, T, wb, nj, rd, nb, t0, fs, lim, any);
+    if (SAFE) {
+      float2 t = acc;
+      cfma(t, R, part);
+      acc = any ? t : acc;
+    } else {
+      cfma(acc, R, part);
+    }
   }
 }
 
 // One tile of one frame; every thread of the block calls it (it ends with a
 // block barrier and the split-ordered reduction through red).
-template <int SM, int WC, int WS>
+template <int SM, int WC, int WS, bool SAFE = false>
 __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, float2 (*red)[64]) {
   constexpr int TP = 64;
   constexpr int ST = SampleMode<SM>::kStride;
@@ -234,11 +253,11 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
       }
     }
     if (nj == WC)
-      rxtab_channels<SM, WC, false>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst, A.t0, A.fs, lim,
-                                    acc);
+      rxtab_channels<SM, WC, false, SAFE>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst, A.t0, A.fs,
+                                          lim, acc);
     else
-      rxtab_channels<SM, WC, true>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst, A.t0, A.fs, lim,
-                                   acc);
+      rxtab_channels<SM, WC, true, SAFE>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst, A.t0, A.fs,
+                                         lim, acc);
   }
   red[s][lane] = acc;
   __syncthreads();

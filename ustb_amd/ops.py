@@ -200,7 +200,8 @@ class GeometryPlan:
     def set_variant(self, variant: int):
         """0: per-lane wave chunks; 1: receive state shared in LDS (dimension
         both, IQ, separable phase, one frame); 2: resident receive table
-        (dimension both, separable or RF phase, finite weights)."""
+        (dimension both with the separable or RF phase; dimension transmit
+        with finite nonzero weights and delays)."""
         st = L.Status()
         L.check(L.lib().ustbx_plan_set_variant(self.handle, int(variant), ctypes.byref(st)), st)
 
@@ -217,6 +218,12 @@ class GeometryPlan:
         if rc != L.E_UNSUPPORTED:
             L.check(rc, st)
         return False
+
+    @property
+    def batchable(self) -> bool:
+        """Whether batches and the batcher take this plan: dimension-both
+        receive-table state whose nonzero weights and delays are finite."""
+        return int(L.lib().ustbx_plan_positions(self.handle)) >= 0
 
     @property
     def variant(self) -> tuple[int, int, int]:
