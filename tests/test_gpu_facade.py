@@ -30,8 +30,11 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
-@pytest.mark.parametrize("L", [777, 1024])
+@pytest.mark.parametrize("L", [777, 1000, 512, 1024, 2048, 4096])
 def test_hilbert_matches_scipy(cuda, L):
+    """hipFFT path (777, 1000) and the one-block LDS FFT path (powers of two
+    512 .. 4096)."""
+    print(f"case hilbert L={L}", flush=True)
     rng = np.random.default_rng(L)
     x = rng.standard_normal((2, 3, 5, L)).astype(np.float32)
     got = hilbert(torch.from_numpy(x).to(cuda)).cpu().numpy()
@@ -40,6 +43,12 @@ def test_hilbert_matches_scipy(cuda, L):
     # scipy's hilbert (ifft(fft(x) .* h)): equal to x up to FP32 rounding
     assert _rel(got.real, x) <= TOL
     assert _rel(got, ref) <= TOL
+    # a pure tone at bin k: the analytic signal is exp(i 2 pi k n / L) exactly
+    n = np.arange(L)
+    k = 37
+    tone = np.cos(2 * np.pi * k * n / L).astype(np.float32)[None]
+    z = hilbert(torch.from_numpy(np.repeat(tone, 3, 0)).to(cuda)).cpu().numpy()
+    assert np.max(np.abs(z - np.exp(2j * np.pi * k * n / L)[None])) <= 2e-5
 
 
 def _das(case, data, dim=Dimension.both):

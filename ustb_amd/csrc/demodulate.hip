@@ -7,20 +7,30 @@
 // does with single data); low-pass FIR; keep every down-th sample from the
 // first. The two filters differ:
 //  - causal (MATLAB filter(b, 1, x) with zero initial state): taps at offsets
-//    -(K-1) .. 0, samples before the trace are zero, summed as filter's
-//    transposed direct form nests them (the last tap innermost);
+//    -(K-1) .. 0, samples before the trace are zero;
 //  - zero-phase (scipy filtfilt(b, 1, x) with its odd extension of 3K
 //    samples): on the cropped output the forward and backward passes never
 //    reach the extension's ends, so the result is one FIR with the taps'
 //    autocorrelation c[d] = sum_k b[k] b[k+d], d = -(K-1) .. K-1, over the
 //    odd-extended signal (2 e[0] - e[-m] before the start, 2 e[L-1] -
 //    e[2L-2-m] after the end).
-// One block per (trace, run of OB outputs) (grid x: traces, y: runs): the mixed samples of its input
-// window are staged in LDS once, then each lane forms one output.
+//
+// The mixing factors depend on the sample index only: they are computed once
+// per (device, L, fs, t0, fmod) into a small device table that later calls
+// reuse. One block per (trace, run of up to kR * kDB outputs) stages the
+// mixed samples of its input window in LDS, split by decimation phase
+// (phase r holds window samples r, r + down, ...; one pad slot every 4
+// entries, so the lanes' reads, kR entries apart, hit distinct banks). Each
+// lane forms kR consecutive outputs: per phase, a window of kR samples slides
+// along the phase's taps in registers, so every staged sample a lane needs is
+// read once. The sum runs phase by phase (within a phase in tap order).
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "das_status.h"
 
@@ -28,27 +38,50 @@ namespace ustbx {
 
 namespace {
 
-constexpr int kDB = 256;
+constexpr int kDB = 256;         // threads per block
+constexpr int kR = 4;            // consecutive outputs per lane
 constexpr int kMaxTaps = 767;    // passed by value (kernel arguments stay under 4 KB); zero-phase: K <= 384
-constexpr int kWindow = 7680;    // staged samples per block (60 KB; with the taps under 64 KB of LDS)
+constexpr size_t kLdsMax = 64 * 1024;
 
 struct DemodArgs {
   const float* rf;
+  const float2* mix;   // [L] mixing factors
   float2* out;
   int L, Lout, down, ntaps, off0, OB, mode;
-  double fs, t0, fmod;
+  int PS;              // LDS entries per phase (padded)
+  int TP;              // staged taps, zero beyond ntaps
   float taps[kMaxTaps];
 };
 
-// mixed sample e[m] of the trace, m in [0, L)
-__device__ __forceinline__ float2 mixed(const float* __restrict__ x, int m, const DemodArgs& A) {
+// entries of one phase for m window samples (one pad slot per 4) and the
+// dynamic LDS of a block of OB outputs
+inline int phase_slots(int mr) { return mr + (mr >> 2) + 1; }
+inline int phase_len(int OB, int down, int T) {
+  const int count = (OB - 1) * down + T;
+  return (count + down - 1) / down + 2 * kR;
+}
+inline size_t demod_lds(int OB, int down, int T) {
+  return sizeof(float2) * (size_t)down * phase_slots(phase_len(OB, down, T)) +
+         sizeof(float) * (size_t)(T + (kR + 1) * down);
+}
+
+__global__ __launch_bounds__(kDB) void mix_table_kernel(float2* __restrict__ tab, int L, double fs, double t0,
+                                                        double fmod) {
 #pragma clang fp contract(off)
-  const double t = A.t0 + (double)m / A.fs;
-  const double ph = ((-2.0 * M_PI) * A.fmod) * t;
+  const int m = blockIdx.x * kDB + threadIdx.x;
+  if (m >= L) return;
+  const double t = t0 + (double)m / fs;
+  const double ph = ((-2.0 * M_PI) * fmod) * t;
   double s, c;
   sincos(ph, &s, &c);
+  tab[m] = make_float2((float)c, (float)s);
+}
+
+// mixed sample e[m] of the trace, m in [0, L)
+__device__ __forceinline__ float2 mixed(const float* __restrict__ x, int m, const DemodArgs& A) {
   const float v = x[m];
-  return make_float2(v * (float)c, v * (float)s);
+  const float2 e = A.mix[m];
+  return make_float2(v * e.x, v * e.y);
 }
 
 // e at extended index m (zero / odd extension outside [0, L))
@@ -61,30 +94,90 @@ __device__ __forceinline__ float2 extended(const float* __restrict__ x, int m, c
   return make_float2(2.0f * edge.x - refl.x, 2.0f * edge.y - refl.y);
 }
 
+__device__ __forceinline__ int slot(int m) { return m + (m >> 2); }
+
 __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
-#pragma clang fp contract(off)
-  extern __shared__ float2 win[];
-  __shared__ float tp[kMaxTaps];
+  extern __shared__ float2 lds[];
+  const int down = A.down, T = A.ntaps, PS = A.PS;
+  float* tp = reinterpret_cast<float*>(lds + (size_t)down * PS);
   const size_t trace = blockIdx.x;
   const float* __restrict__ x = A.rf + trace * (size_t)A.L;
   const int j0 = (int)blockIdx.y * A.OB;
   const int jn = min(A.OB, A.Lout - j0);
-  const int lo = j0 * A.down + A.off0;                         // extended index of win[0]
-  const int count = (jn - 1) * A.down + A.ntaps;
-  for (int i = threadIdx.x; i < A.ntaps; i += kDB) tp[i] = A.taps[i];
-  for (int i = threadIdx.x; i < count; i += kDB) win[i] = extended(x, lo + i, A);
-  __syncthreads();
-  const int j = threadIdx.x;
-  if (j < jn) {
-    const float2* w = win + j * A.down;
-    float2 acc = make_float2(tp[0] * w[0].x, tp[0] * w[0].y);
-    for (int i = 1; i < A.ntaps; ++i) {
-      const float t = tp[i];
-      acc.x = t * w[i].x + acc.x;
-      acc.y = t * w[i].y + acc.y;
+  const int lo = j0 * down + A.off0;            // extended index of window sample 0
+  const int count = (jn - 1) * down + T;        // window samples the outputs read
+  const int mr = (count + down - 1) / down + 2 * kR;
+  for (int i = threadIdx.x; i < A.TP; i += kDB) tp[i] = i < T ? A.taps[i] : 0.0f;
+  for (int r = 0; r < down; ++r) {
+    float2* ph = lds + (size_t)r * PS;
+    for (int m = threadIdx.x; m < mr; m += kDB) {
+      const int i = m * down + r;
+      ph[slot(m)] = i < count ? extended(x, lo + i, A) : make_float2(0.0f, 0.0f);
     }
-    A.out[trace * (size_t)A.Lout + j0 + j] = acc;
   }
+  __syncthreads();
+  const int jt = kR * (int)threadIdx.x;         // first output of this lane (block-local)
+  if (jt >= jn) return;
+  float2 acc[kR];
+#pragma unroll
+  for (int k = 0; k < kR; ++k) acc[k] = make_float2(0.0f, 0.0f);
+  for (int r = 0; r < down; ++r) {
+    const float2* __restrict__ ph = lds + (size_t)r * PS;
+    const int Q = (T - r + down - 1) / down;    // taps r, r + down, ... of this phase
+    // w[s] holds phase sample jt + q0 + s at the start of each block of kR taps
+    float2 w[kR];
+#pragma unroll
+    for (int k = 0; k < kR; ++k) w[k] = ph[slot(jt + k)];
+    for (int q0 = 0; q0 < Q; q0 += kR) {
+#pragma unroll
+      for (int u = 0; u < kR; ++u) {
+        if (q0 + u >= Q) break;                 // uniform
+        const float t = tp[(q0 + u) * down + r];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+          const float2 s = w[(u + k) % kR];
+          acc[k].x = fmaf(t, s.x, acc[k].x);
+          acc[k].y = fmaf(t, s.y, acc[k].y);
+        }
+        w[u] = ph[slot(jt + q0 + u + kR)];
+      }
+    }
+  }
+  float2* __restrict__ o = A.out + trace * (size_t)A.Lout + j0 + jt;
+#pragma unroll
+  for (int k = 0; k < kR; ++k)
+    if (jt + k < jn) o[k] = acc[k];
+}
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, double, double, double>, float2*> g_mix;
+
+// The mixing-factor table of (device, L, fs, t0, fmod), built on first use on
+// the caller's stream (and waited for, so that other streams may read it).
+int mix_table(int L, double fs, double t0, double fmod, hipStream_t s, const float2** out, ustbx_status* st) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(st, e, "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_mu);
+  const auto key = std::make_tuple(dev, L, fs, t0, fmod);
+  auto it = g_mix.find(key);
+  if (it != g_mix.end()) {
+    *out = it->second;
+    return ok(st);
+  }
+  float2* tab = nullptr;
+  e = hipMalloc(&tab, sizeof(float2) * (size_t)L);
+  if (e != hipSuccess) return hip_fail(st, e, "hipMalloc (mixing factors)");
+  hipLaunchKernelGGL(mix_table_kernel, dim3((unsigned)((L + kDB - 1) / kDB)), dim3(kDB), 0, s, tab, L, fs, t0, fmod);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(tab);
+    return hip_fail(st, e, "mixing factors");
+  }
+  g_mix.emplace(key, tab);
+  *out = tab;
+  return ok(st);
 }
 
 }  // namespace
@@ -112,7 +205,11 @@ int ustbx_demodulate(const float* rf, void* iq, int32_t L, int64_t traces, doubl
   if (L == 0 || traces == 0) return ok(st);
   if (traces > 0x7FFFFFFF) return set_status(st, USTBX_E_ARG, "too many traces");
   if (!rf || !iq) return set_status(st, USTBX_E_ARG, "NULL buffer");
-  if ((int64_t)T + down > kWindow) return set_status(st, USTBX_E_ARG, "decimation factor too large for the filter");
+  // outputs per block: kR per lane, fewer where the window would not fit in LDS
+  int OB = kR * kDB;
+  while (OB > kR && demod_lds(OB, down, T) > kLdsMax) OB /= 2;
+  if (demod_lds(OB, down, T) > kLdsMax)
+    return set_status(st, USTBX_E_ARG, "decimation factor too large for the filter");
   // taps in FP32 (MATLAB filter runs in the data's single precision); the
   // zero-phase form's autocorrelation is formed in FP64 first
   DemodArgs A;
@@ -126,23 +223,26 @@ int ustbx_demodulate(const float* rf, void* iq, int32_t L, int64_t traces, doubl
       A.taps[d + K - 1] = (float)c;
     }
   }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const float2* mix = nullptr;
+  const int rc = mix_table(L, fs, t0, fmod, s, &mix, st);
+  if (rc != USTBX_OK) return rc;
   A.rf = rf;
+  A.mix = mix;
   A.out = reinterpret_cast<float2*>(iq);
   A.L = L;
   A.Lout = (L + down - 1) / down;
   A.down = down;
   A.ntaps = T;
   A.off0 = -(K - 1);
-  A.OB = (int)std::min<int64_t>(kDB, (kWindow - T) / down + 1);
+  A.OB = OB;
   A.mode = mode;
-  A.fs = fs;
-  A.t0 = t0;
-  A.fmod = fmod;
+  A.PS = phase_slots(phase_len(OB, down, T));
+  A.TP = T + (kR + 1) * down;
   const int64_t gy = (A.Lout + A.OB - 1) / A.OB;
   if (gy > 65535) return set_status(st, USTBX_E_ARG, "trace too long for the decimation factor");
-  const size_t lds = sizeof(float2) * (size_t)((A.OB - 1) * down + T);
-  hipLaunchKernelGGL(demodulate_kernel, dim3((unsigned)traces, (unsigned)gy), dim3(kDB), lds,
-                     reinterpret_cast<hipStream_t>(stream), A);
+  hipLaunchKernelGGL(demodulate_kernel, dim3((unsigned)traces, (unsigned)gy), dim3(kDB), demod_lds(OB, down, T), s,
+                     A);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(st, e, "demodulate_kernel launch");
   return ok(st);

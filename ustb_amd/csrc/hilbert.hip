@@ -3,8 +3,13 @@
 // python/src/ustb/midprocess/das.py:144-145 (scipy.signal.hilbert):
 // X = FFT(x) along time; Z[0] = X[0], Z[k] = 2 X[k] for 0 < k < L/2,
 // Z[L/2] = X[L/2] (L even), Z[k] = 0 above; z = IFFT(Z).
-// FFTs are hipFFT's (R2C forward, C2C inverse); plans are cached per
-// (device, L, traces).
+// L = 512, 1024, 2048 or 4096: one block per trace computes all of it in LDS
+// (forward and inverse radix-8 Stockham FFTs, a last radix-2/4 stage where
+// log2 L is not a multiple of 3, twiddles from a per-device table rounded
+// from FP64), so HBM sees the trace read once and the analytic signal
+// written once. Other lengths: hipFFT (R2C forward, C2C
+// inverse, with the weights in between); plans are cached per (device, L,
+// traces), and calls of one (L, traces) share the plan's spectrum buffer.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 #include <stdarg.h>
@@ -59,6 +64,193 @@ __global__ void analytic_weights_kernel(const float2* __restrict__ spec, float2*
   z[i] = v;
 }
 
+// ---- LDS path (L a power of two, 512 .. 4096) ----
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// (x + iy) * (s i), s = +-1
+__device__ __forceinline__ float2 mul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
+
+// In-register DFTs of size 2, 4, 8: X[q] = sum_r v[r] exp(s 2 pi i r q / R).
+__device__ __forceinline__ void dft2(float2* v, float) {
+  const float2 a = v[0], b = v[1];
+  v[0] = make_float2(a.x + b.x, a.y + b.y);
+  v[1] = make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ void dft4v(float2& v0, float2& v1, float2& v2, float2& v3, float s) {
+  const float2 t0 = make_float2(v0.x + v2.x, v0.y + v2.y), t1 = make_float2(v0.x - v2.x, v0.y - v2.y);
+  const float2 t2 = make_float2(v1.x + v3.x, v1.y + v3.y);
+  const float2 t3 = mul_si(make_float2(v1.x - v3.x, v1.y - v3.y), s);
+  v0 = make_float2(t0.x + t2.x, t0.y + t2.y);
+  v2 = make_float2(t0.x - t2.x, t0.y - t2.y);
+  v1 = make_float2(t1.x + t3.x, t1.y + t3.y);
+  v3 = make_float2(t1.x - t3.x, t1.y - t3.y);
+}
+__device__ __forceinline__ void dft4(float2* v, float s) { dft4v(v[0], v[1], v[2], v[3], s); }
+__device__ __forceinline__ void dft8(float2* v, float s) {
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4v(e0, e1, e2, e3, s);
+  dft4v(o0, o1, o2, o3, s);
+  const float c = 0.70710678118654752f;
+  // W8^q = exp(s 2 pi i q / 8): 1, c (1 + s i), s i, c (-1 + s i)
+  const float2 w1 = cmulf(o1, make_float2(c, s * c));
+  const float2 w2 = mul_si(o2, s);
+  const float2 w3 = cmulf(o3, make_float2(-c, s * c));
+  v[0] = make_float2(e0.x + o0.x, e0.y + o0.y);
+  v[4] = make_float2(e0.x - o0.x, e0.y - o0.y);
+  v[1] = make_float2(e1.x + w1.x, e1.y + w1.y);
+  v[5] = make_float2(e1.x - w1.x, e1.y - w1.y);
+  v[2] = make_float2(e2.x + w2.x, e2.y + w2.y);
+  v[6] = make_float2(e2.x - w2.x, e2.y - w2.y);
+  v[3] = make_float2(e3.x + w3.x, e3.y + w3.y);
+  v[7] = make_float2(e3.x - w3.x, e3.y - w3.y);
+}
+
+// LDS slot of point e: one pad slot per 8 points, so that the strided writes
+// of the first two stages fall on distinct banks.
+__device__ __forceinline__ int pad8(int e) { return e + (e >> 3); }
+
+// Twiddle table of one length N (stages after the first, which has none):
+// stage Ns = 8 and 64 (radix 8), then Ns = 512 (radix N / 512); entry
+// (r - 1) Ns + k of a stage is exp(-2 pi i k r / (Ns R)), so the lanes of a
+// wave (consecutive k) read consecutive entries.
+constexpr int kTw8 = 0, kTw64 = 7 * 8, kTw512 = kTw64 + 7 * 64;  // 504 entries before the last stage
+__host__ __device__ constexpr int twiddle_count(int N) { return kTw512 + (N / 512 - 1) * 512; }
+
+// One Stockham stage of radix R over buf (N points, N / 8 threads, 8 points
+// per thread): butterfly j reads j + r N/R, twiddles by exp(s 2 pi i k r /
+// (Ns R)), k = j mod Ns, and writes (j - k) R + k + r Ns. All reads happen
+// before the barrier, all writes after it.
+template <int N, int R>
+__device__ __forceinline__ void fft_stage(float2* buf, const float2* tw, int Ns, float s) {
+  constexpr int NT = N / 8, ITER = 8 / R, NB = N / R;
+  const int tid = threadIdx.x;
+  float2 v[ITER][R];
+  float2 w[ITER][R];
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int k = (tid + it * NT) & (Ns - 1);
+#pragma unroll
+    for (int r = 1; r < R; ++r) w[it][r] = Ns > 1 ? tw[(r - 1) * Ns + k] : make_float2(1.0f, 0.0f);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[it][r] = buf[pad8(tid + it * NT + r * NB)];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 t = w[it][r];  // exp(-2 pi i k r / (Ns R)); conjugated for the inverse
+        t.y = -s * t.y;
+        v[it][r] = cmulf(v[it][r], t);
+      }
+    }
+    if (R == 8) dft8(v[it], s);
+    else if (R == 4) dft4(v[it], s);
+    else dft2(v[it], s);
+  }
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int j = tid + it * NT;
+    const int k = j & (Ns - 1);
+    const int d = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[pad8(d + r * Ns)] = v[it][r];
+  }
+  __syncthreads();
+}
+
+// s = -1: forward DFT; s = +1: inverse without the 1/N. The twiddles of the
+// stages Ns = 8, 64 come from LDS (tws), those of the last stage from the
+// device table (twg; consecutive lanes read consecutive entries).
+template <int N>
+__device__ __forceinline__ void fft_lds(float2* buf, const float2* tws, const float2* __restrict__ twg, float s) {
+  fft_stage<N, 8>(buf, tws, 1, s);
+  fft_stage<N, 8>(buf, tws + kTw8, 8, s);
+  fft_stage<N, 8>(buf, tws + kTw64, 64, s);
+  if (N == 1024) fft_stage<N, 2>(buf, twg + kTw512, 512, s);
+  if (N == 2048) fft_stage<N, 4>(buf, twg + kTw512, 512, s);
+  if (N == 4096) fft_stage<N, 8>(buf, twg + kTw512, 512, s);
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8) void hilbert_lds_kernel(const float* __restrict__ rf, float2* __restrict__ z,
+                                                            const float2* __restrict__ twg) {
+  constexpr int NT = N / 8;
+  __shared__ float2 buf[N + N / 8];
+  __shared__ float2 tws[kTw512];
+  const size_t t = blockIdx.x;
+  const float* __restrict__ x = rf + t * N;
+  for (int i = threadIdx.x; i < N; i += NT) buf[pad8(i)] = make_float2(x[i], 0.0f);
+  for (int i = threadIdx.x; i < kTw512; i += NT) tws[i] = twg[i];
+  __syncthreads();
+  fft_lds<N>(buf, tws, twg, -1.0f);
+  // MATLAB hilbert's weights with ifft's 1/N (exact: powers of two)
+  for (int i = threadIdx.x; i < N; i += NT) {
+    const float h = (i == 0 || i == N / 2) ? 1.0f / N : (i < N / 2 ? 2.0f / N : 0.0f);
+    const float2 v = buf[pad8(i)];
+    buf[pad8(i)] = make_float2(v.x * h, v.y * h);
+  }
+  __syncthreads();
+  fft_lds<N>(buf, tws, twg, 1.0f);
+  float2* __restrict__ o = z + t * N;
+  for (int i = threadIdx.x; i < N; i += NT) o[i] = buf[pad8(i)];
+}
+
+// Entry i of the twiddle table of length N (FP64, rounded to FP32).
+__global__ void twiddle_kernel(float2* __restrict__ tw, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= twiddle_count(N)) return;
+  int Ns, R, q;
+  if (i < kTw64) {
+    Ns = 8, R = 8, q = i - kTw8;
+  } else if (i < kTw512) {
+    Ns = 64, R = 8, q = i - kTw64;
+  } else {
+    Ns = 512, R = N / 512, q = i - kTw512;
+  }
+  const int r = q / Ns + 1, k = q % Ns;
+  double sv, cv;
+  sincospi(-2.0 * (double)(k * r) / (double)(Ns * R), &sv, &cv);
+  tw[i] = make_float2((float)cv, (float)sv);
+}
+
+std::map<std::pair<int, int>, float2*> g_twiddles;
+
+// The twiddle table of length N on the current device: built once (and
+// waited for on the caller's stream), then shared by every call. Caller
+// holds g_mu.
+hipError_t twiddles(int dev, int N, hipStream_t s, const float2** out) {
+  const auto key = std::make_pair(dev, N);
+  auto it = g_twiddles.find(key);
+  if (it != g_twiddles.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  float2* tw = nullptr;
+  const int n = twiddle_count(N);
+  hipError_t e = hipMalloc(&tw, sizeof(float2) * (size_t)n);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, tw, N);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(tw);
+    return e;
+  }
+  g_twiddles.emplace(key, tw);
+  *out = tw;
+  return hipSuccess;
+}
+
+template <int N>
+hipError_t launch_hilbert_lds(const float* rf, float2* z, long long traces, const float2* tw, hipStream_t s) {
+  hipLaunchKernelGGL(hilbert_lds_kernel<N>, dim3((unsigned)traces), dim3(N / 8), 0, s, rf, z, tw);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 extern "C" int ustbx_hilbert(const float* rf, void* analytic, int32_t L, int64_t traces, void* stream,
@@ -72,6 +264,25 @@ extern "C" int ustbx_hilbert(const float* rf, void* analytic, int32_t L, int64_t
   if (e != hipSuccess) return fail(st, USTBX_E_GPU, "HIP error: %s", hipGetErrorString(e));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   std::lock_guard<std::mutex> lock(g_mu);
+  if (L == 512 || L == 1024 || L == 2048 || L == 4096) {
+    const float2* tw = nullptr;
+    e = twiddles(dev, L, s, &tw);
+    if (e == hipSuccess) {
+      float2* z = reinterpret_cast<float2*>(analytic);
+      switch (L) {
+        case 512: e = launch_hilbert_lds<512>(rf, z, traces, tw, s); break;
+        case 1024: e = launch_hilbert_lds<1024>(rf, z, traces, tw, s); break;
+        case 2048: e = launch_hilbert_lds<2048>(rf, z, traces, tw, s); break;
+        default: e = launch_hilbert_lds<4096>(rf, z, traces, tw, s); break;
+      }
+    }
+    if (e != hipSuccess) return fail(st, USTBX_E_GPU, "HIP error: %s", hipGetErrorString(e));
+    if (st) {
+      st->code = USTBX_OK;
+      st->message[0] = 0;
+    }
+    return USTBX_OK;
+  }
   auto key = std::make_tuple(dev, (int)L, (long long)traces);
   auto it = g_plans.find(key);
   if (it == g_plans.end()) {
