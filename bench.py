@@ -193,13 +193,8 @@ def main():
     alg_bytes = in_bytes + out_bytes
     achieved_gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
     copy = copy_probe(torch, dev)
-    traffic = None
     kname = "das_tx_rxtab" if int(acq.dimension) == 2 else "das_both_rxtab"
-    tpath = os.path.join(ROOT, "profiles", f"{kname}_{args.config.lower()}_traffic.txt")
-    if os.path.exists(tpath):
-        for line in open(tpath):
-            if line.startswith("traffic_bytes_per_launch"):
-                traffic = float(line.split("=")[1])
+    traffic = _traffic(f"{kname}_{args.config.lower()}_traffic.txt")
 
     result = {
         "metric": "DAS Gsamples/s (pixel x channel x wave)",
@@ -240,6 +235,18 @@ def main():
         result["cpu_baseline"] = cpu_baseline(acq, plan, torch)
     plan.close()
     print(json.dumps(result))
+
+
+def _traffic(name):
+    """HBM bytes per launch of the dominant kernel from its committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, calibrated on the
+    copy probe), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    return None
 
 
 def build_acquisition(args):
@@ -343,7 +350,8 @@ def run_frontend(args, torch, dev):
         "config": {"workload": "RF2IQ fast_demodulation of STA RF, 128 waves x 128 channels x 2048 samples, "
                                "20.8 MHz, f0 5.2 MHz", "rf_samples": S},
         "roofline": {"bound": "hbm", "achieved": demod["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": demod["frac"], "traffic": None, "kernel_ms": demod["kernel_ms"],
+                     "frac": demod["frac"], "traffic": _traffic("rf_frontend_rf2iq_traffic.txt"),
+                     "kernel_ms": demod["kernel_ms"],
                      "achievable_copy_GBs": copy["GBs"], "frac_of_achievable": demod["frac_of_achievable"]},
         "fast_demodulation": demod,
         "python_fast_demodulation": zp,
@@ -488,12 +496,7 @@ def run_matrix(args, torch, dev):
     copy = copy_probe(torch, dev)
     log(f"matrix ABI {args.steps} steps: {ms_per_step:.4f} ms/step (kernel {kernel_ms:.4f} ms), "
         f"{achieved:.0f} GB/s of {copy['GBs']:.0f} achievable")
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"das_matrix_{args.config.lower()}_traffic.txt")
-    if os.path.exists(tpath):
-        for line in open(tpath):
-            if line.startswith("traffic_bytes_per_launch"):
-                traffic = float(line.split("=")[1])
+    traffic = _traffic(f"das_matrix_{args.config.lower()}_traffic.txt")
     result = {
         "metric": "DAS Gsamples/s (pixel x channel x wave)",
         "value": round(value, 3),
