@@ -519,6 +519,7 @@ def run_matrix(args, torch, dev):
                      "algorithmic_bytes": alg_bytes, "achievable_copy_GBs": copy["GBs"],
                      "frac_of_achievable": round(achieved / copy["GBs"], 4)},
     }
+    result["host_entry"] = host_entry(acq, [t.cpu().numpy() for t in (ta, ra, td, rd)], dim, interp)
     if not args.no_check:
         pix = _sample_pixels(acq.P, 2048, seed=5)
         sel = lambda t: np.ascontiguousarray(t.cpu().numpy()[:, pix])
@@ -530,6 +531,40 @@ def run_matrix(args, torch, dev):
                             "bitwise_vs_dasc": bool(np.array_equal(got, ref, equal_nan=True))}
         log(f"errors {result['errors']}")
     print(json.dumps(result))
+
+
+def host_entry(acq, mats, dim, interp, runs=5):
+    """The same call through ustbx_das_run_host, the MEX gateway's entry: host
+    (pageable) channel data and matrices, as MATLAB hands them over, copied in,
+    beamformed and copied back per call, with a device allocation per call as
+    das_cuda makes one. Median of `runs` calls after one warm-up: the
+    PCIe-inclusive rate of the drop-in boundary."""
+    import ctypes
+
+    from ustb_amd import _lib as L
+
+    ta, ra, td, rd = (np.ascontiguousarray(m) for m in mats)
+    F, W, N, Ls = acq.data.shape
+    from ustb_amd import ops
+
+    out = np.zeros(ops.out_shape(dim, acq.P, N, W, F), np.complex64)
+    d = L.DasDesc(L=Ls, N=N, W=W, F=F, P=acq.P, fs=acq.fs, t0=acq.t0, fd=acq.fd, dim=dim, interp=interp, edge=0,
+                  flags=0, split=0)
+    ch = np.ascontiguousarray(acq.data)
+    times = []
+    for k in range(runs + 1):
+        st = L.Status()
+        t = time.perf_counter()
+        rc = L.lib().ustbx_das_run_host(ctypes.byref(d), ch.ctypes.data, ta.ctypes.data, ra.ctypes.data,
+                                        td.ctypes.data, rd.ctypes.data, out.ctypes.data, 0, ctypes.byref(st))
+        L.check(rc, st)
+        if k:
+            times.append(time.perf_counter() - t)
+    ms = float(np.median(times)) * 1e3
+    host_bytes = ch.nbytes + sum(m.nbytes for m in (ta, ra, td, rd)) + out.nbytes
+    log(f"host entry: {ms:.3f} ms per call ({host_bytes / 1e6:.1f} MB over the host link)")
+    return {"ms": round(ms, 3), "Gsamples_s": round(acq.units / (ms * 1e-3) / 1e9, 3), "host_bytes": host_bytes,
+            "note": "pageable host buffers copied in and out per call, per-call device allocation"}
 
 
 def run_batch(args, torch, dev):
