@@ -137,3 +137,41 @@ def test_synth_deterministic_across_thread_counts():
         b = synth.echoes(prb, seq, pts, amps, threads=7, **kw)
         assert np.array_equal(a, b)
         assert a.shape == (1, 2, 8, 512) and np.abs(a).max() > 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import json
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_rtb_workload_follows_the_reference_example():
+    """examples/fresnel/linear_array/RTB_linear_array.m: 128 elements at
+    0.3 mm, 31 focused waves with sources at x = -15..15 mm, z = 20 mm and
+    origins on the array below them, fs 50 MHz, 512 x 512 scan over x
+    +-19.2 mm and z 0-45 mm, Hamming receive f# 2.5 and transmit f# 1.5 with
+    3 mm minimum apertures, dimension both."""
+    from ustb_amd import configs
+    from ustb_amd.uff import LinearScan
+
+    acq = configs.rtb(with_data=False)
+    assert acq.probe.N_elements == 128 and np.isclose(acq.probe.x[1] - acq.probe.x[0], 0.3e-3)
+    assert len(acq.sequence) == 31 and acq.fs == 50e6 and int(acq.dimension) == int(Dimension.both)
+    src = np.array([w.source.xyz for w in acq.sequence])
+    org = np.array([w.origin.xyz for w in acq.sequence])
+    np.testing.assert_allclose(src[:, 0], np.linspace(-15e-3, 15e-3, 31), atol=1e-12)
+    np.testing.assert_allclose(src[:, 2], 20e-3, atol=1e-12)
+    np.testing.assert_allclose(org, np.stack([src[:, 0], np.zeros(31), np.zeros(31)], 1), atol=1e-12)
+    assert all(w.wavefront == Wavefront.spherical for w in acq.sequence)
+    assert isinstance(acq.scan, LinearScan) and acq.scan.shape[:2] == (512, 512)
+    assert np.isclose(acq.scan.z_axis[0], 0.0) and np.isclose(acq.scan.z_axis[-1], 45e-3)
+    assert np.isclose(acq.scan.x_axis[0], -19.2e-3) and np.isclose(acq.scan.x_axis[-1], 19.2e-3)
+    for ap, f in ((acq.rx_apod, 2.5), (acq.tx_apod, 1.5)):
+        assert ap.window == Window.hamming and ap.f_number == (f, f) and ap.minimum_aperture == (3e-3, 3e-3)
+    assert acq.tx_model == SphericalTransmitDelayModel.hybrid
+    assert configs.rtb(with_data=False, tx_model=SphericalTransmitDelayModel.unified).tx_model == 2
