@@ -3,11 +3,11 @@
 // python/src/ustb/midprocess/das.py:144-145 (scipy.signal.hilbert):
 // X = FFT(x) along time; Z[0] = X[0], Z[k] = 2 X[k] for 0 < k < L/2,
 // Z[L/2] = X[L/2] (L even), Z[k] = 0 above; z = IFFT(Z).
-// L = 512, 1024, 2048 or 4096: one block per trace computes all of it in LDS
-// (forward and inverse radix-8 Stockham FFTs, a last radix-2/4 stage where
-// log2 L is not a multiple of 3, twiddles from a per-device table rounded
-// from FP64), so HBM sees the trace read once and the analytic signal
-// written once. Other lengths: hipFFT (R2C forward, C2C
+// L = 512, 1024, 2048 or 4096: one block per two traces computes all of it in
+// LDS (one complex signal a + i b; forward and inverse radix-8 Stockham FFTs,
+// a last radix-2/4 stage where log2 L is not a multiple of 3, twiddles from a
+// per-device table rounded from FP64), so HBM sees the traces read once and
+// the analytic signals written once. Other lengths: hipFFT (R2C forward, C2C
 // inverse, with the weights in between); plans are cached per (device, L,
 // traces), and calls of one (L, traces) share the plan's spectrum buffer.
 #include <hip/hip_runtime.h>
@@ -175,15 +175,30 @@ __device__ __forceinline__ void fft_lds(float2* buf, const float2* tws, const fl
   if (N == 4096) fft_stage<N, 8>(buf, twg + kTw512, 512, s);
 }
 
+// Two traces a, b per block. The analytic-signal operator (the weights h on
+// the spectrum) is linear over complex input, so w = ifft(h .* fft(a + i b))
+// = z_a + i z_b, where z_a = a + i H(a) and z_b = b + i H(b) with H(.) real:
+// Re w = a - H(b), Im w = H(a) + b. The outputs' real parts are the inputs
+// themselves (MATLAB's come back through the FFT round trip, equal up to
+// FP32 rounding); their imaginary parts are H(a) = Im w - b, H(b) = a - Re w.
 template <int N>
 __global__ __launch_bounds__(N / 8) void hilbert_lds_kernel(const float* __restrict__ rf, float2* __restrict__ z,
-                                                            const float2* __restrict__ twg) {
+                                                            const float2* __restrict__ twg, long long traces) {
   constexpr int NT = N / 8;
   __shared__ float2 buf[N + N / 8];
   __shared__ float2 tws[kTw512];
-  const size_t t = blockIdx.x;
-  const float* __restrict__ x = rf + t * N;
-  for (int i = threadIdx.x; i < N; i += NT) buf[pad8(i)] = make_float2(x[i], 0.0f);
+  const size_t ta = 2 * (size_t)blockIdx.x;
+  const bool has_b = (long long)ta + 1 < traces;
+  const float* __restrict__ xa = rf + ta * N;
+  const float* __restrict__ xb = xa + N;
+  float av[8], bv[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = threadIdx.x + k * NT;
+    av[k] = xa[i];
+    bv[k] = has_b ? xb[i] : 0.0f;
+    buf[pad8(i)] = make_float2(av[k], bv[k]);
+  }
   for (int i = threadIdx.x; i < kTw512; i += NT) tws[i] = twg[i];
   __syncthreads();
   fft_lds<N>(buf, tws, twg, -1.0f);
@@ -195,8 +210,15 @@ __global__ __launch_bounds__(N / 8) void hilbert_lds_kernel(const float* __restr
   }
   __syncthreads();
   fft_lds<N>(buf, tws, twg, 1.0f);
-  float2* __restrict__ o = z + t * N;
-  for (int i = threadIdx.x; i < N; i += NT) o[i] = buf[pad8(i)];
+  float2* __restrict__ za = z + ta * N;
+  float2* __restrict__ zb = za + N;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int i = threadIdx.x + k * NT;
+    const float2 w = buf[pad8(i)];
+    za[i] = make_float2(av[k], w.y - bv[k]);
+    if (has_b) zb[i] = make_float2(bv[k], av[k] - w.x);
+  }
 }
 
 // Entry i of the twiddle table of length N (FP64, rounded to FP32).
@@ -247,7 +269,8 @@ hipError_t twiddles(int dev, int N, hipStream_t s, const float2** out) {
 
 template <int N>
 hipError_t launch_hilbert_lds(const float* rf, float2* z, long long traces, const float2* tw, hipStream_t s) {
-  hipLaunchKernelGGL(hilbert_lds_kernel<N>, dim3((unsigned)traces), dim3(N / 8), 0, s, rf, z, tw);
+  hipLaunchKernelGGL(hilbert_lds_kernel<N>, dim3((unsigned)((traces + 1) / 2)), dim3(N / 8), 0, s, rf, z, tw,
+                     traces);
   return hipGetLastError();
 }
 
