@@ -40,6 +40,7 @@ namespace {
 
 constexpr int kDB = 256;         // threads per block
 constexpr int kR = 4;            // consecutive outputs per lane
+constexpr int kSB = 8;           // window samples a lane loads per staging round
 constexpr int kMaxTaps = 767;    // passed by value (kernel arguments stay under 4 KB); zero-phase: K <= 384
 constexpr size_t kLdsMax = 64 * 1024;
 
@@ -84,16 +85,6 @@ __device__ __forceinline__ float2 mixed(const float* __restrict__ x, int m, cons
   return make_float2(v * e.x, v * e.y);
 }
 
-// e at extended index m (zero / odd extension outside [0, L))
-__device__ __forceinline__ float2 extended(const float* __restrict__ x, int m, const DemodArgs& A) {
-#pragma clang fp contract(off)
-  if (m >= 0 && m < A.L) return mixed(x, m, A);
-  if (A.mode == USTBX_FIR_CAUSAL) return make_float2(0.0f, 0.0f);
-  const float2 edge = mixed(x, m < 0 ? 0 : A.L - 1, A);
-  const float2 refl = mixed(x, m < 0 ? -m : 2 * A.L - 2 - m, A);
-  return make_float2(2.0f * edge.x - refl.x, 2.0f * edge.y - refl.y);
-}
-
 __device__ __forceinline__ int slot(int m) { return m + (m >> 2); }
 
 __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
@@ -108,11 +99,43 @@ __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
   const int count = (jn - 1) * down + T;        // window samples the outputs read
   const int mr = (count + down - 1) / down + 2 * kR;
   for (int i = threadIdx.x; i < A.TP; i += kDB) tp[i] = i < T ? A.taps[i] : 0.0f;
-  for (int r = 0; r < down; ++r) {
-    float2* ph = lds + (size_t)r * PS;
-    for (int m = threadIdx.x; m < mr; m += kDB) {
-      const int i = m * down + r;
-      ph[slot(m)] = i < count ? extended(x, lo + i, A) : make_float2(0.0f, 0.0f);
+  // Window sample i (extended index lo + i) goes to phase i mod down, entry
+  // i / down. Each lane issues kSB loads per round, all in bounds (an index
+  // outside the trace reads its odd-extension reflection, or a sample whose
+  // value is not used), then forms and stores the samples.
+  const bool zero_phase = A.mode == USTBX_FIR_ZERO_PHASE;
+  const int total = mr * down;
+  float2 e0 = make_float2(0.0f, 0.0f), e1 = e0;
+  if (zero_phase) {
+    e0 = mixed(x, 0, A);
+    e1 = mixed(x, A.L - 1, A);
+  }
+  for (int base = 0; base < total; base += kSB * kDB) {
+    float xv[kSB];
+    float2 mv[kSB];
+#pragma unroll
+    for (int k = 0; k < kSB; ++k) {
+      const int m = lo + base + k * kDB + (int)threadIdx.x;
+      int src = m < 0 ? (zero_phase ? -m : 0) : (m >= A.L ? (zero_phase ? 2 * A.L - 2 - m : 0) : m);
+      src = min(max(src, 0), A.L - 1);  // lanes past the window read a sample they do not use
+      xv[k] = x[src];
+      mv[k] = A.mix[src];
+    }
+#pragma unroll
+    for (int k = 0; k < kSB; ++k) {
+      const int i = base + k * kDB + (int)threadIdx.x;
+      if (i < total) {
+#pragma clang fp contract(off)
+        const int m = lo + i;
+        float2 v = make_float2(xv[k] * mv[k].x, xv[k] * mv[k].y);
+        if (m < 0 || m >= A.L) {
+          const float2 edge = m < 0 ? e0 : e1;
+          v = zero_phase ? make_float2(2.0f * edge.x - v.x, 2.0f * edge.y - v.y) : make_float2(0.0f, 0.0f);
+        }
+        if (i >= count) v = make_float2(0.0f, 0.0f);
+        const int q = i / down;
+        lds[(size_t)(i - q * down) * PS + slot(q)] = v;
+      }
     }
   }
   __syncthreads();
