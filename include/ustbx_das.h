@@ -246,12 +246,11 @@ int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
  *       (dimension both, IQ, separable phase, one frame);
  *   2 = resident receive table and per-tile wave/channel lists (dimension
  *       both with the separable or RF phase outside das_c's order, or
- *       dimension transmit with any phase and finite nonzero weights and
- *       delays; any frame count, receive table within half the device
- *       memory; the default where it applies). Dimension both computes the
+ *       dimension transmit with any phase; any frame count, receive table
+ *       within half the device memory; the default where it applies). The
  *       64-pixel tiles that hold a non-finite nonzero weight or delay (such
- *       as the NaN apodization of pixels at depth 0) with a per-term window
- *       test, in a second launch. For dimension transmit set_tiles'
+ *       as the NaN apodization of pixels at depth 0) are computed with a
+ *       per-term window test, in a second launch. For dimension transmit set_tiles'
  *       arguments are frames per block (4 or 8) and channels per block (4 or
  *       8; one wave64 each), and without the separable phase (or with RF
  *       data) it keeps das_c's term arithmetic and summation order;

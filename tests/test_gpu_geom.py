@@ -176,12 +176,6 @@ def test_tile_shapes_agree(cuda, variant, wc, split, name):
     plan.close()
 
 
-def _nonfinite_weight(case):
-    """Whether a nonzero weight of the oracle's matrices is not finite."""
-    ta, ra, td, rd = oracle_matrices(case)
-    return bool(np.any((ta != 0) & ~np.isfinite(ta)) or np.any((ra != 0) & ~np.isfinite(ra)))
-
-
 @pytest.mark.parametrize("name", sorted(CASES))
 @pytest.mark.parametrize("interp", [Interpolation.linear, Interpolation.cubic])
 def test_receive_table_variant(cuda, name, interp):
@@ -224,12 +218,9 @@ def test_transmit_table_variant(cuda, name, interp):
     9 frames, so the second chunk of 8 holds one frame; against variant 0 and
     the FP64 oracle, over its frame/channel-split shapes."""
     case = CASES[name]
+    print(f"case txtab {name} {interp.name}", flush=True)
     plan = _plan(case, dim=2, F=9, separable_phase=True, interpolation=int(interp))
-    applies = plan.has_variant(2)
-    assert applies == (not _nonfinite_weight(case))
-    if not applies:
-        plan.close()
-        return
+    assert plan.has_variant(2)  # tiles with non-finite weights take the SAFE body
     assert plan.variant[0] == 2
     ch = channel_noise(case, F=9, seed=23)
     chd = torch.from_numpy(ch).to(cuda)
@@ -263,11 +254,7 @@ def test_transmit_table_dasc_phase(cuda, name, dasc_order):
     F = 9
     print(f"case tx dasc phase {name} dasc_order={dasc_order}", flush=True)
     plan = _plan(case, dim=2, F=F, separable_phase=False, dasc_order=dasc_order)
-    applies = plan.has_variant(2)
-    assert applies == (not _nonfinite_weight(case))
-    if not applies:
-        plan.close()
-        return
+    assert plan.has_variant(2)
     assert plan.variant[0] == 2
     ch = channel_noise(case, F=F, seed=29)
     chd = torch.from_numpy(ch).to(cuda)
@@ -275,14 +262,14 @@ def test_transmit_table_dasc_phase(cuda, name, dasc_order):
     for fc, cs in ((8, 4), (4, 8)):
         plan.set_tiles(fc, cs)
         outs.append(plan.run(chd).cpu().numpy())
-    assert np.array_equal(outs[1], outs[0])
+    assert np.array_equal(outs[1], outs[0], equal_nan=True)
     ta, ra, td, rd = [t.cpu().numpy() for t in plan.materialize()]
     plan.set_variant(0)
     v0 = plan.run(chd).cpu().numpy()
     ref = O.das_c(ch, case["fs"], 0.0, ta, ra, td, rd, case["fd"], 2)
-    assert np.array_equal(outs[0], v0)
+    assert np.array_equal(outs[0], v0, equal_nan=True)
     if case["fd"] == 0.0:
-        assert np.array_equal(outs[0], ref)
+        assert np.array_equal(outs[0], ref, equal_nan=True)
     else:
         assert _rel(outs[0], ref) <= TOL
     plan.close()

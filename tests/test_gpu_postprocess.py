@@ -30,7 +30,12 @@ CASES = geom_cases()
 
 
 def _rel(a, b):
-    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+    """max |a - b| / max |b| over the entries that are not NaN; the NaN
+    positions must agree."""
+    na, nb = np.isnan(a), np.isnan(b)
+    assert np.array_equal(na, nb), "NaN positions differ"
+    a, b = a[~na], b[~nb]
+    return float(np.max(np.abs(a - b), initial=0.0) / max(np.max(np.abs(b), initial=0.0), 1e-30))
 
 
 def _data(seed, F=3, W=5, N=7, P=301):
@@ -98,12 +103,15 @@ def test_coherence_factor_matches_oracle(cuda, dim, flavour):
     assert _rel(img[fi], rimg[fi]) <= 1e-5
 
 
-@pytest.mark.parametrize("name", ["dw_sector", "cpwc_tukey", "pw_rf_hanning"])
+@pytest.mark.parametrize("name", ["dw_sector", "cpwc_tukey", "pw_rf_hanning", "cpwc_depth0", "focused_hybrid"])
 @pytest.mark.parametrize("separable", [True, False])
 def test_fused_coherence_factor_equals_unfused(cuda, name, separable):
     """ustbx_plan_run_cf against ustbx_plan_run + ustbx_coherence_factor
-    (receive) within FP32 rounding of the grouped sums (1e-6), and the
-    oracle's das_c + coherence factor within 1e-5."""
+    (receive) within FP32 rounding of the grouped sums (2e-6 of the peak
+    image, 1e-6 on the factor), and the
+    oracle's das_c + coherence factor within 1e-5. cpwc_depth0 and
+    focused_hybrid hold NaN weights (depth 0, focal depth): their tiles take
+    the SAFE body and the NaN positions must agree."""
     case = CASES[name]
     F = 9
     print(f"case fused cf {name} separable={separable}", flush=True)
@@ -121,11 +129,11 @@ def test_fused_coherence_factor_equals_unfused(cuda, name, separable):
         plan.set_tiles(fc, cs)
         img_f, cf_f = plan.run_cf(chd, M, L.CF_MATLAB)
         a, b = img_f.cpu().numpy(), img_u[:, 0, 0].cpu().numpy()
-        assert _rel(a, b) <= 1e-6
-        assert np.max(np.abs(cf_f.cpu().numpy() - cf_u[:, 0, 0].cpu().numpy())) <= 1e-6
+        assert _rel(a, b) <= 2e-6
+        assert np.max(np.abs(cf_f.cpu().numpy() - cf_u[:, 0, 0].cpu().numpy())) <= 1e-6  # NaN -> 0
     img_p, _ = plan.run_cf(chd, None, L.CF_PYTHON)
     img_pu, _ = ops.coherence_factor(per_channel, L.DIM_RX, None, L.CF_PYTHON)
-    assert _rel(img_p.cpu().numpy(), img_pu[:, 0, 0].cpu().numpy()) <= 1e-6
+    assert _rel(img_p.cpu().numpy(), img_pu[:, 0, 0].cpu().numpy()) <= 2e-6
     # against the oracle: das_c per-channel images, then the coherence factor
     ta, ra, td, rd = oracle_matrices(case)
     ref_bf = O.das_c(ch, case["fs"], 0.0, ta, ra, td, rd, case["fd"], 2)

@@ -385,9 +385,8 @@ bool coef_applies(const ustbx_plan* pl) {
 }
 
 // Builds the variant-2 state once; returns false (and frees it) when the
-// variant does not apply, or for dimension transmit when a nonzero weight or
-// its delay is not finite. Dimension both keeps such tiles last in both
-// visiting orders, for the tile body's SAFE form.
+// variant does not apply. Tiles where a nonzero weight or its delay is not
+// finite go last in both visiting orders, for the kernels' SAFE form.
 bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   err = hipSuccess;
   if (pl->rxtab_state != 0) return pl->rxtab_state > 0;
@@ -434,7 +433,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
     n_bad += bad[t] != 0;
     max_nw = std::max(max_nw, nw[t]);
   }
-  if (err != hipSuccess || (n_bad > 0 && G.dim != 3)) {
+  if (err != hipSuccess) {
     (void)hipFree(m);
     return false;
   }
@@ -874,8 +873,7 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) 
       if (e != hipSuccess) return hip_fail(st, e, "receive table");
       return set_status(st, USTBX_E_UNSUPPORTED,
                         "the receive-table kernels need dimension both with the separable (or RF) phase outside das_c's "
-                        "order, or dimension transmit with finite nonzero weights and delays, and a receive table within "
-                        "half the device memory");
+                        "order, or dimension transmit, and a receive table within half the device memory");
     }
     if (variant == 3 && !coef_build(plan, e)) {
       if (e != hipSuccess) return hip_fail(st, e, "coefficient buffer");

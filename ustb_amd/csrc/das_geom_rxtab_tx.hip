@@ -34,8 +34,11 @@ namespace ustbx {
 // One channel's images of FC frames f0 .. f0+nf-1 on one 64-pixel tile (a
 // wave64, lane = pixel): v[f] is what das_c writes to out[f][n][p] (times the
 // reference phase where the plan has it), zeros for a channel with no receive
-// weight on the tile. Wave-uniform: tile, n, f0, nf.
-template <int SM, int FC, int PH>
+// weight on the tile. Wave-uniform: tile, n, f0, nf. SAFE (tiles where a
+// nonzero weight or its delay is not finite): a term enters the sum only when
+// it is in the window, and the separable receive factor multiplies the sum
+// only when some term did, as das_c skips the others before multiplying.
+template <int SM, int FC, int PH, bool SAFE>
 __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, int n, int f0, int nf, int lane,
                                            float2 (&v)[FC]) {
   using M = SampleMode<SM>;
@@ -74,6 +77,7 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
   // its index one further ahead
   int w = nw > 0 ? __builtin_amdgcn_readfirstlane(wl[0]) : 0;
   int w_next = nw > 1 ? __builtin_amdgcn_readfirstlane(wl[1]) : 0;
+  bool any = false;  // SAFE: some term of the channel was in the window
   float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
   if (active && nw > 0) vn = A.tx[(size_t)w * PS + p];
   for (int i = 0; i < nw; ++i) {
@@ -109,6 +113,7 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
       }
     }
     const bool ok = d >= lo_ok && d < lim;
+    if (SAFE) any = any || ok;
     const int n0 = (int)d;
     const float b = __builtin_amdgcn_fractf(d);
     // the first gather's offset folded into the wave-uniform part (n0 >= 1
@@ -127,48 +132,51 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
     if (SM == SM_CUBIC) keys_weights_horner(b, c0, c1, c2, c3);
 #pragma unroll
     for (int f = 0; f < FC; ++f) {
-      float2 sm;
+      float2 np = part[f];
       if (PH != PH_SEP && SM == SM_LINEAR) {
         Term t;
         t.a = 1.0f - b;
         t.b = b;
         t.wr = wr;
         t.wi = wi;
-        cadd(part[f], dasc_contrib(t, q[f][0]));  // das_c.cpp:389-399
-        continue;
-      }
-      if (SM == SM_LINEAR) {
-        const float4 x = q[f][0];
-        sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
+        cadd(np, dasc_contrib(t, q[f][0]));  // das_c.cpp:389-399
       } else {
-        const float4 lo = q[f][0], hi = q[f][M::kLoads - 1];
-        sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
-                         fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
-      }
-      if (PH == PH_SEP) {
-        cfma(part[f], T, sm);
-      } else {
+        float2 sm;
+        if (SM == SM_LINEAR) {
+          const float4 x = q[f][0];
+          sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
+        } else {
+          const float4 lo = q[f][0], hi = q[f][M::kLoads - 1];
+          sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
+                           fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
+        }
+        if (PH == PH_SEP) {
+          cfma(np, T, sm);
+        } else {
 #pragma clang fp contract(off)
-        cadd(part[f], make_float2(wr * sm.x - wi * sm.y, wr * sm.y + wi * sm.x));
+          cadd(np, make_float2(wr * sm.x - wi * sm.y, wr * sm.y + wi * sm.x));
+        }
       }
+      part[f] = (!SAFE || ok) ? np : part[f];
     }
   }
   const float2 refp = (active && A.ref) ? A.ref[p] : make_float2(1.0f, 0.0f);
 #pragma unroll
   for (int f = 0; f < FC; ++f) {
-    float2 x = PH == PH_SEP ? cmul(R, part[f]) : part[f];
+    float2 x = (PH == PH_SEP && (!SAFE || any)) ? cmul(R, part[f]) : part[f];
     if (A.ref) x = cmul(x, refp);
     v[f] = x;
   }
 }
 
-template <int SM, int FC, int CS, int PH>
+// Blocks cover slabs x tiles entries of the visiting order (order[0 ..
+// tiles), or the tiles themselves where order is null).
+template <int SM, int FC, int CS, int PH, bool SAFE>
 __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, float wd,
-                                                               const int* __restrict__ order) {
+                                                               const int* __restrict__ order, int tiles) {
   constexpr int TP = 64;
   const int lane = threadIdx.x & (TP - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
-  const int tiles = (A.P + TP - 1) / TP;
   const int chunks = (F + FC - 1) / FC;
   const int item = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   const int slab = item / tiles;
@@ -180,7 +188,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, f
   const int f0 = (slab - group * chunks) * FC;
   const int nf = min(FC, F - f0);
   float2 v[FC];
-  tx_channel<SM, FC, PH>(A, wd, tile, n, f0, nf, lane, v);
+  tx_channel<SM, FC, PH, SAFE>(A, wd, tile, n, f0, nf, lane, v);
   const int p = tile * TP + lane;
   if (p < A.P) {
     const size_t PS = (size_t)A.P;
@@ -197,15 +205,14 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, f
 // its CS channels in channel order: S_g = sum v, E_g = sum |v|^2
 // (coherence_factor.m:126-127), through LDS. part is [G][F][P] float4
 // {S.x, S.y, E, 0}, G = ceil(N / CS).
-template <int SM, int FC, int CS, int PH>
+template <int SM, int FC, int CS, int PH, bool SAFE>
 __global__ __launch_bounds__(64 * CS) void das_tx_cf_part_kernel(RxAcq A, int F, float wd,
-                                                                 const int* __restrict__ order,
+                                                                 const int* __restrict__ order, int tiles,
                                                                  float4* __restrict__ part) {
   constexpr int TP = 64;
   __shared__ float2 red[CS][TP];
   const int lane = threadIdx.x & (TP - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
-  const int tiles = (A.P + TP - 1) / TP;
   const int chunks = (F + FC - 1) / FC;
   const int item = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   const int slab = item / tiles;
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_cf_part_kernel(RxAcq A, int F,
   const int nf = min(FC, F - f0);
   float2 v[FC];
   if (n < A.N) {
-    tx_channel<SM, FC, PH>(A, wd, tile, n, f0, nf, lane, v);
+    tx_channel<SM, FC, PH, SAFE>(A, wd, tile, n, f0, nf, lane, v);
   } else {
 #pragma unroll
     for (int f = 0; f < FC; ++f) v[f] = make_float2(0.0f, 0.0f);
@@ -263,13 +270,29 @@ __global__ __launch_bounds__(256) void das_tx_cf_finish_kernel(const float4* __r
   }
 }
 
+// The first tile_good entries of the visiting order take the branch-free
+// body; the rest (a nonzero weight or its delay not finite) the SAFE body, in
+// a second launch with 8 frames per block and the same channels per block.
 template <int SM, int FC, int CS, int PH>
 static hipError_t launch_tx_t(const GeomLaunch& G, hipStream_t stream) {
-  const int64_t blocks = (int64_t)((G.P + 63) / 64) * ((G.F + FC - 1) / FC) * ((G.N + CS - 1) / CS);
-  if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, FC, CS, PH>), dim3((unsigned)blocks), dim3(64 * CS), 0, stream,
-                     rx_acq(G, SM), G.F, G.wd, G.tile_order);
-  return hipGetLastError();
+  const int tiles = (G.P + 63) / 64, good = G.tile_good;
+  if (good < tiles && !G.tile_order) return hipErrorInvalidValue;
+  const int64_t slabs = (int64_t)((G.F + FC - 1) / FC) * ((G.N + CS - 1) / CS);
+  if ((int64_t)tiles * slabs > 0x7FFFFFFF) return hipErrorInvalidValue;
+  const RxAcq A = rx_acq(G, SM);
+  if (good > 0) {
+    hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, FC, CS, PH, false>), dim3((unsigned)(good * slabs)), dim3(64 * CS),
+                       0, stream, A, G.F, G.wd, G.tile_order, good);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (good < tiles) {
+    const int64_t bslabs = (int64_t)((G.F + 7) / 8) * ((G.N + CS - 1) / CS);
+    hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, 8, CS, PH, true>), dim3((unsigned)((tiles - good) * bslabs)),
+                       dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order + good, tiles - good);
+    return hipGetLastError();
+  }
+  return hipSuccess;
 }
 
 template <int SM, int PH>
@@ -289,11 +312,24 @@ static hipError_t launch_tx_ph(const GeomLaunch& G, hipStream_t stream) {
 
 template <int SM, int FC, int CS, int PH>
 static hipError_t launch_cf_t(const GeomLaunch& G, float4* part, hipStream_t stream) {
-  const int64_t blocks = (int64_t)((G.P + 63) / 64) * ((G.F + FC - 1) / FC) * ((G.N + CS - 1) / CS);
-  if (blocks > 0x7FFFFFFF) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, FC, CS, PH>), dim3((unsigned)blocks), dim3(64 * CS), 0, stream,
-                     rx_acq(G, SM), G.F, G.wd, G.tile_order, part);
-  return hipGetLastError();
+  const int tiles = (G.P + 63) / 64, good = G.tile_good;
+  if (good < tiles && !G.tile_order) return hipErrorInvalidValue;
+  const int64_t slabs = (int64_t)((G.F + FC - 1) / FC) * ((G.N + CS - 1) / CS);
+  if ((int64_t)tiles * slabs > 0x7FFFFFFF) return hipErrorInvalidValue;
+  const RxAcq A = rx_acq(G, SM);
+  if (good > 0) {
+    hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, FC, CS, PH, false>), dim3((unsigned)(good * slabs)),
+                       dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order, good, part);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (good < tiles) {
+    const int64_t bslabs = (int64_t)((G.F + 7) / 8) * ((G.N + CS - 1) / CS);
+    hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, 8, CS, PH, true>), dim3((unsigned)((tiles - good) * bslabs)),
+                       dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order + good, tiles - good, part);
+    return hipGetLastError();
+  }
+  return hipSuccess;
 }
 
 template <int SM, int PH>

@@ -55,8 +55,8 @@ struct GeomLaunch {
   int rx_cull;
   // Kernel variant: 0 = per-lane wave chunks (all modes), 1 = receive state
   // shared through LDS across wave-splits (dimension both, separable phase,
-  // one frame), 2 = resident receive table (dimension both, separable phase;
-  // dimension transmit with finite weights and delays).
+  // one frame), 2 = resident receive table (dimension both with the
+  // separable or RF phase; dimension transmit).
   int variant;
   // Variant 2 state. Resident: receive table [N][P] of {rx delay, rx
   // apodization, rx apodization * exp(i wd rx delay)}; per 64-pixel tile the

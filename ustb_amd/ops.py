@@ -200,8 +200,8 @@ class GeometryPlan:
     def set_variant(self, variant: int):
         """0: per-lane wave chunks; 1: receive state shared in LDS (dimension
         both, IQ, separable phase, one frame); 2: resident receive table
-        (dimension both with the separable or RF phase; dimension transmit
-        with finite nonzero weights and delays)."""
+        (dimension both with the separable or RF phase; dimension
+        transmit)."""
         st = L.Status()
         L.check(L.lib().ustbx_plan_set_variant(self.handle, int(variant), ctypes.byref(st)), st)
 
