@@ -312,8 +312,7 @@ void ustbx_batch_destroy(ustbx_batch* batch);
  * images equal ustbx_plan_run with the same tile shape bit for bit. */
 typedef struct ustbx_batcher ustbx_batcher;
 
-/* Positions per share of a plan with variant-2 state whose nonzero weights
- * and delays are finite (-1 otherwise). */
+/* Positions per share of a plan with variant-2 state (-1 otherwise). */
 int32_t ustbx_plan_positions(const ustbx_plan* plan);
 /* interp: the plans' interpolation; capacities: acquisitions and work items
  * (tiles x frames) per submission; tile shape as ustbx_batch_set_tiles. */

@@ -29,7 +29,8 @@ def _plan(case, F=1, interp=Interpolation.linear, dim=3):
 
 def _mix(interp, frames=(1, 2, 1)):
     """Plans of the cases that have variant-2 state, with mixed frame counts,
-    one plan listed twice (two acquisitions of one configuration)."""
+    one plan listed twice (two acquisitions of one configuration). The focused
+    cases and cpwc_depth0 hold NaN weights: their tiles take the SAFE body."""
     plans, datas, used = [], [], []
     k = 0
     for name in sorted(CASES):
@@ -65,7 +66,7 @@ def _close(plans):
 @pytest.mark.parametrize("interp", [Interpolation.linear, Interpolation.cubic])
 def test_batch_equals_plan_runs(cuda, interp):
     plans, datas = _mix(interp)
-    assert len(plans) >= 4
+    assert len(plans) >= 8
     chs = [torch.from_numpy(d).to(cuda) for d in datas]
     b = Batch(plans)
     shape = b.shape
@@ -75,7 +76,7 @@ def test_batch_equals_plan_runs(cuda, interp):
     torch.cuda.synchronize()
     ref = _reference(plans, chs, shape["wave_chunk"], shape["split"])
     for o, r in zip(outs, ref):
-        assert np.array_equal(o.cpu().numpy(), r)
+        assert np.array_equal(o.cpu().numpy(), r, equal_nan=True)
     b.close()
     _close(plans)
 
@@ -90,7 +91,7 @@ def test_batch_tile_shapes(cuda, wc, ws):
     b.launch()
     torch.cuda.synchronize()
     for o, r in zip(outs, _reference(plans, chs, wc, ws)):
-        assert np.array_equal(o.cpu().numpy(), r)
+        assert np.array_equal(o.cpu().numpy(), r, equal_nan=True)
     b.close()
     _close(plans)
 
@@ -111,7 +112,7 @@ def test_batch_graph_replays(cuda):
         b.replay()
         torch.cuda.synchronize()
         for o, e in zip(outs, eager):
-            assert np.array_equal(o.cpu().numpy(), e), f"replay {rep}"
+            assert np.array_equal(o.cpu().numpy(), e, equal_nan=True), f"replay {rep}"
     # new buffers under the captured graph
     chs2 = [torch.flip(c, dims=[-1]).contiguous() for c in chs]
     outs2 = b.set_io(chs2)
@@ -119,7 +120,7 @@ def test_batch_graph_replays(cuda):
     torch.cuda.synchronize()
     shape = b.shape
     for o, r in zip(outs2, _reference(plans, chs2, shape["wave_chunk"], shape["split"])):
-        assert np.array_equal(o.cpu().numpy(), r)
+        assert np.array_equal(o.cpu().numpy(), r, equal_nan=True)
     b.close()
     _close(plans)
 
@@ -137,11 +138,6 @@ def test_batch_rejections(cuda):
     with pytest.raises(L.USTBXError, match="buffers first"):
         b.launch()
     b.close()
-    # NaN receive weights on the scan's first row: the plan runs variant 2
-    # alone, batches refuse it
-    nan_plan = _plan(CASES["cpwc_depth0"])
-    assert nan_plan.variant[0] == 2 and not nan_plan.batchable and lin.batchable
-    with pytest.raises(L.USTBXError, match="non-finite"):
-        Batch([lin, nan_plan])
-    for p in (lin, cub, tx, nan_plan):
+    assert lin.batchable and not tx.batchable
+    for p in (lin, cub, tx):
         p.close()

@@ -14,7 +14,7 @@ from ustb_amd.serve import Server, poisson_trace
 
 pytestmark = pytest.mark.gpu
 CASES = geom_cases()
-NAMES = ["cpwc_tukey", "dw_sector", "matrix3d_dw"]
+NAMES = ["cpwc_tukey", "dw_sector", "matrix3d_dw", "cpwc_depth0"]  # the last: NaN weights on its first row
 
 
 def _setup():
@@ -55,7 +55,7 @@ def test_served_images_equal_alone(cuda, budget_frames, slice_ms):
     assert res.iterations >= 2
     ref = _alone(plans, datas, srv.tiles)
     for r in trace:
-        assert np.array_equal(res.outputs[r.rid], ref[(r.config, r.data)]), r
+        assert np.array_equal(res.outputs[r.rid], ref[(r.config, r.data)], equal_nan=True), r
     assert np.all(res.latency_s > 0)
     # the plain loop: one launch per request with the plans' default tiles
     for p in plans:
@@ -63,7 +63,8 @@ def test_served_images_equal_alone(cuda, budget_frames, slice_ms):
     plain = srv.plain_loop(trace[:20], keep_outputs=True)
     for r in trace[:20]:
         a, b = plain.outputs[r.rid], ref[(r.config, r.data)]
-        assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(b))
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.nanmax(np.abs(a - b)) <= 1e-5 * np.nanmax(np.abs(b))
     srv.close()
     for p in plans:
         p.close()

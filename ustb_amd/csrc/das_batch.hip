@@ -8,8 +8,9 @@
 // acquisitions in decreasing order of work. The grid is the number of blocks
 // the device holds at once; each block claims items from its own XCD's queue
 // with an atomic counter and, when that queue is empty, from the others in
-// turn. Every item runs the same tile body as the one-plan kernel, so each
-// image equals the one-plan launch with the same tile shape bit for bit.
+// turn. Every item runs the same tile body as the one-plan kernel (the SAFE
+// form on tiles with a non-finite nonzero weight or delay), so each image
+// equals the one-plan launch with the same tile shape bit for bit.
 //
 // A launch is a counter-reset kernel followed by the persistent kernel (no
 // memset, no allocation, no host synchronization), so it can be captured in a
@@ -36,7 +37,7 @@ constexpr int kQueues = 8;
 
 struct BatchLaunch {
   const RxAcq* acqs;
-  const int4* items;  // {acquisition, tile, frame, 0}
+  const int4* items;  // {acquisition, tile, frame, 1 where the tile takes the SAFE body}
   const int* qoff;    // [kQueues + 1] queue offsets into items
   int* ctr;           // [kQueues] claim counters, zeroed before each launch
 };
@@ -82,7 +83,10 @@ __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
     if (it < 0) break;  // every queue is empty
     const __attribute__((address_space(4))) int* w = (const __attribute__((address_space(4))) int*)B.items + 4 * it;
     const int acq = w[0], tile = w[1], frame = w[2];
-    rxtab_tile<SM, WC, WS, false>(load_constant(B.acqs + acq), tile, frame, red);
+    if (w[3])  // a nonzero weight or its delay not finite on this tile
+      rxtab_tile<SM, WC, WS, true>(load_constant(B.acqs + acq), tile, frame, red);
+    else
+      rxtab_tile<SM, WC, WS, false>(load_constant(B.acqs + acq), tile, frame, red);
     __syncthreads();  // all threads are past this item before the next claim
   }
 }
@@ -221,11 +225,6 @@ int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batc
                         "plan %d has no dimension-both receive-table state (select variant 2 first; it needs "
                         "dimension both and the separable or RF phase)",
                         i);
-    if (pl->n_bad > 0)
-      return set_status(st, USTBX_E_UNSUPPORTED,
-                        "plan %d has pixels with a non-finite nonzero weight or delay (%d tiles); batches take plans "
-                        "whose weights and delays are finite",
-                        i, pl->n_bad);
     if (interp >= 0 && pl->G.interp != interp)
       return set_status(st, USTBX_E_ARG, "the plans of a batch share one interpolation");
     interp = pl->G.interp;
@@ -255,7 +254,10 @@ int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batc
     for (int k = 0; k < kQueues; ++k) {
       const int s0 = k * qq + std::min(k, r), cnt = qq + (k < r ? 1 : 0);
       for (int f = 0; f < pl->G.F; ++f)
-        for (int x = s0; x < s0 + cnt; ++x) q[k].push_back(make_int4(i, pl->h_order[x], f, 0));
+        for (int x = s0; x < s0 + cnt; ++x) {
+          const int t = pl->h_order[x];
+          q[k].push_back(make_int4(i, t, f, pl->h_bad[t] ? 1 : 0));
+        }
     }
   }
   std::vector<int> qoff(kQueues + 1, 0);
@@ -419,7 +421,7 @@ struct ustbx_batcher {
 extern "C" {
 
 int32_t ustbx_plan_positions(const ustbx_plan* plan) {
-  if (!plan || plan->rxtab_state != 1 || plan->G.dim != 3 || plan->n_bad > 0) return -1;
+  if (!plan || plan->rxtab_state != 1 || plan->G.dim != 3) return -1;
   const int tiles = (int)plan->h_order.size();
   return (tiles + kQueues - 1) / kQueues;
 }
@@ -490,9 +492,6 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
     if (!pl || pl->rxtab_state != 1 || pl->G.dim != 3)
       return set_status(st, USTBX_E_UNSUPPORTED, "acquisition %d: plan without dimension-both receive-table state",
                         i);
-    if (pl->n_bad > 0)
-      return set_status(st, USTBX_E_UNSUPPORTED,
-                        "acquisition %d: plan with a non-finite nonzero weight or delay (batches need finite ones)", i);
     if (pl->device != b->device) return set_status(st, USTBX_E_ARG, "acquisition %d: plan of another device", i);
     if ((pl->G.interp == USTBX_INTERP_CUBIC) != (b->sm == SM_CUBIC))
       return set_status(st, USTBX_E_ARG, "acquisition %d: interpolation differs from the batcher's", i);
@@ -515,7 +514,10 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
       const int s0 = k * q + std::min(k, r), cnt = q + (k < r ? 1 : 0);
       const int pb = std::max(0, pos_begin[i]), pe = std::min(cnt, pos_end[i]);
       for (int f = 0; f < pl->G.F; ++f)
-        for (int x = pb; x < pe; ++x) items[at++] = make_int4(i, pl->h_order[s0 + x], f, 0);
+        for (int x = pb; x < pe; ++x) {
+          const int t = pl->h_order[s0 + x];
+          items[at++] = make_int4(i, t, f, pl->h_bad[t] ? 1 : 0);
+        }
     }
   }
   qoff[kQueues] = (int)at;

@@ -438,6 +438,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
     return false;
   }
   std::vector<int> order = band_major_order(G.scan, (int)tiles, G.dim == 2 ? 1 : 8);
+  std::vector<int> batch_order = order;  // batches flag the SAFE tiles per item instead
   std::vector<int> natural(tiles);
   for (size_t t = 0; t < tiles; ++t) natural[t] = (int)t;
   auto finite_first = [&](int t) { return bad[t] == 0; };
@@ -456,9 +457,13 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   pl->n_bad = n_bad;
   pl->max_nw = max_nw;
   pl->rxtab_state = 1;
-  pl->h_order = std::move(order);
+  pl->h_order = std::move(batch_order);
   pl->h_cost.resize(tiles);
-  for (size_t t = 0; t < tiles; ++t) pl->h_cost[t] = (int64_t)nw[t] * nc[t];
+  pl->h_bad.resize(tiles);
+  for (size_t t = 0; t < tiles; ++t) {
+    pl->h_cost[t] = (int64_t)nw[t] * nc[t];
+    pl->h_bad[t] = bad[t] != 0;
+  }
   G.rx_table = rx;
   G.tile_nc = tnc;
   G.tile_chl = tcl;

@@ -118,7 +118,8 @@ __global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(RxAcq A, const 
 
 // The tiles of the visiting order's first tile_good entries take the
 // branch-free tile body; the rest (a nonzero weight or its delay not finite)
-// take the SAFE body, in a second launch of one fixed tile shape.
+// take the SAFE body, in a second launch of the same tile shape (so that a
+// batch, which runs either body per tile, computes the same sums).
 template <int SM, int WC, int WS>
 static hipError_t launch_rxtab_t(const GeomLaunch& G, hipStream_t stream) {
   const int tiles = (G.P + 63) / 64;
@@ -132,8 +133,8 @@ static hipError_t launch_rxtab_t(const GeomLaunch& G, hipStream_t stream) {
     if (e != hipSuccess) return e;
   }
   if (good < tiles) {
-    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, 8, 4, true>), dim3((unsigned)(tiles - good), (unsigned)G.F),
-                       dim3(64 * 4), 0, stream, A, G.tile_order + good);
+    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS, true>), dim3((unsigned)(tiles - good), (unsigned)G.F),
+                       dim3(64 * WS), 0, stream, A, G.tile_order + good);
     return hipGetLastError();
   }
   return hipSuccess;

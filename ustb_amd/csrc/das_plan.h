@@ -26,6 +26,7 @@ struct ustbx_plan {
   const int* tile_order; // band-major order (G.tile_order when selected)
   const int* tile_natural;  // tiles in index order, the same partition
   int n_bad;             // tiles with a non-finite nonzero weight or delay (last in both orders)
-  std::vector<int> h_order;     // host copies for batches: tile order,
-  std::vector<int64_t> h_cost;  // and terms per tile (waves x channels)
+  std::vector<int> h_order;     // host copies for batches: band-major tile order,
+  std::vector<int64_t> h_cost;  // terms per tile (waves x channels),
+  std::vector<char> h_bad;      // and whether a tile takes the SAFE body
 };

@@ -222,7 +222,7 @@ class GeometryPlan:
     @property
     def batchable(self) -> bool:
         """Whether batches and the batcher take this plan: dimension-both
-        receive-table state whose nonzero weights and delays are finite."""
+        receive-table state."""
         return int(L.lib().ustbx_plan_positions(self.handle)) >= 0
 
     @property

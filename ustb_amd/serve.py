@@ -150,7 +150,7 @@ class Server:
         for k, n in enumerate(self.positions):
             if n < 0:
                 raise ValueError(f"configuration {k}: the batcher needs a dimension-both plan with receive-table "
-                                 "state and finite nonzero weights and delays")
+                                 "state")
         # host channel data in pinned memory (the requests' payloads)
         self.host = [[torch.from_numpy(np.ascontiguousarray(d)).pin_memory() for d in ds] for ds in datasets]
         self.in_bytes = [int(ds[0].nbytes) for ds in datasets]
