@@ -396,14 +396,15 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   const size_t P = (size_t)G.P, tiles = (P + 63) / 64;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   const size_t b_rx = al(sizeof(float4) * (size_t)G.N * P);
-  const size_t b_nc = al(sizeof(int) * tiles);
+  const size_t b_nc = al(sizeof(int) * 2 * tiles);  // channel counts, then counts of the non-NaN ones
   const size_t b_cl = al(sizeof(int) * tiles * (size_t)G.N);
   const size_t b_wl = al(sizeof(int) * tiles * (size_t)G.W);
   const size_t b_nw = al(sizeof(int) * tiles);
   const size_t b_bad = al(sizeof(int) * tiles);
   const size_t b_ref = G.ref_phase ? al(sizeof(float2) * P) : 0;
   const size_t b_ord = al(sizeof(int) * tiles);
-  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + 2 * b_ord;
+  const size_t b_mask = al(sizeof(unsigned long long) * tiles);
+  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + 2 * b_ord + b_mask;
   char* m = nullptr;
   err = hipMalloc(&m, total);
   if (err != hipSuccess) return false;
@@ -422,7 +423,8 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   float2* ref = b_ref ? (float2*)take(b_ref) : nullptr;
   int* ord = (int*)take(b_ord);
   int* nat = (int*)take(b_ord);
-  err = launch_rxtab_tables(G, rx, ref, tnc, tcl, twl, tnw, tbad, nullptr);
+  unsigned long long* nmask = (unsigned long long*)take(b_mask);
+  err = launch_rxtab_tables(G, rx, ref, tnc, tcl, twl, tnw, tbad, nmask, nullptr);
   if (err == hipSuccess) err = hipDeviceSynchronize();
   std::vector<int> nw(tiles), nc(tiles), bad(tiles);
   if (err == hipSuccess) err = hipMemcpy(nw.data(), tnw, sizeof(int) * tiles, hipMemcpyDeviceToHost);
@@ -466,6 +468,8 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   }
   G.rx_table = rx;
   G.tile_nc = tnc;
+  G.tile_ncf = tnc + tiles;
+  G.nan_mask = nmask;
   G.tile_chl = tcl;
   G.tile_waves = twl;
   G.tile_nw = tnw;

@@ -43,7 +43,9 @@ __global__ __launch_bounds__(kBlock) void geom_ref_table_kernel(GeomLaunch G, fl
 
 // One wave64 per 64-pixel tile: the lists of waves and channels with a
 // nonzero weight on some pixel of the tile, and whether a nonzero weight or
-// its delay is not finite.
+// its delay is not finite. The channel list holds first the channels with a
+// nonzero weight other than NaN on some pixel (tile_nc[tiles + tile] of
+// them), then those whose nonzero weights on the tile are all NaN.
 __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const float4* __restrict__ rxt,
                                                             int* __restrict__ tile_nc, int* __restrict__ tile_chl,
                                                             int* __restrict__ tile_waves, int* __restrict__ tile_nw,
@@ -68,7 +70,16 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
     const float4 v = active ? rxt[(size_t)n * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool on = v.y != 0.0f;
     if (on && !(isfinite(v.x) && isfinite(v.z) && isfinite(v.w))) bad = true;
-    if (__any(on)) {
+    if (__any(on && v.y == v.y)) {
+      if (threadIdx.x == 0) tile_chl[(size_t)tile * G.N + nc] = n;
+      ++nc;
+    }
+  }
+  const int nfin = nc;
+  for (int n = 0; n < G.N; ++n) {
+    const float4 v = active ? rxt[(size_t)n * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool on = v.y != 0.0f;
+    if (__any(on) && !__any(on && v.y == v.y)) {
       if (threadIdx.x == 0) tile_chl[(size_t)tile * G.N + nc] = n;
       ++nc;
     }
@@ -76,9 +87,48 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
   const bool any_bad = __any(bad);
   if (threadIdx.x == 0) {
     tile_nc[tile] = nc;
+    tile_nc[gridDim.x + tile] = nfin;
     tile_nw[tile] = cnt;
     tile_bad[tile] = any_bad ? 1 : 0;
   }
+}
+
+// Per tile, the lanes whose das_c sum is NaN through the channels whose
+// nonzero weights on the tile are all NaN (the tail of the tile's channel
+// list): some term of such a channel with a nonzero transmit weight is in the
+// sample window, by the tile body's own index arithmetic. Geometry only, so
+// a plan constant. One wave64 per tile, after geom_tile_scan_kernel.
+__global__ __launch_bounds__(64) void geom_nan_mask_kernel(GeomLaunch G, const float4* __restrict__ rxt,
+                                                           const int* __restrict__ tile_nc,
+                                                           const int* __restrict__ tile_chl,
+                                                           const int* __restrict__ tile_waves,
+                                                           const int* __restrict__ tile_nw,
+                                                           unsigned long long* __restrict__ mask) {
+  const int tile = blockIdx.x;
+  const int p = tile * 64 + (int)threadIdx.x;
+  const bool active = p < G.P;
+  const size_t PS = (size_t)G.P;
+  const int nc = tile_nc[tile], nfin = tile_nc[gridDim.x + tile], nw = tile_nw[tile];
+  const float lim = (float)(G.interp == INTERP_LINEAR ? G.L - 1 : G.L - 2);
+  bool any = false;
+  for (int k = nfin; k < nc; ++k) {
+    const int n = tile_chl[(size_t)tile * G.N + k];
+    const float4 rv = active ? rxt[(size_t)n * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (rv.y == 0.0f) continue;
+    for (int i = 0; i < nw && !any; ++i) {
+      const int w = tile_waves[(size_t)tile * G.W + i];
+      const float4 tv = active ? G.tx_table[(size_t)w * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+      if (tv.y == 0.0f) continue;
+      float d;
+      {
+#pragma clang fp contract(off)
+        d = ((tv.x + rv.x) - G.t0) * G.fs;  // das_c.cpp:372,375
+      }
+      any = d >= 1.0f && d < lim;
+    }
+  }
+  const unsigned long long m = __ballot(any);
+  if (threadIdx.x == 0) mask[tile] = m;
 }
 
 // Catmull-Rom coefficients of the interval [n, n+1] of every trace:
@@ -108,36 +158,52 @@ __global__ __launch_bounds__(kBlock) void spline_coef_kernel(const float2* __res
 }
 
 // dimension.both from a plan's resident tables: one block per (tile, frame).
-template <int SM, int WC, int WS, bool SAFE>
+template <int SM, int WC, int WS>
 __global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(RxAcq A, const int* __restrict__ order) {
   __shared__ float2 red[WS][64];
   int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (order) tile = __builtin_amdgcn_readfirstlane(order[tile]);  // block-uniform
-  rxtab_tile<SM, WC, WS, SAFE>(A, tile, (int)blockIdx.y, red);
+  rxtab_tile<SM, WC, WS, false>(A, tile, (int)blockIdx.y, red);
 }
 
-// The tiles of the visiting order's first tile_good entries take the
-// branch-free tile body; the rest (a nonzero weight or its delay not finite)
-// take the SAFE body, in a second launch of the same tile shape (so that a
-// batch, which runs either body per tile, computes the same sums).
+// The same where some tiles hold a non-finite nonzero weight or delay (the
+// visiting order's entries good .. good + bad - 1): blocks 0 .. bad - 1 run
+// those with the SAFE body, dispatched first so that their long channel loops
+// overlap the rest; blocks bad8 .. (bad rounded up to whole XCD rounds) take
+// the other tiles through the XCD remap; blocks bad .. bad8 - 1 have none.
+template <int SM, int WC, int WS>
+__global__ __launch_bounds__(64 * WS) void das_both_rxtab_mixed_kernel(RxAcq A, const int* __restrict__ order,
+                                                                       int good, int bad, int bad8) {
+  __shared__ float2 red[WS][64];
+  const int b = (int)blockIdx.x;
+  if (b < bad8) {
+    if (b >= bad) return;  // whole block, before any barrier
+    const int tile = __builtin_amdgcn_readfirstlane(order[good + b]);
+    rxtab_tile<SM, WC, WS, true>(A, tile, (int)blockIdx.y, red);
+  } else {
+    const int tile = __builtin_amdgcn_readfirstlane(order[xcd_tile(b - bad8, good)]);
+    rxtab_tile<SM, WC, WS, false>(A, tile, (int)blockIdx.y, red);
+  }
+}
+
+// Plans whose weights are all finite launch the branch-free kernel; the
+// others the mixed one (same tile shape for both bodies, so that a batch,
+// which runs either body per tile, computes the same sums).
 template <int SM, int WC, int WS>
 static hipError_t launch_rxtab_t(const GeomLaunch& G, hipStream_t stream) {
   const int tiles = (G.P + 63) / 64;
-  const int good = G.tile_good;
-  if (good < tiles && !G.tile_order) return hipErrorInvalidValue;
+  const int good = G.tile_good, bad = tiles - good;
   const RxAcq A = rx_acq(G, SM);
-  if (good > 0) {
-    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS, false>), dim3((unsigned)good, (unsigned)G.F),
-                       dim3(64 * WS), 0, stream, A, G.tile_order);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  if (good < tiles) {
-    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS, true>), dim3((unsigned)(tiles - good), (unsigned)G.F),
-                       dim3(64 * WS), 0, stream, A, G.tile_order + good);
+  if (bad == 0) {
+    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS>), dim3((unsigned)tiles, (unsigned)G.F), dim3(64 * WS), 0,
+                       stream, A, G.tile_order);
     return hipGetLastError();
   }
-  return hipSuccess;
+  if (!G.tile_order) return hipErrorInvalidValue;
+  const int bad8 = (bad + 7) & ~7;
+  hipLaunchKernelGGL((das_both_rxtab_mixed_kernel<SM, WC, WS>), dim3((unsigned)(bad8 + good), (unsigned)G.F),
+                     dim3(64 * WS), 0, stream, A, G.tile_order, good, bad, bad8);
+  return hipGetLastError();
 }
 
 template <int SM, int WS>
@@ -180,7 +246,8 @@ hipError_t launch_both_rxtab(const GeomLaunch& G, hipStream_t stream) {
 }
 
 hipError_t launch_rxtab_tables(const GeomLaunch& G, float4* rx_table, float2* ref_table, int* tile_nc,
-                               int* tile_chl, int* tile_waves, int* tile_nw, int* tile_bad, hipStream_t stream) {
+                               int* tile_chl, int* tile_waves, int* tile_nw, int* tile_bad,
+                               unsigned long long* nan_mask, hipStream_t stream) {
   const dim3 rgrid((unsigned)((G.P + kBlock - 1) / kBlock), (unsigned)G.N);
   hipLaunchKernelGGL(geom_rx_table_kernel, rgrid, dim3(kBlock), 0, stream, G, rx_table);
   hipError_t e = hipGetLastError();
@@ -194,6 +261,10 @@ hipError_t launch_rxtab_tables(const GeomLaunch& G, float4* rx_table, float2* re
   const unsigned tiles = (unsigned)((G.P + 63) / 64);
   hipLaunchKernelGGL(geom_tile_scan_kernel, dim3(tiles), dim3(64), 0, stream, G, rx_table, tile_nc, tile_chl,
                      tile_waves, tile_nw, tile_bad);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(geom_nan_mask_kernel, dim3(tiles), dim3(64), 0, stream, G, rx_table, tile_nc, tile_chl,
+                     tile_waves, tile_nw, nan_mask);
   return hipGetLastError();
 }
 

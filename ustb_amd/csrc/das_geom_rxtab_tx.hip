@@ -69,6 +69,25 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
   const float rd = ron ? rv.x : kFar;
   const float ra = rv.y;
   const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
+  if (SAFE && !__any(ron && ra == ra)) {
+    // every nonzero receive weight on the tile is NaN: das_c's sum is NaN where
+    // some term is in the window, else zero; no gathers
+    bool any = false;
+    for (int i = 0; i < nw; ++i) {
+      const int wi = __builtin_amdgcn_readfirstlane(wl[i]);
+      const float4 tv = active ? A.tx[(size_t)wi * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+      float d;
+      {
+#pragma clang fp contract(off)
+        d = (((tv.y != 0.0f ? tv.x : kFar) + rd) - t0) * fs;
+      }
+      any = any || (d >= lo_ok && d < lim);
+    }
+    const float nan = __builtin_nanf("");
+#pragma unroll
+    for (int f = 0; f < FC; ++f) v[f] = any ? make_float2(nan, nan) : make_float2(0.0f, 0.0f);
+    return;
+  }
   const unsigned nb = (unsigned)(n * lst);
   float2 part[FC];
 #pragma unroll

@@ -66,6 +66,8 @@ struct GeomLaunch {
   // coefficients [F][W][N][L] of two float4.
   const float4* rx_table;
   const int* tile_nc;
+  const int* tile_ncf;  // [tiles] leading entries of the channel list with a weight other than NaN
+  const unsigned long long* nan_mask;  // [tiles] lanes whose sum the NaN-only channels make NaN
   const int* tile_chl;
   const int* tile_waves;
   const int* tile_nw;
@@ -88,10 +90,12 @@ hipError_t launch_tx_cf(const GeomLaunch& G, float4* part, float2* image, float*
                         hipStream_t stream);
 size_t tx_cf_workspace_bytes(const GeomLaunch& G);
 
-// Builds the variant-2 tables; tile_bad[t] = 1 where a nonzero weight or its
-// delay is not finite.
+// Builds the variant-2 tables; tile_nc is [2][tiles] (channel counts, then
+// the counts of the list's leading channels with a weight other than NaN);
+// tile_bad[t] = 1 where a nonzero weight or its delay is not finite.
 hipError_t launch_rxtab_tables(const GeomLaunch& g, float4* rx_table, float2* ref_table, int* tile_nc,
-                               int* tile_chl, int* tile_waves, int* tile_nw, int* tile_bad, hipStream_t stream);
+                               int* tile_chl, int* tile_waves, int* tile_nw, int* tile_bad,
+                               unsigned long long* nan_mask, hipStream_t stream);
 
 hipError_t launch_das_geom(const GeomLaunch& g, hipStream_t stream);
 

@@ -20,7 +20,11 @@
 // nonzero weight or its delay is not finite (e.g. the NaN apodization of
 // pixels at depth 0) take the SAFE form instead: a term enters the sum only
 // when it is in the window, and a channel's receive weight multiplies its
-// partial sum only when some term of the channel did.
+// partial sum only when some term of the channel did. Channels whose nonzero
+// weights on the tile are all NaN come last in the tile's list and are not
+// run: whether one of their terms is in the window (which makes das_c's sum
+// NaN) depends on the plan's geometry only, so the plan stores it as a
+// per-tile lane mask.
 #pragma once
 #include "das_geom_impl.h"
 
@@ -56,6 +60,8 @@ struct RxAcq {
   const float4* tx;        // [W][P] {td, ta, ta*cos(wd td), ta*sin(wd td)}
   const float4* rx;        // [N][P] {rd, ra, ra*cos(wd rd), ra*sin(wd rd)}
   const int* tile_nc;      // [tiles] channels with a nonzero receive weight
+  const int* tile_ncf;     // [tiles] leading entries of tile_chl with a weight other than NaN
+  const unsigned long long* nan_mask;  // [tiles] lanes made NaN by the list's NaN-only tail
   const int* tile_chl;     // [tiles][N] their indices, ascending
   const int* tile_nw;      // [tiles] waves with a nonzero transmit weight
   const int* tile_waves;   // [tiles][W] their indices, ascending
@@ -204,7 +210,9 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
   const int p = tile * TP + lane;
   const bool active = p < A.P;
   const size_t PS = (size_t)A.P;
-  const int nc = __builtin_amdgcn_readfirstlane(A.tile_nc[tile]);
+  // SAFE: the splits run the list's leading channels with a weight other
+  // than NaN; the NaN-only tail enters through the plan's lane mask
+  const int nc = __builtin_amdgcn_readfirstlane(SAFE ? A.tile_ncf[tile] : A.tile_nc[tile]);
   const int nw = __builtin_amdgcn_readfirstlane(A.tile_nw[tile]);
   const int* __restrict__ wl = A.tile_waves + (size_t)tile * A.W;
   const int* __restrict__ chl = A.tile_chl + (size_t)tile * A.N;
@@ -266,6 +274,7 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
 #pragma unroll
     for (int k = 1; k < WS; ++k) cadd(v, red[k][lane]);
     if (A.ref) v = cmul(v, A.ref[p]);
+    if (SAFE && ((A.nan_mask[tile] >> lane) & 1ull)) v = make_float2(__builtin_nanf(""), __builtin_nanf(""));
     A.out[(size_t)f * PS + p] = v;
   }
 }
@@ -278,6 +287,8 @@ inline RxAcq rx_acq(const GeomLaunch& G, int sm) {
   A.tx = G.tx_table;
   A.rx = G.rx_table;
   A.tile_nc = G.tile_nc;
+  A.tile_ncf = G.tile_ncf;
+  A.nan_mask = G.nan_mask;
   A.tile_chl = G.tile_chl;
   A.tile_nw = G.tile_nw;
   A.tile_waves = G.tile_waves;
