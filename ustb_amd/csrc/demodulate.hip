@@ -115,9 +115,11 @@ __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
     float2 mv[kSB];
 #pragma unroll
     for (int k = 0; k < kSB; ++k) {
-      const int m = lo + base + k * kDB + (int)threadIdx.x;
+      const int i = base + k * kDB + (int)threadIdx.x;
+      const int m = lo + i;
       int src = m < 0 ? (zero_phase ? -m : 0) : (m >= A.L ? (zero_phase ? 2 * A.L - 2 - m : 0) : m);
-      src = min(max(src, 0), A.L - 1);  // lanes past the window read a sample they do not use
+      // lanes past the window read the window's first sample (cached; unused)
+      src = i < total ? min(max(src, 0), A.L - 1) : min(max(lo, 0), A.L - 1);
       xv[k] = x[src];
       mv[k] = A.mix[src];
     }
