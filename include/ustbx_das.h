@@ -70,6 +70,18 @@ typedef struct ustbx_status {
                                          exp(i wd tx) * exp(i wd rx) */
 #define USTBX_FLAG_REF_PHASE 4u  /* geometry path: multiply by
                                     exp(-1j*2*w0*r_ref/c) (das.m:306-309) */
+#define USTBX_FLAG_PYTHON_ARITHMETIC 8u /* matrix ABI, with USTBX_EDGE_PYTHON and
+   linear interpolation: the Python reference's DAS._beamform arithmetic
+   (python das.py:217-250). `fs` holds the sample period dt [s] of the time
+   vector and `fd` holds w0 [rad/s]. Index (tau - t0) / dt in FP32; floor and
+   fraction exact; interpolation in FP64 rounded to complex64; times
+   rx_apo * tx_apo; times exp(1j * fl32(w0 * tau)) in complex64 where w0 != 0.
+   Every term with a nonzero (or NaN) apo * weight product is added, in or
+   out of the window (out: apo * 0), so NaN weights propagate as in the
+   reference. With USTBX_FLAG_DASC_ORDER the sums run in its loop order (w
+   outer, n inner). Limits: NaN channel data where apo is zero, and a NaN
+   weight whose other weight column is zero at every pixel (the reference
+   skips such columns, das.py:218-222), are not reproduced. */
 
 /* ---------------------------------------------------------------------------
  * 1. Matrix ABI: the das_c / das_cuda MEX contract.

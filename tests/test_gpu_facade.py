@@ -106,3 +106,28 @@ def test_beamform_static_matches_python_reference(cuda, dim):
     ref = O.python_beamform(*args)
     assert got.shape == ref.shape
     assert _rel(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("s,dim", [(s, d) for s in (0, 1) for d in (0, 1, 2, 3)])
+def test_beamform_static_matches_reference_vectors(cuda, s, dim):
+    """DAS._beamform on the Python reference's own output (golden vectors in
+    physical units; USTBX_FLAG_PYTHON_ARITHMETIC): RF bit for bit, IQ within
+    2e-6 of the peak (its cos/sin and complex product are libm's and NumPy's
+    on the host), NaN positions identical."""
+    from tests.test_golden_python import golden_cases
+
+    for s_, dim_, args, w0, ref in golden_cases():
+        if (s_, dim_) != (s, dim):
+            continue
+        print(f"case python golden set={s} dim={dim}", flush=True)
+        ch, t, ta, ra, td, rd = args
+        L_, N, W, F = ch.shape
+        got = DAS._beamform(ch, t, ta, ra, td, rd, w0, dim, N_pixels=ta.shape[0], N_channels=N, N_waves=W,
+                            N_frames=F)
+        assert got.shape == ref.shape and got.dtype == ref.dtype
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        if s == 0:
+            assert np.array_equal(got, ref, equal_nan=True)
+        else:
+            ok = ~np.isnan(ref)
+            assert np.max(np.abs(got[ok] - ref[ok])) <= 2e-6 * np.max(np.abs(ref[ok]))

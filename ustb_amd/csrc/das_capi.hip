@@ -226,6 +226,11 @@ static int check_desc(const ustbx_das_desc* d, ustbx_status* st) {
   if (d->L < 0 || d->N < 0 || d->W < 0 || d->F < 0 || d->P < 0)
     return set_status(st, USTBX_E_DIMENSIONS, "negative dimension");
   if (d->split < 0) return set_status(st, USTBX_E_ARG, "negative split");
+  if ((d->flags & USTBX_FLAG_PYTHON_ARITHMETIC) &&
+      (d->edge != USTBX_EDGE_PYTHON || d->interp != USTBX_INTERP_LINEAR || !(d->fs > 0.0f)))
+    return set_status(st, USTBX_E_ARG,
+                      "the Python arithmetic needs the Python edge rule, linear interpolation and a positive sample "
+                      "period");
   return USTBX_OK;
 }
 
@@ -273,6 +278,10 @@ int ustbx_das_run(const ustbx_das_desc* d, const void* ch, const float* tx_apo,
   m.dim = d->dim;
   m.interp = d->interp;
   m.edge = d->edge;
+  m.py = (d->flags & USTBX_FLAG_PYTHON_ARITHMETIC) != 0;
+  m.dt = m.py ? d->fs : 0.0f;
+  m.w0 = m.py ? d->fd : 0.0f;
+  if (m.py) m.wd = fabsf(d->fd) > 1.1920929e-7f ? d->fd : 0.0f;  // the phase where |w0| > FP32 eps (das.py:249)
   m.split = (d->flags & USTBX_FLAG_DASC_ORDER) ? 1 : (d->split ? d->split : matrix_auto_split(d->P));
   if (m.split != 1 && m.split != 2 && m.split != 4 && m.split != 8)
     return set_status(st, USTBX_E_ARG, "split must be 1, 2, 4 or 8");

@@ -18,6 +18,8 @@ struct MatLaunch {
   int L, N, W, F, P;
   float fs, t0, wd;
   int dim, interp, edge, split;
+  bool py;         // the Python reference's arithmetic (USTBX_FLAG_PYTHON_ARITHMETIC)
+  float dt, w0;    // its sample period and angular frequency
 };
 
 hipError_t launch_das_matrix(const MatLaunch& m, hipStream_t stream);

@@ -14,7 +14,8 @@
 //
 // Per-term arithmetic is das_c's (das_common.h: dasc_term / dasc_contrib),
 // with FMA contraction off, so for fd == 0 (no sin/cos) and SPLIT = 1 the
-// output is bit-identical to das_c.
+// output is bit-identical to das_c. PY instead follows the Python
+// reference's DAS._beamform term by term (py_term / py_contrib).
 #include "das_common.h"
 #include "das_launch.h"
 
@@ -30,7 +31,62 @@ struct MatArgs {
   int L, N, W, F, P;
   float fs, t0, wd;
   int edge;
+  float dt, w0;  // PY: sample period, angular frequency
 };
+
+// The Python reference's term (python das.py:223-250) in NumPy's dtypes:
+// delay = rx + tx and apo = rx_apo * tx_apo in float32; the index
+// (delay - t0) / dt in float32 (the Python scalars cast to float32); floor and
+// fraction exact (NumPy promotes float32 - int32 to float64); the
+// interpolation x0 (1 - b) + x1 b in float64, stored as complex64; times apo;
+// times exp(1j * w0 * delay) in complex64, whose argument is the float32
+// product and whose cos/sin are taken in FP64 and rounded.
+struct PyTerm {
+  int n0;
+  bool valid;
+  double b;
+  float apo, c, s;
+};
+
+template <bool IQ>
+__device__ __forceinline__ PyTerm py_term(float td, float rd, float ta, float ra, float t0, float dt, float w0,
+                                          int L) {
+#pragma clang fp contract(off)
+  PyTerm t;
+  const float delay = rd + td;                  // das.py:224
+  t.apo = ra * ta;                              // das.py:223
+  const float idx = (delay - t0) / dt;          // das.py:227
+  const float fl = floorf(idx);                 // das.py:228
+  t.valid = fl >= 0.0f && fl < (float)(L - 1);  // das.py:231 (NaN: false)
+  t.n0 = t.valid ? (int)fl : 0;
+  t.b = (double)idx - (double)fl;               // das.py:229
+  t.c = 1.0f;
+  t.s = 0.0f;
+  if (IQ) {
+    double sd, cd;
+    sincos((double)(w0 * delay), &sd, &cd);     // das.py:250
+    t.c = (float)cd;
+    t.s = (float)sd;
+  }
+  return t;
+}
+
+// apo * interpolated sample (zero outside the window, das.py:240-247), times
+// the phase factor.
+template <bool IQ>
+__device__ __forceinline__ float2 py_contrib(const PyTerm& t, const float2* __restrict__ trace) {
+#pragma clang fp contract(off)
+  float ivr = 0.0f, ivi = 0.0f;
+  if (t.valid) {
+    const float4 x = load_pair(trace, t.n0);
+    const double a = 1.0 - t.b;
+    ivr = (float)((double)x.x * a + (double)x.z * t.b);
+    ivi = (float)((double)x.y * a + (double)x.w * t.b);
+  }
+  const float vr = t.apo * ivr, vi = t.apo * ivi;
+  if (!IQ) return make_float2(vr, vi);
+  return make_float2(t.c * vr - t.s * vi, t.c * vi + t.s * vr);
+}
 
 // Gather and weight one term for one frame.
 template <int INTERP>
@@ -52,7 +108,7 @@ __device__ __forceinline__ float2 mat_contrib(const Term& t,
 // Unit loop over channels [nb, ne) for fixed (pixel, wave), U channels per
 // step: rx loads of the U channels are issued first, then the gathers, then
 // the accumulation in channel order (das_c's order within the group).
-template <int INTERP, bool IQ, int FC, int U>
+template <int INTERP, bool IQ, bool PY, int FC, int U>
 __device__ __forceinline__ void sum_channels(const MatArgs& A, int p, int w,
                                              float ta, float td, int nb, int ne,
                                              int f0, float2 (&acc)[FC]) {
@@ -72,7 +128,17 @@ __device__ __forceinline__ void sum_channels(const MatArgs& A, int p, int w,
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (ra[u] != 0.0f) {
+      if (PY) {
+        if (n + u < ne) {
+          const PyTerm t = py_term<IQ>(td, rd[u], ta, ra[u], A.t0, A.dt, A.w0, A.L);
+          if (t.apo != 0.0f) {  // NaN included
+            const float2* trace = A.ch + (size_t)w * LN + (size_t)(n + u) * A.L;
+#pragma unroll
+            for (int f = 0; f < FC; ++f)
+              if (f0 + f < A.F) cadd(acc[f], py_contrib<IQ>(t, trace + (size_t)(f0 + f) * LNW));
+          }
+        }
+      } else if (ra[u] != 0.0f) {
         const Term t = dasc_term<INTERP, IQ>(td, rd[u], ta, ra[u], A.t0, A.fs,
                                              A.wd, A.L, A.edge);
         if (t.valid) {
@@ -112,7 +178,7 @@ __device__ __forceinline__ void reduce_groups(float2 (&acc)[FC], int lp, int s,
   __syncthreads();
 }
 
-template <int MODE, int INTERP, bool IQ, int SPLIT, int FC>
+template <int MODE, int INTERP, bool IQ, bool PY, int SPLIT, int FC>
 __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
   constexpr int TP = kBlock / SPLIT;
   constexpr int U = (FC == 1) ? 4 : 1;
@@ -134,9 +200,9 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
       if (active) {
         for (int w = 0; w < A.W; ++w) {
           const float ta = A.tx_apo[(size_t)w * PS + p];
-          if (ta != 0.0f) {
+          if (PY || ta != 0.0f) {
             const float td = A.tx_delay[(size_t)w * PS + p];
-            sum_channels<INTERP, IQ, FC, U>(A, p, w, ta, td, nb, ne, f0, acc);
+            sum_channels<INTERP, IQ, PY, FC, U>(A, p, w, ta, td, nb, ne, f0, acc);
           }
         }
       }
@@ -153,9 +219,9 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
         for (int f = 0; f < FC; ++f) acc[f] = make_float2(0.0f, 0.0f);
         if (active) {
           const float ta = A.tx_apo[(size_t)w * PS + p];
-          if (ta != 0.0f) {
+          if (PY || ta != 0.0f) {
             const float td = A.tx_delay[(size_t)w * PS + p];
-            sum_channels<INTERP, IQ, FC, U>(A, p, w, ta, td, nb, ne, f0, acc);
+            sum_channels<INTERP, IQ, PY, FC, U>(A, p, w, ta, td, nb, ne, f0, acc);
           }
         }
         reduce_groups<SPLIT, FC>(acc, lp, s, red);
@@ -175,7 +241,20 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
 #pragma unroll
           for (int f = 0; f < FC; ++f) acc[f] = make_float2(0.0f, 0.0f);
           const float ra = A.rx_apo[(size_t)n * PS + p];
-          if (ra != 0.0f) {
+          if (PY) {
+            const float rd = A.rx_delay[(size_t)n * PS + p];
+            const float2* trace = A.ch + (size_t)n * A.L;
+            for (int w = 0; w < A.W; ++w) {
+              const PyTerm t = py_term<IQ>(A.tx_delay[(size_t)w * PS + p], rd, A.tx_apo[(size_t)w * PS + p], ra,
+                                           A.t0, A.dt, A.w0, A.L);
+              if (t.apo != 0.0f) {
+#pragma unroll
+                for (int f = 0; f < FC; ++f)
+                  if (f0 + f < A.F)
+                    cadd(acc[f], py_contrib<IQ>(t, trace + (size_t)w * LN + (size_t)(f0 + f) * LNW));
+              }
+            }
+          } else if (ra != 0.0f) {
             const float rd = A.rx_delay[(size_t)n * PS + p];
             for (int w = 0; w < A.W; ++w) {
               const float ta = A.tx_apo[(size_t)w * PS + p];
@@ -212,7 +291,15 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
 #pragma unroll
             for (int f = 0; f < FC; ++f) v[f] = make_float2(0.0f, 0.0f);
             const float ra = A.rx_apo[(size_t)n * PS + p];
-            if (ta != 0.0f && ra != 0.0f) {
+            if (PY) {
+              const PyTerm t = py_term<IQ>(td, A.rx_delay[(size_t)n * PS + p], ta, ra, A.t0, A.dt, A.w0, A.L);
+              if (t.apo != 0.0f) {
+                const float2* trace = A.ch + (size_t)w * LN + (size_t)n * A.L;
+#pragma unroll
+                for (int f = 0; f < FC; ++f)
+                  if (f0 + f < A.F) v[f] = py_contrib<IQ>(t, trace + (size_t)(f0 + f) * LNW);
+              }
+            } else if (ta != 0.0f && ra != 0.0f) {
               const float rd = A.rx_delay[(size_t)n * PS + p];
               const Term t = dasc_term<INTERP, IQ>(td, rd, ta, ra, A.t0, A.fs,
                                                    A.wd, A.L, A.edge);
@@ -237,41 +324,42 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
 // ---------------------------------------------------------------------------
 // launcher
 // ---------------------------------------------------------------------------
-template <int MODE, int INTERP, bool IQ, int SPLIT, int FC>
+template <int MODE, int INTERP, bool IQ, bool PY, int SPLIT, int FC>
 static hipError_t launch_mat(const MatArgs& A, hipStream_t stream) {
   constexpr int TP = kBlock / SPLIT;
   const unsigned grid = (unsigned)((A.P + TP - 1) / TP);
-  hipLaunchKernelGGL((das_matrix_kernel<MODE, INTERP, IQ, SPLIT, FC>), dim3(grid),
+  hipLaunchKernelGGL((das_matrix_kernel<MODE, INTERP, IQ, PY, SPLIT, FC>), dim3(grid),
                      dim3(kBlock), 0, stream, A);
   return hipGetLastError();
 }
 
-template <int MODE, int INTERP, bool IQ, int SPLIT>
+template <int MODE, int INTERP, bool IQ, bool PY, int SPLIT>
 static hipError_t pick_fc(const MatArgs& A, hipStream_t s) {
-  if (A.F == 1) return launch_mat<MODE, INTERP, IQ, SPLIT, 1>(A, s);
-  return launch_mat<MODE, INTERP, IQ, SPLIT, 4>(A, s);
+  if (A.F == 1) return launch_mat<MODE, INTERP, IQ, PY, SPLIT, 1>(A, s);
+  return launch_mat<MODE, INTERP, IQ, PY, SPLIT, 4>(A, s);
 }
 
-template <int MODE, int INTERP, bool IQ>
+template <int MODE, int INTERP, bool IQ, bool PY>
 static hipError_t pick_split(const MatArgs& A, int split, hipStream_t s) {
   switch (split) {
-    case 1: return pick_fc<MODE, INTERP, IQ, 1>(A, s);
-    case 2: return pick_fc<MODE, INTERP, IQ, 2>(A, s);
-    case 4: return pick_fc<MODE, INTERP, IQ, 4>(A, s);
-    default: return pick_fc<MODE, INTERP, IQ, 8>(A, s);
+    case 1: return pick_fc<MODE, INTERP, IQ, PY, 1>(A, s);
+    case 2: return pick_fc<MODE, INTERP, IQ, PY, 2>(A, s);
+    case 4: return pick_fc<MODE, INTERP, IQ, PY, 4>(A, s);
+    default: return pick_fc<MODE, INTERP, IQ, PY, 8>(A, s);
   }
 }
 
-template <int MODE, int INTERP>
+template <int MODE, int INTERP, bool PY>
 static hipError_t pick_iq(const MatArgs& A, int split, hipStream_t s) {
-  if (A.wd != 0.0f) return pick_split<MODE, INTERP, true>(A, split, s);
-  return pick_split<MODE, INTERP, false>(A, split, s);
+  if (A.wd != 0.0f) return pick_split<MODE, INTERP, true, PY>(A, split, s);
+  return pick_split<MODE, INTERP, false, PY>(A, split, s);
 }
 
 template <int MODE>
-static hipError_t pick_interp(const MatArgs& A, int interp, int split, hipStream_t s) {
-  if (interp == INTERP_CUBIC) return pick_iq<MODE, INTERP_CUBIC>(A, split, s);
-  return pick_iq<MODE, INTERP_LINEAR>(A, split, s);
+static hipError_t pick_interp(const MatArgs& A, int interp, bool py, int split, hipStream_t s) {
+  if (interp == INTERP_CUBIC) return pick_iq<MODE, INTERP_CUBIC, false>(A, split, s);
+  if (py) return pick_iq<MODE, INTERP_LINEAR, true>(A, split, s);
+  return pick_iq<MODE, INTERP_LINEAR, false>(A, split, s);
 }
 
 __global__ __launch_bounds__(kBlock) void zero_fill_kernel(float2* __restrict__ p, size_t n) {
@@ -298,11 +386,12 @@ hipError_t launch_das_matrix(const MatLaunch& m, hipStream_t stream) {
   A.out = reinterpret_cast<float2*>(m.out);
   A.L = m.L; A.N = m.N; A.W = m.W; A.F = m.F; A.P = m.P;
   A.fs = m.fs; A.t0 = m.t0; A.wd = m.wd; A.edge = m.edge;
+  A.dt = m.dt; A.w0 = m.w0;
   switch (m.dim) {
-    case 0: return pick_interp<0>(A, m.interp, m.split, stream);
-    case 1: return pick_interp<1>(A, m.interp, m.split, stream);
-    case 2: return pick_interp<2>(A, m.interp, m.split, stream);
-    default: return pick_interp<3>(A, m.interp, m.split, stream);
+    case 0: return pick_interp<0>(A, m.interp, m.py, m.split, stream);
+    case 1: return pick_interp<1>(A, m.interp, m.py, m.split, stream);
+    case 2: return pick_interp<2>(A, m.interp, m.py, m.split, stream);
+    default: return pick_interp<3>(A, m.interp, m.py, m.split, stream);
   }
 }
 

@@ -14,7 +14,6 @@ USTBX_E_UNSUPPORTED for it.
 from __future__ import annotations
 
 import hashlib
-import math
 import time
 
 import numpy as np
@@ -121,24 +120,24 @@ class DAS:
     def _beamform(ch_data, ch_data_time, tx_apodization, rx_apodization, transmit_delay, receive_delay, w0, dim,
                   N_pixels, N_channels, N_waves, N_frames):
         """The reference's DAS._beamform contract (das.py:203-261) on the HIP
-        matrix kernel: ch_data (L, N, W, F) complex64; apodization and delays
-        (P, N) / (P, W) float32; w0 in rad/s; the Python edge rule
+        matrix kernel with its arithmetic (USTBX_FLAG_PYTHON_ARITHMETIC) and
+        loop order: ch_data (L, N, W, F) complex64; apodization and delays
+        (P, N) / (P, W) float32; the time vector (t0 = t[0], dt = t[1] - t[0]
+        in float32, das.py:217-218); w0 in rad/s; the Python edge rule
         0 <= floor(d) < L-1. Returns (P, N', W', F) complex64 (NumPy)."""
         import torch
 
         from .ops import das
 
         dev = torch.device("cuda", 0)
-        t = np.asarray(ch_data_time, dtype=np.float32)
+        t = np.asarray(ch_data_time)
         t0 = float(t[0])
         dt = float(t[1] - t[0]) if len(t) > 1 else 1.0
-        fs = 1.0 / dt
-        fd = float(np.float32(w0) / np.float32(2.0 * math.pi)) if abs(w0) > np.finfo(np.float32).eps else 0.0
         ch = np.asarray(ch_data)
         while ch.ndim < 4:
             ch = ch[..., None]
         tch = torch.from_numpy(np.ascontiguousarray(ch.transpose(3, 2, 1, 0).astype(np.complex64))).to(dev)
         T = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32).T)).to(dev)
-        out = das(tch, fs, t0, T(tx_apodization), T(rx_apodization), T(transmit_delay), T(receive_delay), fd,
-                  int(dim), edge=L.EDGE_PYTHON)
+        out = das(tch, dt, t0, T(tx_apodization), T(rx_apodization), T(transmit_delay), T(receive_delay),
+                  float(np.float32(w0)), int(dim), edge=L.EDGE_PYTHON, python_arithmetic=True, dasc_order=True)
         return out.permute(3, 2, 1, 0).cpu().numpy()

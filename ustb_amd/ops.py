@@ -39,8 +39,11 @@ def _require_cuda(*ts):
 def das(ch: torch.Tensor, fs: float, t0: float, tx_apo: torch.Tensor, rx_apo: torch.Tensor,
         tx_delay: torch.Tensor, rx_delay: torch.Tensor, fd: float, dim: int, *,
         interp: int = L.INTERP_LINEAR, edge: int = L.EDGE_DASC, dasc_order: bool = False,
-        split: int = 0, out: torch.Tensor | None = None) -> torch.Tensor:
-    """mex.das_c / mex.das_cuda contract on device tensors (all four modes)."""
+        python_arithmetic: bool = False, split: int = 0, out: torch.Tensor | None = None) -> torch.Tensor:
+    """mex.das_c / mex.das_cuda contract on device tensors (all four modes).
+    python_arithmetic: the Python reference's DAS._beamform arithmetic
+    (linear, Python edge rule); then fs is the sample period dt [s] and fd is
+    w0 [rad/s]."""
     _require_cuda(ch, tx_apo, rx_apo, tx_delay, rx_delay)
     if ch.dtype != torch.complex64:
         raise TypeError("The channel data should be single precision complex")
@@ -60,8 +63,9 @@ def das(ch: torch.Tensor, fs: float, t0: float, tx_apo: torch.Tensor, rx_apo: to
         out = torch.empty(shape, dtype=torch.complex64, device=ch.device)
     elif tuple(out.shape) != shape or out.dtype != torch.complex64 or not out.is_contiguous():
         raise ValueError(f"out must be a contiguous complex64 tensor of shape {shape}")
+    flags = (L.FLAG_DASC_ORDER if dasc_order else 0) | (L.FLAG_PYTHON_ARITHMETIC if python_arithmetic else 0)
     d = L.DasDesc(L=Ls, N=N, W=W, F=F, P=P, fs=fs, t0=t0, fd=fd, dim=int(dim), interp=int(interp),
-                  edge=int(edge), flags=(L.FLAG_DASC_ORDER if dasc_order else 0), split=int(split))
+                  edge=int(edge), flags=flags, split=int(split))
     st = L.Status()
     rc = L.lib().ustbx_das_run(ctypes.byref(d), ch.data_ptr(), tx_apo.data_ptr(), rx_apo.data_ptr(),
                                tx_delay.data_ptr(), rx_delay.data_ptr(), out.data_ptr(),
