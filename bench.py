@@ -304,13 +304,13 @@ def run_frontend(args, torch, dev):
     an = torch.empty((1, W, N, Ls), dtype=torch.complex64, device=dev)
     copy = copy_probe(torch, dev)
 
-    def line(name, fn, out_elems, taps):
+    def line(name, fn, out_elems, taps, samples=S):
         k_ms, w_ms = _timed(torch, stream, fn, args.warmup, args.steps)
-        alg = 4 * S + 8 * out_elems
+        alg = 4 * samples + 8 * out_elems
         gbs = alg / (k_ms * 1e-3) / 1e9
         log(f"{name}: {k_ms:.4f} ms/step (kernel), {w_ms:.4f} ms wall, {gbs:.0f} GB/s")
         return {"kernel_ms": round(k_ms, 4), "wall_ms": round(w_ms, 4), "taps": taps,
-                "Gsamples_s": round(S / (w_ms * 1e-3) / 1e9, 3), "algorithmic_bytes": alg,
+                "Gsamples_s": round(samples / (w_ms * 1e-3) / 1e9, 3), "algorithmic_bytes": alg,
                 "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                 "frac_of_achievable": round(gbs / copy["GBs"], 4)}
 
@@ -319,6 +319,10 @@ def run_frontend(args, torch, dev):
     zp = line("FastDemodulation (zero-phase, /1)",
               lambda: ops.demodulate(x, fs, t0, f0, b_p, down_p, L.FIR_ZERO_PHASE), S, len(b_p))
     hil = line("hilbert", lambda: ops.hilbert(x, an), S, 0)
+    # a length of the form 2^a 3^b 5^c 7^d (the mixed-radix LDS path), as RTB's 3500
+    x35 = x.reshape(-1, Ls)[:, :1750].contiguous().reshape(-1, 3500)
+    an35 = torch.empty(x35.shape, dtype=torch.complex64, device=dev)
+    hil35 = line("hilbert L=3500", lambda: ops.hilbert(x35, an35), x35.numel(), 0, samples=x35.numel())
 
     errors = {}
     if not args.no_check:
@@ -333,6 +337,9 @@ def run_frontend(args, torch, dev):
         got = ops.hilbert(x, an).reshape(-1, Ls)[:k].cpu().numpy()
         ref = O.hilbert(sub.astype(np.float64))
         errors["hilbert_vs_scipy"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        got = ops.hilbert(x35, an35)[:k].cpu().numpy()
+        ref = O.hilbert(x35[:k].cpu().numpy().astype(np.float64))
+        errors["hilbert_L3500_vs_scipy"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
         log(f"errors {errors}")
     result = {
         "metric": "RF front end Gsamples/s (RF time samples)",
@@ -356,6 +363,7 @@ def run_frontend(args, torch, dev):
         "fast_demodulation": demod,
         "python_fast_demodulation": zp,
         "hilbert": hil,
+        "hilbert_L3500": hil35,
         "errors": errors,
     }
     if not args.no_cpu_baseline:

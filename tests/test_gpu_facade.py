@@ -30,10 +30,11 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
-@pytest.mark.parametrize("L", [777, 1000, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("L", [777, 1001, 512, 1024, 2048, 4096, 600, 1000, 1536, 2100, 3500])
 def test_hilbert_matches_scipy(cuda, L):
-    """hipFFT path (777, 1000) and the one-block LDS FFT path (powers of two
-    512 .. 4096)."""
+    """hipFFT path (777 = 3 7 37, 1001 = 7 11 13), the one-block LDS FFT path
+    (powers of two 512 .. 4096) and its mixed-radix form (600, 1000, 1536,
+    2100, 3500: products of 2, 3, 5, 7)."""
     print(f"case hilbert L={L}", flush=True)
     rng = np.random.default_rng(L)
     x = rng.standard_normal((2, 3, 5, L)).astype(np.float32)

@@ -7,9 +7,11 @@
 // LDS (one complex signal a + i b; forward and inverse radix-8 Stockham FFTs,
 // a last radix-2/4 stage where log2 L is not a multiple of 3, twiddles from a
 // per-device table rounded from FP64), so HBM sees the traces read once and
-// the analytic signals written once. Other lengths: hipFFT (R2C forward, C2C
-// inverse, with the weights in between); plans are cached per (device, L,
-// traces), and calls of one (L, traces) share the plan's spectrum buffer.
+// the analytic signals written once. Other lengths of the form 2^a 3^b 5^c
+// 7^d up to 4096: the same with mixed-radix stages. The rest: hipFFT (R2C
+// forward, C2C inverse, with the weights in between); plans are cached per
+// (device, L, traces), and calls of one (L, traces) share the plan's spectrum
+// buffer.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 #include <stdarg.h>
@@ -221,6 +223,209 @@ __global__ __launch_bounds__(N / 8) void hilbert_lds_kernel(const float* __restr
   }
 }
 
+// ---- LDS path, other lengths: N = 2^a 3^b 5^c 7^d <= 4096 ----
+// Mixed-radix Stockham stages (radix 8 while 8 divides what is left, then 4
+// or 2, then 3, 5, 7), kMixT threads per block, each holding up to
+// mix_maxb(R) butterflies of a stage; twiddles exp(-2 pi i m / N) from a plain device
+// table (L2); radix 3, 5, 7 as direct DFTs with constant cos/sin.
+constexpr int kMixT = 512;
+constexpr int kMixMax = 4096;
+constexpr int kMixStages = 12;
+
+template <int R>
+struct RootsR;
+template <>
+struct RootsR<3> {
+  static constexpr float c[3] = {1.0f, -0.5f, -0.5f};
+  static constexpr float s[3] = {0.0f, 0.8660254037844387f, -0.8660254037844387f};
+};
+template <>
+struct RootsR<5> {
+  static constexpr float c[5] = {1.0f, 0.30901699437494745f, -0.8090169943749475f, -0.8090169943749475f,
+                                 0.30901699437494745f};
+  static constexpr float s[5] = {0.0f, 0.9510565162951535f, 0.5877852522924731f, -0.5877852522924731f,
+                                 -0.9510565162951535f};
+};
+template <>
+struct RootsR<7> {
+  static constexpr float c[7] = {1.0f, 0.6234898018587336f, -0.22252093395631434f, -0.900968867902419f,
+                                 -0.900968867902419f, -0.22252093395631434f, 0.6234898018587336f};
+  static constexpr float s[7] = {0.0f, 0.7818314824680298f, 0.9749279121818236f, 0.43388373911755823f,
+                                 -0.43388373911755823f, -0.9749279121818236f, -0.7818314824680298f};
+};
+
+// X[q] = sum_r v[r] exp(s 2 pi i r q / R), R = 3, 5, 7.
+template <int R>
+__device__ __forceinline__ void dft_odd(float2* v, float s) {
+  float2 x[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    float re = 0.0f, im = 0.0f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int k = (r * q) % R;
+      const float c = RootsR<R>::c[k], sn = s * RootsR<R>::s[k];
+      re += v[r].x * c - v[r].y * sn;
+      im += v[r].x * sn + v[r].y * c;
+    }
+    x[q] = make_float2(re, im);
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = x[q];
+}
+
+// butterflies per thread of a radix-R stage at N <= kMixMax, kMixT threads
+__host__ __device__ constexpr int mix_maxb(int R) { return (kMixMax / R + kMixT - 1) / kMixT; }
+
+template <int R>
+__device__ __forceinline__ void mix_stage(float2* buf, const float2* __restrict__ tw, int N, int Ns, float s) {
+  constexpr int MB = mix_maxb(R);
+  const int nb = N / R, step = N / (Ns * R);
+  const int tid = threadIdx.x;
+  float2 v[MB][R];
+#pragma unroll
+  for (int k = 0; k < MB; ++k) {
+    const int j = tid + k * kMixT;
+    if (j < nb) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[k][r] = buf[pad8(j + r * nb)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < MB; ++k) {
+    const int j = tid + k * kMixT;
+    if (j < nb) {
+      const int kk = j % Ns;
+      if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[kk * r * step];  // exp(-2 pi i m / N); conjugated for the inverse
+          w.y = -s * w.y;
+          v[k][r] = cmulf(v[k][r], w);
+        }
+      }
+      if (R == 8) dft8(v[k], s);
+      else if (R == 4) dft4(v[k], s);
+      else if (R == 2) dft2(v[k], s);
+      else dft_odd<(R == 3 || R == 5 || R == 7) ? R : 3>(v[k], s);
+      const int d = (j - kk) * R + kk;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pad8(d + r * Ns)] = v[k][r];
+    }
+  }
+  __syncthreads();
+}
+
+struct MixPlan {
+  int n;                     // stages
+  int radix[kMixStages];
+};
+
+__device__ __forceinline__ void fft_mixed(float2* buf, const float2* __restrict__ tw, int N, const MixPlan& pl,
+                                          float s) {
+  int Ns = 1;
+  for (int i = 0; i < pl.n; ++i) {
+    const int R = pl.radix[i];  // uniform
+    switch (R) {
+      case 8: mix_stage<8>(buf, tw, N, Ns, s); break;
+      case 4: mix_stage<4>(buf, tw, N, Ns, s); break;
+      case 2: mix_stage<2>(buf, tw, N, Ns, s); break;
+      case 3: mix_stage<3>(buf, tw, N, Ns, s); break;
+      case 5: mix_stage<5>(buf, tw, N, Ns, s); break;
+      default: mix_stage<7>(buf, tw, N, Ns, s); break;
+    }
+    Ns *= R;
+  }
+}
+
+// Two traces per block, as hilbert_lds_kernel; the weights for odd N
+// (MATLAB hilbert): h[0] = 1, h[k] = 2 for 0 < k < N/2, h[N/2] = 1 for even N.
+__global__ __launch_bounds__(kMixT) void hilbert_mixed_kernel(const float* __restrict__ rf, float2* __restrict__ z,
+                                                              const float2* __restrict__ tw, long long traces,
+                                                              int N, MixPlan pl) {
+  __shared__ float2 buf[kMixMax + kMixMax / 8];
+  const size_t ta = 2 * (size_t)blockIdx.x;
+  const bool has_b = (long long)ta + 1 < traces;
+  const float* __restrict__ xa = rf + ta * N;
+  const float* __restrict__ xb = xa + N;
+  for (int i = threadIdx.x; i < N; i += kMixT) buf[pad8(i)] = make_float2(xa[i], has_b ? xb[i] : 0.0f);
+  __syncthreads();
+  fft_mixed(buf, tw, N, pl, -1.0f);
+  const float inv = 1.0f / (float)N;
+  for (int i = threadIdx.x; i < N; i += kMixT) {
+    const float h = (i == 0 || 2 * i == N) ? inv : (2 * i < N ? 2.0f * inv : 0.0f);
+    const float2 v = buf[pad8(i)];
+    buf[pad8(i)] = make_float2(v.x * h, v.y * h);
+  }
+  __syncthreads();
+  fft_mixed(buf, tw, N, pl, 1.0f);
+  // the inputs again (L2-resident by now) for the real parts
+  float2* __restrict__ za = z + ta * N;
+  float2* __restrict__ zb = za + N;
+  for (int i = threadIdx.x; i < N; i += kMixT) {
+    const float2 w = buf[pad8(i)];
+    const float a = xa[i], bb = has_b ? xb[i] : 0.0f;
+    za[i] = make_float2(a, w.y - bb);
+    if (has_b) zb[i] = make_float2(bb, a - w.x);
+  }
+}
+
+// The stage radices of N (8s, then 4 or 2, then 3s, 5s, 7s), or false.
+bool mix_plan(int N, MixPlan& pl) {
+  if (N < 2 || N > kMixMax) return false;
+  pl.n = 0;
+  int n = N;
+  auto push = [&](int r) {
+    if (pl.n >= kMixStages) return false;
+    pl.radix[pl.n++] = r;
+    n /= r;
+    return true;
+  };
+  while (n % 8 == 0)
+    if (!push(8)) return false;
+  if (n % 4 == 0 && !push(4)) return false;
+  if (n % 2 == 0 && !push(2)) return false;
+  for (int p : {3, 5, 7})
+    while (n % p == 0)
+      if (!push(p)) return false;
+  return n == 1;
+}
+
+__global__ void plain_twiddle_kernel(float2* __restrict__ tw, int N) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= N) return;
+  double sv, cv;
+  sincospi(-2.0 * (double)m / (double)N, &sv, &cv);
+  tw[m] = make_float2((float)cv, (float)sv);
+}
+
+std::map<std::pair<int, int>, float2*> g_plain_twiddles;
+
+// exp(-2 pi i m / N), m < N, of the current device: built once (and waited
+// for on the caller's stream). Caller holds g_mu.
+hipError_t plain_twiddles(int dev, int N, hipStream_t s, const float2** out) {
+  const auto key = std::make_pair(dev, N);
+  auto it = g_plain_twiddles.find(key);
+  if (it != g_plain_twiddles.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  float2* tw = nullptr;
+  hipError_t e = hipMalloc(&tw, sizeof(float2) * (size_t)N);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(plain_twiddle_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, tw, N);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    (void)hipFree(tw);
+    return e;
+  }
+  g_plain_twiddles.emplace(key, tw);
+  *out = tw;
+  return hipSuccess;
+}
+
 // Entry i of the twiddle table of length N (FP64, rounded to FP32).
 __global__ void twiddle_kernel(float2* __restrict__ tw, int N) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -298,6 +503,22 @@ extern "C" int ustbx_hilbert(const float* rf, void* analytic, int32_t L, int64_t
         case 2048: e = launch_hilbert_lds<2048>(rf, z, traces, tw, s); break;
         default: e = launch_hilbert_lds<4096>(rf, z, traces, tw, s); break;
       }
+    }
+    if (e != hipSuccess) return fail(st, USTBX_E_GPU, "HIP error: %s", hipGetErrorString(e));
+    if (st) {
+      st->code = USTBX_OK;
+      st->message[0] = 0;
+    }
+    return USTBX_OK;
+  }
+  MixPlan pl;
+  if (mix_plan(L, pl)) {
+    const float2* tw = nullptr;
+    e = plain_twiddles(dev, L, s, &tw);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(hilbert_mixed_kernel, dim3((unsigned)((traces + 1) / 2)), dim3(kMixT), 0, s, rf,
+                         reinterpret_cast<float2*>(analytic), tw, (long long)traces, (int)L, pl);
+      e = hipGetLastError();
     }
     if (e != hipSuccess) return fail(st, USTBX_E_GPU, "HIP error: %s", hipGetErrorString(e));
     if (st) {
