@@ -175,3 +175,19 @@ def test_rtb_workload_follows_the_reference_example():
         assert ap.window == Window.hamming and ap.f_number == (f, f) and ap.minimum_aperture == (3e-3, 3e-3)
     assert acq.tx_model == SphericalTransmitDelayModel.hybrid
     assert configs.rtb(with_data=False, tx_model=SphericalTransmitDelayModel.unified).tx_model == 2
+
+
+def test_python_arithmetic_flag_argument_checks():
+    """USTBX_FLAG_PYTHON_ARITHMETIC needs the Python edge rule, linear
+    interpolation and a positive sample period; the descriptor is checked
+    before any device pointer is used."""
+    import ctypes
+
+    base = dict(L=16, N=2, W=1, F=1, P=4, fs=1e-7, t0=0.0, fd=0.0, dim=3, interp=L.INTERP_LINEAR,
+                edge=L.EDGE_PYTHON, flags=L.FLAG_PYTHON_ARITHMETIC, split=0)
+    for over in (dict(edge=L.EDGE_DASC), dict(interp=L.INTERP_CUBIC), dict(fs=0.0)):
+        d = L.DasDesc(**dict(base, **over))
+        st = L.Status()
+        rc = L.lib().ustbx_das_run(ctypes.byref(d), None, None, None, None, None, None, None, ctypes.byref(st))
+        assert rc == L.E_ARG, over
+        assert b"Python arithmetic" in st.message

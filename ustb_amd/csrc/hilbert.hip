@@ -9,9 +9,8 @@
 // per-device table rounded from FP64), so HBM sees the traces read once and
 // the analytic signals written once. Other lengths of the form 2^a 3^b 5^c
 // 7^d up to 4096: the same with mixed-radix stages. The rest: hipFFT (R2C
-// forward, C2C inverse, with the weights in between); plans are cached per
-// (device, L, traces), and calls of one (L, traces) share the plan's spectrum
-// buffer.
+// forward, C2C inverse, with the weights in between); plans and their
+// spectrum buffers are cached per (device, L, traces, stream).
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 #include <stdarg.h>
@@ -31,7 +30,9 @@ struct FftPlans {
 };
 
 std::mutex g_mu;
-std::map<std::tuple<int, int, long long>, FftPlans> g_plans;
+// per (device, L, traces, stream): a plan's work areas and spectrum buffer are
+// never shared by calls that may run at the same time on different streams
+std::map<std::tuple<int, int, long long, hipStream_t>, FftPlans> g_plans;
 
 int fail(ustbx_status* st, int code, const char* fmt, ...) {
   if (st) {
@@ -527,7 +528,7 @@ extern "C" int ustbx_hilbert(const float* rf, void* analytic, int32_t L, int64_t
     }
     return USTBX_OK;
   }
-  auto key = std::make_tuple(dev, (int)L, (long long)traces);
+  auto key = std::make_tuple(dev, (int)L, (long long)traces, s);
   auto it = g_plans.find(key);
   if (it == g_plans.end()) {
     FftPlans p;
