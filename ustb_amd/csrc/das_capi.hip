@@ -404,7 +404,8 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   GeomLaunch& G = pl->G;
   const size_t P = (size_t)G.P, tiles = (P + 63) / 64;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  const size_t b_rx = al(sizeof(float4) * (size_t)G.N * P);
+  G.rx_compact = G.wd == 0.0f ? 1 : 0;  // RF: {rd, ra} (the phase factor is (ra, 0))
+  const size_t b_rx = al((G.rx_compact ? sizeof(float2) : sizeof(float4)) * (size_t)G.N * P);
   const size_t b_nc = al(sizeof(int) * 2 * tiles);  // channel counts, then counts of the non-NaN ones
   const size_t b_cl = al(sizeof(int) * tiles * (size_t)G.N);
   const size_t b_wl = al(sizeof(int) * tiles * (size_t)G.W);

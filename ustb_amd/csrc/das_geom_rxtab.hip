@@ -15,7 +15,8 @@
 
 namespace ustbx {
 
-// Receive table: one lane per (pixel, channel); grid.y = channel.
+// Receive table: one lane per (pixel, channel); grid.y = channel. RF plans
+// (rx_compact) store float2 {rd, ra}.
 __global__ __launch_bounds__(kBlock) void geom_rx_table_kernel(GeomLaunch G, float4* __restrict__ table) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
   const int n = blockIdx.y;
@@ -25,6 +26,10 @@ __global__ __launch_bounds__(kBlock) void geom_rx_table_kernel(GeomLaunch G, flo
   const double ex = G.ex[n], ey = G.ey[n], ez = G.ez[n];
   const float ra = rx_apod_f32(G.rx, px, py, pz, ex, ey, ez, n, p, G.P);
   const float rd = rx_delay_f32(px, py, pz, ex, ey, ez, G.c, G.lens);
+  if (G.rx_compact) {
+    reinterpret_cast<float2*>(table)[(size_t)n * G.P + p] = make_float2(rd, ra);
+    return;
+  }
   float sv = 0.0f, cv = 1.0f;
   if (G.wd != 0.0f) sincos_wd_hw(G.wd, rd, sv, cv);
   table[(size_t)n * G.P + p] = make_float4(rd, ra, ra * cv, ra * sv);
@@ -67,7 +72,7 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
   }
   int nc = 0;
   for (int n = 0; n < G.N; ++n) {
-    const float4 v = active ? rxt[(size_t)n * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 v = active ? rx_entry(rxt, G.rx_compact, (size_t)n * PS + p) : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool on = v.y != 0.0f;
     if (on && !(isfinite(v.x) && isfinite(v.z) && isfinite(v.w))) bad = true;
     if (__any(on && v.y == v.y)) {
@@ -77,7 +82,7 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
   }
   const int nfin = nc;
   for (int n = 0; n < G.N; ++n) {
-    const float4 v = active ? rxt[(size_t)n * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 v = active ? rx_entry(rxt, G.rx_compact, (size_t)n * PS + p) : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool on = v.y != 0.0f;
     if (__any(on) && !__any(on && v.y == v.y)) {
       if (threadIdx.x == 0) tile_chl[(size_t)tile * G.N + nc] = n;
@@ -113,7 +118,7 @@ __global__ __launch_bounds__(64) void geom_nan_mask_kernel(GeomLaunch G, const f
   bool any = false;
   for (int k = nfin; k < nc; ++k) {
     const int n = tile_chl[(size_t)tile * G.N + k];
-    const float4 rv = active ? rxt[(size_t)n * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 rv = active ? rx_entry(rxt, G.rx_compact, (size_t)n * PS + p) : make_float4(0.f, 0.f, 0.f, 0.f);
     if (rv.y == 0.0f) continue;
     for (int i = 0; i < nw && !any; ++i) {
       const int w = tile_waves[(size_t)tile * G.W + i];
@@ -158,12 +163,12 @@ __global__ __launch_bounds__(kBlock) void spline_coef_kernel(const float2* __res
 }
 
 // dimension.both from a plan's resident tables: one block per (tile, frame).
-template <int SM, int WC, int WS>
+template <int SM, int WC, int WS, int RXL>
 __global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(RxAcq A, const int* __restrict__ order) {
   __shared__ float2 red[WS][64];
   int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (order) tile = __builtin_amdgcn_readfirstlane(order[tile]);  // block-uniform
-  rxtab_tile<SM, WC, WS, false>(A, tile, (int)blockIdx.y, red);
+  rxtab_tile<SM, WC, WS, false, RXL>(A, tile, (int)blockIdx.y, red);
 }
 
 // The same where some tiles hold a non-finite nonzero weight or delay (the
@@ -171,7 +176,7 @@ __global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(RxAcq A, const 
 // those with the SAFE body, dispatched first so that their long channel loops
 // overlap the rest; blocks bad8 .. (bad rounded up to whole XCD rounds) take
 // the other tiles through the XCD remap; blocks bad .. bad8 - 1 have none.
-template <int SM, int WC, int WS>
+template <int SM, int WC, int WS, int RXL>
 __global__ __launch_bounds__(64 * WS) void das_both_rxtab_mixed_kernel(RxAcq A, const int* __restrict__ order,
                                                                        int good, int bad, int bad8) {
   __shared__ float2 red[WS][64];
@@ -179,31 +184,37 @@ __global__ __launch_bounds__(64 * WS) void das_both_rxtab_mixed_kernel(RxAcq A, 
   if (b < bad8) {
     if (b >= bad) return;  // whole block, before any barrier
     const int tile = __builtin_amdgcn_readfirstlane(order[good + b]);
-    rxtab_tile<SM, WC, WS, true>(A, tile, (int)blockIdx.y, red);
+    rxtab_tile<SM, WC, WS, true, RXL>(A, tile, (int)blockIdx.y, red);
   } else {
     const int tile = __builtin_amdgcn_readfirstlane(order[xcd_tile(b - bad8, good)]);
-    rxtab_tile<SM, WC, WS, false>(A, tile, (int)blockIdx.y, red);
+    rxtab_tile<SM, WC, WS, false, RXL>(A, tile, (int)blockIdx.y, red);
   }
 }
 
 // Plans whose weights are all finite launch the branch-free kernel; the
 // others the mixed one (same tile shape for both bodies, so that a batch,
 // which runs either body per tile, computes the same sums).
-template <int SM, int WC, int WS>
-static hipError_t launch_rxtab_t(const GeomLaunch& G, hipStream_t stream) {
+template <int SM, int WC, int WS, int RXL>
+static hipError_t launch_rxtab_l(const GeomLaunch& G, hipStream_t stream) {
   const int tiles = (G.P + 63) / 64;
   const int good = G.tile_good, bad = tiles - good;
   const RxAcq A = rx_acq(G, SM);
   if (bad == 0) {
-    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS>), dim3((unsigned)tiles, (unsigned)G.F), dim3(64 * WS), 0,
-                       stream, A, G.tile_order);
+    hipLaunchKernelGGL((das_both_rxtab_kernel<SM, WC, WS, RXL>), dim3((unsigned)tiles, (unsigned)G.F),
+                       dim3(64 * WS), 0, stream, A, G.tile_order);
     return hipGetLastError();
   }
   if (!G.tile_order) return hipErrorInvalidValue;
   const int bad8 = (bad + 7) & ~7;
-  hipLaunchKernelGGL((das_both_rxtab_mixed_kernel<SM, WC, WS>), dim3((unsigned)(bad8 + good), (unsigned)G.F),
+  hipLaunchKernelGGL((das_both_rxtab_mixed_kernel<SM, WC, WS, RXL>), dim3((unsigned)(bad8 + good), (unsigned)G.F),
                      dim3(64 * WS), 0, stream, A, G.tile_order, good, bad, bad8);
   return hipGetLastError();
+}
+
+template <int SM, int WC, int WS>
+static hipError_t launch_rxtab_t(const GeomLaunch& G, hipStream_t stream) {
+  return G.rx_compact ? launch_rxtab_l<SM, WC, WS, RXL_COMPACT>(G, stream)
+                      : launch_rxtab_l<SM, WC, WS, RXL_FULL>(G, stream);
 }
 
 template <int SM, int WS>

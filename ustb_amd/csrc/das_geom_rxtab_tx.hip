@@ -63,7 +63,7 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
 #pragma unroll
   for (int f = 0; f < FC; ++f) v[f] = make_float2(0.0f, 0.0f);
   float4 rv = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (active) rv = A.rx[(size_t)n * PS + p];
+  if (active) rv = rx_entry(A.rx, A.rx_compact, (size_t)n * PS + p);
   const bool ron = rv.y != 0.0f;
   if (!__any(ron)) return;  // no receive weight on the tile: zeros
   const float rd = ron ? rv.x : kFar;

@@ -67,6 +67,7 @@ struct GeomLaunch {
   // reference phase per pixel (or null). Per run (variant 3): interpolation
   // coefficients [F][W][N][L] of two float4.
   const float4* rx_table;
+  int rx_compact;       // RF plans: the receive table holds float2 {rd, ra}
   const int* tile_nc;
   const int* tile_ncf;  // [tiles] leading entries of the channel list with a weight other than NaN
   const unsigned long long* nan_mask;  // [tiles] lanes whose sum the NaN-only channels make NaN

@@ -58,7 +58,7 @@ struct RxAcq {
   const void* data;        // [F][W][N][L] complex64 (SM_COEF: coefficients)
   float2* out;             // [F][P]
   const float4* tx;        // [W][P] {td, ta, ta*cos(wd td), ta*sin(wd td)}
-  const float4* rx;        // [N][P] {rd, ra, ra*cos(wd rd), ra*sin(wd rd)}
+  const float4* rx;        // [N][P] {rd, ra, ra*cos(wd rd), ra*sin(wd rd)}; rx_compact: float2 {rd, ra}
   const int* tile_nc;      // [tiles] channels with a nonzero receive weight
   const int* tile_ncf;     // [tiles] leading entries of tile_chl with a weight other than NaN
   const unsigned long long* nan_mask;  // [tiles] lanes made NaN by the list's NaN-only tail
@@ -69,8 +69,22 @@ struct RxAcq {
   int P, N, W, L;
   float t0, fs;
   int frame_bytes;         // W * N * L * stride (< 0x7FFFFE00)
-  int pad_;
+  int rx_compact;          // RF plans (wd = 0): the receive table holds float2 {rd, ra}
 };
+
+// Receive-table entry idx as {rd, ra, ra*cos(wd rd), ra*sin(wd rd)}: RF plans
+// store {rd, ra} only (half the bytes), whose phase factor is (ra, ra * 0).
+template <bool COMPACT>
+__device__ __forceinline__ float4 rx_entry_t(const float4* __restrict__ rxt, size_t idx) {
+  if (COMPACT) {
+    const float2 v = reinterpret_cast<const float2*>(rxt)[idx];
+    return make_float4(v.x, v.y, v.y, v.y * 0.0f);
+  }
+  return rxt[idx];
+}
+__device__ __forceinline__ float4 rx_entry(const float4* __restrict__ rxt, int compact, size_t idx) {
+  return compact ? rx_entry_t<true>(rxt, idx) : rx_entry_t<false>(rxt, idx);
+}
 
 // Keys' cubic weights (a = -1/2) in Horner form.
 __device__ __forceinline__ void keys_weights_horner(float b, float& c0, float& c1, float& c2, float& c3) {
@@ -163,7 +177,26 @@ __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const
   return part;
 }
 
+// One channel's fold: acc += R * (its partial sum over the split's waves).
 template <int SM, int WC, bool GUARD, bool SAFE>
+__device__ __forceinline__ void rxtab_fold(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC], const float2 (&T)[WC],
+                                           const int (&wb)[WC], int nj, float4 rv, int nb, float t0, float fs,
+                                           float lim, float2& acc) {
+  const bool ron = rv.y != 0.0f;
+  const float rd = ron ? rv.x : kFar;
+  const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
+  bool any = false;
+  const float2 part = rxtab_channel<SM, WC, GUARD, SAFE>(rs, td, T, wb, nj, rd, nb, t0, fs, lim, any);
+  if (SAFE) {
+    float2 t = acc;
+    cfma(t, R, part);
+    acc = any ? t : acc;
+  } else {
+    cfma(acc, R, part);
+  }
+}
+
+template <int SM, int WC, bool GUARD, bool SAFE, bool COMPACT>
 __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
                                                const float2 (&T)[WC], const int (&wb)[WC], int nj,
                                                const float4* __restrict__ rxt, const int* __restrict__ chl,
@@ -171,37 +204,32 @@ __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const 
                                                float t0, float fs, float lim, float2& acc) {
   // channels chl[0], chl[cstride], ... (nc of them); the receive row of the
   // next one in flight during this one, its index one further ahead (a
-  // scalar load off the critical path)
+  // scalar load off the critical path). (Four rows in flight for splits of
+  // few waves measured slower: the registers cost more occupancy than the
+  // latency they hide.)
   float4 rn = make_float4(0.f, 0.f, 0.f, 0.f);
   int n = __builtin_amdgcn_readfirstlane(chl[0]);
   int n_next = nc > 1 ? __builtin_amdgcn_readfirstlane(chl[cstride]) : 0;
-  if (active) rn = rxt[(size_t)n * PS + p];
+  if (active) rn = rx_entry_t<COMPACT>(rxt, (size_t)n * PS + p);
   for (int k = 0; k < nc; ++k) {
     const float4 rv = rn;
     const int nb = n * lst;
     if (k + 1 < nc) {
       n = n_next;
-      if (active) rn = rxt[(size_t)n * PS + p];
+      if (active) rn = rx_entry_t<COMPACT>(rxt, (size_t)n * PS + p);
       if (k + 2 < nc) n_next = __builtin_amdgcn_readfirstlane(chl[(k + 2) * cstride]);
     }
-    const bool ron = rv.y != 0.0f;
-    const float rd = ron ? rv.x : kFar;
-    const float2 R = ron ? make_float2(rv.z, rv.w) : make_float2(0.0f, 0.0f);
-    bool any = false;
-    const float2 part = rxtab_channel<SM, WC, GUARD, SAFE>(rs, td, T, wb, nj, rd, nb, t0, fs, lim, any);
-    if (SAFE) {
-      float2 t = acc;
-      cfma(t, R, part);
-      acc = any ? t : acc;
-    } else {
-      cfma(acc, R, part);
-    }
+    rxtab_fold<SM, WC, GUARD, SAFE>(rs, td, T, wb, nj, rv, nb, t0, fs, lim, acc);
   }
 }
 
+// Receive-table layout of a kernel: float4 only, float2 only (RF plans), or
+// either, chosen per acquisition at run time (the batch kernel).
+enum { RXL_FULL = 0, RXL_COMPACT = 1, RXL_EITHER = 2 };
+
 // One tile of one frame; every thread of the block calls it (it ends with a
 // block barrier and the split-ordered reduction through red).
-template <int SM, int WC, int WS, bool SAFE = false>
+template <int SM, int WC, int WS, bool SAFE = false, int RXL = RXL_EITHER>
 __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, float2 (*red)[64]) {
   constexpr int TP = 64;
   constexpr int ST = SampleMode<SM>::kStride;
@@ -222,6 +250,7 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
   const __amdgpu_buffer_rsrc_t rs =
       make_rsrc((const char*)A.data + (size_t)f * A.frame_bytes, A.frame_bytes);
   const float lim = (float)(SM == SM_LINEAR ? A.L - 1 : A.L - 2);
+  const int compact = RXL == RXL_EITHER ? __builtin_amdgcn_readfirstlane(A.rx_compact) : (RXL == RXL_COMPACT);
 
   // The splits share out the tile's waves round-robin, WC per split and round.
   // When the waves need fewer splits than WS in one round (wneed), the block
@@ -260,12 +289,21 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
         }
       }
     }
-    if (nj == WC)
-      rxtab_channels<SM, WC, false, SAFE>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst, A.t0, A.fs,
-                                          lim, acc);
-    else
-      rxtab_channels<SM, WC, true, SAFE>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst, A.t0, A.fs,
-                                         lim, acc);
+    if (nj == WC) {
+      if (compact)
+        rxtab_channels<SM, WC, false, SAFE, true>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst,
+                                                  A.t0, A.fs, lim, acc);
+      else
+        rxtab_channels<SM, WC, false, SAFE, false>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst,
+                                                   A.t0, A.fs, lim, acc);
+    } else {
+      if (compact)
+        rxtab_channels<SM, WC, true, SAFE, true>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst,
+                                                 A.t0, A.fs, lim, acc);
+      else
+        rxtab_channels<SM, WC, true, SAFE, false>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst,
+                                                  A.t0, A.fs, lim, acc);
+    }
   }
   red[s][lane] = acc;
   __syncthreads();
@@ -300,7 +338,7 @@ inline RxAcq rx_acq(const GeomLaunch& G, int sm) {
   A.t0 = G.t0;
   A.fs = G.fs;
   A.frame_bytes = G.L * G.N * G.W * (sm == SM_COEF ? 32 : 8);
-  A.pad_ = 0;
+  A.rx_compact = G.rx_compact;
   return A;
 }
 
