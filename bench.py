@@ -37,10 +37,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; C4: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 3; C4: 1)")
-    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4", "C5", "RTB", "RF2IQ"],
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4", "C5", "RTB", "RF2IQ", "UFF"],
                     help="RTB: focused transmits (retrospective transmit beamforming), the focused-wave "
                          "delay models; RF2IQ: the RF front end (fast_demodulation and hilbert) on STA RF "
-                         "of 128 waves x 128 channels x 2048 samples")
+                         "of 128 waves x 128 channels x 2048 samples; UFF: C3's frames streamed from a UFF file through the reader "
+                         "into a one-frame plan")
     ap.add_argument("--tx-model", default="hybrid", choices=["spherical", "unified", "hybrid", "blended"],
                     help="RTB: spherical_transmit_delay_model of the focused waves")
     ap.add_argument("--frames", type=int, default=600, help="C5: requests served (the trace's first ones)")
@@ -137,6 +138,9 @@ def main():
         return
     if args.config == "RF2IQ":
         run_frontend(args, torch, dev)
+        return
+    if args.config == "UFF":
+        run_uff(args, torch, dev)
         return
     if args.abi == "matrix":
         run_matrix(args, torch, dev)
@@ -375,6 +379,94 @@ def run_frontend(args, torch, dev):
         result["cpu_baseline"] = {"value": round(sub.size / dt / 1e9, 4), "unit": "Gsamples/s", "cores": 1,
                                   "kind": "port", "sample": f"the oracle's fast_demodulation (NumPy/SciPy lfilter, "
                                                             f"FP64) on {sub.shape[0]} of {W * N} traces, {dt:.2f} s"}
+    print(json.dumps(result))
+
+
+def run_uff(args, torch, dev):
+    """UFF channel-data reader in the serving path: C3's acquisition (phased
+    64 elements, 31 diverging waves, 2048 IQ samples, sector 256 x 1024,
+    dimension transmit) with 8 frames written to a UFF file
+    (uff.write_object conventions), then streamed frame by frame: the
+    reader's hyperslab of one frame (real and imag datasets), one host to
+    device copy from pinned memory, one beamforming pass of a one-frame plan.
+    Units: frames per second end to end; the reader's rate in MB/s of
+    channel data. The first frame's image equals the in-memory frame's bit
+    for bit."""
+    import shutil
+    import tempfile
+
+    from ustb_amd import configs, uff_io
+    from ustb_amd.geometry import make_plan
+    from ustb_amd.uff import ChannelData
+
+    frames = 8
+    acq = configs.c3(frames=frames)
+    cd = ChannelData(probe=acq.probe, sequence=acq.sequence, sampling_frequency=acq.fs, initial_time=acq.t0,
+                     sound_speed=acq.c, modulation_frequency=acq.fd,
+                     data=np.ascontiguousarray(acq.data.transpose(3, 2, 1, 0)))
+    tmp = tempfile.mkdtemp(prefix="ustbx_uff_")
+    path = os.path.join(tmp, "c3.uff")
+    try:
+        t = time.perf_counter()
+        uff_io.write_object(path, cd, "channel_data")
+        write_s = time.perf_counter() - t
+        fbytes = acq.data[0].nbytes
+        log(f"wrote {frames} frames ({os.path.getsize(path) / 1e6:.0f} MB) in {write_s:.1f} s")
+        plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=False,
+                         **acq.plan_kwargs(n_frames=1))
+        reader = uff_io.FrameReader(path, "/channel_data")
+        host = torch.empty((1,) + acq.data.shape[1:], dtype=torch.complex64).pin_memory()
+        dch = torch.empty(host.shape, dtype=torch.complex64, device=dev)
+        out = torch.empty(plan.out_shape, dtype=torch.complex64, device=dev)
+        read_ms, gpu_ms, total = [], [], []
+        first = None
+        for rep in range(2):  # the first pass warms the page cache and the plan
+            for f in range(frames):
+                t0 = time.perf_counter()
+                x = reader.frame(f)
+                t1 = time.perf_counter()
+                host[0].copy_(torch.from_numpy(x))
+                dch.copy_(host, non_blocking=True)
+                plan.run(dch, out)
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                if rep == 1:
+                    read_ms.append((t1 - t0) * 1e3)
+                    gpu_ms.append((t2 - t1) * 1e3)
+                    total.append((t2 - t0) * 1e3)
+                if rep == 0 and f == 0:
+                    first = out.cpu().numpy()
+        ref_plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=False,
+                             **acq.plan_kwargs(n_frames=1))
+        ref = ref_plan.run(torch.from_numpy(np.ascontiguousarray(acq.data[:1])).to(dev)).cpu().numpy()
+        same = bool(np.array_equal(first, ref))
+        ref_plan.close()
+        plan.close()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    ms = float(np.mean(total))
+    rd = float(np.mean(read_ms))
+    result = {
+        "metric": "UFF frames/s read and beamformed (C3 frames, one at a time)",
+        "value": round(1e3 / ms, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": frames,
+        "warmup": frames,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "UFF: C3 phased DW 31 waves, sector 256x1024, transmit, frames from a UFF file",
+                   "frames": frames, "frame_bytes": fbytes},
+        "reader": {"ms_per_frame": round(rd, 3), "MB_s": round(fbytes / (rd * 1e-3) / 1e6, 1),
+                   "note": "HDF5 hyperslabs of the real and imag datasets, interleaved on the host"},
+        "copy_and_beamform_ms": round(float(np.mean(gpu_ms)), 3),
+        "errors": {"file_frame_equals_memory_frame_bitwise": same},
+    }
+    log(f"UFF: {ms:.2f} ms per frame (reader {rd:.2f} ms, copy + beamform {np.mean(gpu_ms):.2f} ms), same={same}")
     print(json.dumps(result))
 
 
