@@ -387,11 +387,15 @@ def run_uff(args, torch, dev):
     64 elements, 31 diverging waves, 2048 IQ samples, sector 256 x 1024,
     dimension transmit) with 8 frames written to a UFF file
     (uff.write_object conventions), then streamed frame by frame: the
-    reader's hyperslab of one frame (real and imag datasets), one host to
-    device copy from pinned memory, one beamforming pass of a one-frame plan.
-    Units: frames per second end to end; the reader's rate in MB/s of
-    channel data. The first frame's image equals the in-memory frame's bit
-    for bit."""
+    reader's hyperslabs of one frame's real and imag datasets straight into
+    a pinned staging buffer, one host to device copy, the complex frame
+    built on the GPU from the two planes, one beamforming pass of a
+    one-frame plan. Two staging buffers let the host read frame f+1 while
+    the GPU copies and beamforms frame f. Units: frames per second end to
+    end over the pipelined pass; the reader's own rate in MB/s of channel
+    data and the copy + beamform time come from a pass with a
+    synchronisation after every frame. The first frame's image equals the
+    in-memory frame's bit for bit."""
     import shutil
     import tempfile
 
@@ -415,36 +419,58 @@ def run_uff(args, torch, dev):
         plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=False,
                          **acq.plan_kwargs(n_frames=1))
         reader = uff_io.FrameReader(path, "/channel_data")
-        host = torch.empty((1,) + acq.data.shape[1:], dtype=torch.complex64).pin_memory()
-        dch = torch.empty(host.shape, dtype=torch.complex64, device=dev)
-        out = torch.empty(plan.out_shape, dtype=torch.complex64, device=dev)
-        read_ms, gpu_ms, total = [], [], []
+        pshape = (2,) + acq.data.shape[1:]
+        host = [torch.empty(pshape, dtype=torch.float32).pin_memory() for _ in range(2)]
+        hview = [h.numpy() for h in host]
+        dpl = [torch.empty(pshape, dtype=torch.float32, device=dev) for _ in range(2)]
+        dch = [torch.empty((1,) + acq.data.shape[1:], dtype=torch.complex64, device=dev) for _ in range(2)]
+        outs = [torch.empty(plan.out_shape, dtype=torch.complex64, device=dev) for _ in range(2)]
+        copied = [None, None]
+
+        def step(f, k):
+            if copied[k] is not None:
+                copied[k].synchronize()  # staging buffer k's previous copy has left the host
+            reader.planes(f, out=hview[k])
+            dpl[k].copy_(host[k], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            copied[k] = ev
+            torch.complex(dpl[k][0], dpl[k][1], out=dch[k][0])
+            plan.run(dch[k], outs[k])
+
+        read_ms, gpu_ms = [], []
         first = None
-        for rep in range(2):  # the first pass warms the page cache and the plan
-            for f in range(frames):
-                t0 = time.perf_counter()
-                x = reader.frame(f)
-                t1 = time.perf_counter()
-                host[0].copy_(torch.from_numpy(x))
-                dch.copy_(host, non_blocking=True)
-                plan.run(dch, out)
-                torch.cuda.synchronize()
-                t2 = time.perf_counter()
-                if rep == 1:
-                    read_ms.append((t1 - t0) * 1e3)
-                    gpu_ms.append((t2 - t1) * 1e3)
-                    total.append((t2 - t0) * 1e3)
-                if rep == 0 and f == 0:
-                    first = out.cpu().numpy()
+        for f in range(frames):  # serial pass: warms the page cache and the plan, times the parts
+            t0 = time.perf_counter()
+            reader.planes(f, out=hview[0])
+            t1 = time.perf_counter()
+            dpl[0].copy_(host[0], non_blocking=True)
+            torch.complex(dpl[0][0], dpl[0][1], out=dch[0][0])
+            plan.run(dch[0], outs[0])
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if f == 0:
+                first = outs[0].cpu().numpy()
+            else:  # frame 0 carries the plan's first launch and a cold page cache
+                read_ms.append((t1 - t0) * 1e3)
+                gpu_ms.append((t2 - t1) * 1e3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for f in range(frames):  # pipelined pass: the timed one
+            step(f, f % 2)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / frames
         ref_plan = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=False,
                              **acq.plan_kwargs(n_frames=1))
         ref = ref_plan.run(torch.from_numpy(np.ascontiguousarray(acq.data[:1])).to(dev)).cpu().numpy()
         same = bool(np.array_equal(first, ref))
+        last = bool(np.array_equal(outs[(frames - 1) % 2].cpu().numpy(),
+                                   ref_plan.run(torch.from_numpy(np.ascontiguousarray(acq.data[frames - 1:]))
+                                                .to(dev)).cpu().numpy()))
         ref_plan.close()
         plan.close()
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
-    ms = float(np.mean(total))
     rd = float(np.mean(read_ms))
     result = {
         "metric": "UFF frames/s read and beamformed (C3 frames, one at a time)",
@@ -462,11 +488,14 @@ def run_uff(args, torch, dev):
         "config": {"workload": "UFF: C3 phased DW 31 waves, sector 256x1024, transmit, frames from a UFF file",
                    "frames": frames, "frame_bytes": fbytes},
         "reader": {"ms_per_frame": round(rd, 3), "MB_s": round(fbytes / (rd * 1e-3) / 1e6, 1),
-                   "note": "HDF5 hyperslabs of the real and imag datasets, interleaved on the host"},
+                   "note": "HDF5 hyperslabs of the real and imag datasets into pinned staging; "
+                           "the complex frame is built on the GPU"},
         "copy_and_beamform_ms": round(float(np.mean(gpu_ms)), 3),
-        "errors": {"file_frame_equals_memory_frame_bitwise": same},
+        "errors": {"first_frame_equals_memory_frame_bitwise": same,
+                   "last_pipelined_frame_equals_memory_frame_bitwise": last},
     }
-    log(f"UFF: {ms:.2f} ms per frame (reader {rd:.2f} ms, copy + beamform {np.mean(gpu_ms):.2f} ms), same={same}")
+    log(f"UFF: {ms:.2f} ms per frame pipelined (reader {rd:.2f} ms, copy + beamform {np.mean(gpu_ms):.2f} ms), "
+        f"same={same} last={last}")
     print(json.dumps(result))
 
 

@@ -145,3 +145,31 @@ def test_matrix_array_and_errors(tmp_path):
         U.read_object(path, "/nothing")
     with pytest.raises(FileNotFoundError):
         U.read_object(str(tmp_path / "missing.uff"), "/channel_data")
+
+
+@pytest.mark.parametrize("F", [1, 3])
+def test_frame_reader_into_buffers(tmp_path, F):
+    """frame(f, out=...) fills a caller's buffer (IQ and RF) and planes(f)
+    returns the stored real and imag planes of the same frame."""
+    for complex_data in (True, False):
+        cd = _cd(F=F, W=3, complex_data=complex_data)
+        path = str(tmp_path / f"g{int(complex_data)}.uff")
+        U.write_object(path, cd, "channel_data")
+        fr = U.FrameReader(path, "/channel_data")
+        ref = np.asarray(cd.data).transpose(3, 2, 1, 0)
+        out = np.empty(ref.shape[1:], ref.dtype)
+        for f in reversed(range(F)):
+            got = fr.frame(f, out=out)
+            assert got is out and np.array_equal(out, ref[f])
+        with pytest.raises(ValueError, match="C-contiguous"):
+            fr.frame(0, out=np.empty(ref.shape[1:], np.float64))
+        if complex_data:
+            buf = np.empty((2,) + ref.shape[1:], np.float32)
+            for f in range(F):
+                assert fr.planes(f, out=buf) is buf
+                assert np.array_equal(buf[0], ref[f].real) and np.array_equal(buf[1], ref[f].imag)
+            with pytest.raises(IndexError):
+                fr.planes(F)
+        else:
+            with pytest.raises(ValueError, match="IQ"):
+                fr.planes(0)

@@ -224,9 +224,11 @@ def _write_dataset(loc: int, name: str, c_order: np.ndarray, dims: list[int]):
     return ds
 
 
-def _read_dataset(loc: int, name: str, first: int | None = None) -> np.ndarray:
+def _read_dataset(loc: int, name: str, first: int | None = None, out: np.ndarray | None = None) -> np.ndarray:
     """The dataset in its stored (C-order) dimensions as float32; with
-    `first`, only index `first` of the leading dimension (a hyperslab)."""
+    `first`, only index `first` of the leading dimension (a hyperslab). With
+    `out` (C-contiguous float32 holding exactly that many values), HDF5 reads
+    into it and it is returned reshaped to those dimensions."""
     L = h5()
     ds = _ok(L.H5Dopen2(loc, name.encode(), H5P_DEFAULT), f"dataset {name}")
     sp = L.H5Dget_space(ds)
@@ -235,19 +237,25 @@ def _read_dataset(loc: int, name: str, first: int | None = None) -> np.ndarray:
     L.H5Sget_simple_extent_dims(sp, d, None)
     dims = [int(d[i]) for i in range(nd)]
     try:
+        count = dims if first is None else [1] + dims[1:]
+        if out is None:
+            out = np.empty(count, dtype=np.float32)
+        elif out.dtype != np.float32 or not out.flags.c_contiguous or out.size != int(np.prod(count)):
+            raise ValueError(f"out must be C-contiguous float32 with {int(np.prod(count))} values for {name}")
+        else:
+            out = out.reshape(count)
         if first is None:
-            out = np.empty(dims, dtype=np.float32)
             _ok(L.H5Dread(ds, L.FLOAT, H5S_ALL, H5S_ALL, H5P_DEFAULT, out.ctypes.data), f"dataset {name}")
             return out
         if not 0 <= first < dims[0]:
             raise IndexError(f"index {first} outside the leading dimension {dims[0]} of {name}")
-        count = [1] + dims[1:]
         _ok(L.H5Sselect_hyperslab(sp, H5S_SELECT_SET, _dims([first] + [0] * (nd - 1)), None, _dims(count), None),
             "hyperslab")
         mem = L.H5Screate_simple(nd, _dims(count), None)
-        out = np.empty(count, dtype=np.float32)
-        _ok(L.H5Dread(ds, L.FLOAT, mem, sp, H5P_DEFAULT, out.ctypes.data), f"dataset {name}")
-        L.H5Sclose(mem)
+        try:
+            _ok(L.H5Dread(ds, L.FLOAT, mem, sp, H5P_DEFAULT, out.ctypes.data), f"dataset {name}")
+        finally:
+            L.H5Sclose(mem)
         return out
     finally:
         L.H5Sclose(sp)
@@ -600,33 +608,73 @@ class FrameReader:
         # [L N W F] with trailing singletons dropped, reversed
         self.shape = tuple([1] * (4 - len(dims)) + dims)
         self.frames = self.shape[0]
+        self._parts = None  # IQ: the real and imag staging buffers, made on first use
 
-    def frame(self, f: int) -> np.ndarray:
-        """Frame f as (W, N, L): complex64 for IQ data, float32 for RF."""
+    def frame(self, f: int, out: np.ndarray | None = None) -> np.ndarray:
+        """Frame f as (W, N, L): complex64 for IQ data, float32 for RF.
+
+        With `out` (a C-contiguous array of that shape and type, e.g. the
+        NumPy view of a pinned staging buffer) the frame is written there:
+        RF is read straight into it; IQ's real and imag datasets are read
+        into two float32 buffers the reader keeps and then interleaved into
+        it, so a stream of frames allocates nothing per frame."""
         if not 0 <= f < self.frames:
             raise IndexError(f"frame {f} of {self.frames}")
+        shape = self.shape[1:]
+        dtype = np.complex64 if self.complex else np.float32
+        if out is None:
+            out = np.empty(shape, dtype=dtype)
+        elif out.dtype != dtype or tuple(out.shape) != shape or not out.flags.c_contiguous:
+            raise ValueError(f"out must be a C-contiguous {np.dtype(dtype).name} array of shape {shape}")
         group, _, leaf = self.location.rpartition("/")
         if self.complex:
             group, names = self.location, ("real", "imag")
+            if self._parts is None:
+                self._parts = (np.empty(shape, np.float32), np.empty(shape, np.float32))
+            dests = self._parts
         else:
-            names = (leaf,)
-        got = []
+            names, dests = (leaf,), (out,)
         with _File(self.filename, "r") as fh:
             L = h5()
             g = L.H5Gopen2(fh.id, group.encode(), H5P_DEFAULT)
             try:
-                for name in names:
+                for name, dst in zip(names, dests):
                     # a stored 4-D dataset is read one frame at a time; with
                     # the frame dimension dropped (one frame) it is the frame
                     four = self._stored_ndim(g, name) == 4
-                    got.append(_read_dataset(g, name, f if four else None))
+                    _read_dataset(g, name, f if four else None, out=dst)
             finally:
                 L.H5Gclose(g)
-        shape = self.shape[1:]
+        if self.complex:
+            pair = out.view(np.float32).reshape(shape + (2,))
+            pair[..., 0] = self._parts[0]
+            pair[..., 1] = self._parts[1]
+        return out
+
+    def planes(self, f: int, out: np.ndarray | None = None) -> np.ndarray:
+        """IQ frame f as its stored real and imag planes, (2, W, N, L)
+        float32, each read by HDF5 straight into its half of `out` (or of a
+        new array): no host-side interleave. A caller that copies the
+        planes to the GPU builds the complex frame there (the interleave at
+        device-memory speed instead of a strided host pass)."""
         if not self.complex:
-            return got[0].reshape(shape)
-        out = np.empty(shape, dtype=np.complex64)
-        out.real, out.imag = got[0].reshape(shape), got[1].reshape(shape)
+            raise ValueError("planes() is for IQ (real/imag) channel data; RF frames come from frame()")
+        if not 0 <= f < self.frames:
+            raise IndexError(f"frame {f} of {self.frames}")
+        shape = (2,) + self.shape[1:]
+        if out is None:
+            out = np.empty(shape, dtype=np.float32)
+        elif out.dtype != np.float32 or tuple(out.shape) != shape or not out.flags.c_contiguous:
+            raise ValueError(f"out must be a C-contiguous float32 array of shape {shape}")
+        with _File(self.filename, "r") as fh:
+            L = h5()
+            g = L.H5Gopen2(fh.id, self.location.encode(), H5P_DEFAULT)
+            try:
+                for k, name in enumerate(("real", "imag")):
+                    four = self._stored_ndim(g, name) == 4
+                    _read_dataset(g, name, f if four else None, out=out[k])
+            finally:
+                L.H5Gclose(g)
         return out
 
     @staticmethod
