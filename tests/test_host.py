@@ -191,3 +191,35 @@ def test_python_arithmetic_flag_argument_checks():
         rc = L.lib().ustbx_das_run(ctypes.byref(d), None, None, None, None, None, None, None, ctypes.byref(st))
         assert rc == L.E_ARG, over
         assert b"Python arithmetic" in st.message
+
+
+def test_plan_create_rejects_oversized_and_empty_geometry():
+    """ustbx_plan_create checks the geometry on the host before any device
+    work: a frame of channel data of 2 GiB or more (the gathers address a
+    frame with 32-bit byte offsets) is E_CAPACITY; an empty scan, F < 1 or
+    L < 2 are E_DIMENSIONS."""
+    import ctypes
+
+    from ustb_amd.geometry import build_geometry
+    from ustb_amd.uff import LinearArray, LinearScan, Wave
+
+    probe = LinearArray(128, 0.3e-3)
+    scan = LinearScan(np.linspace(-5e-3, 5e-3, 8), np.linspace(5e-3, 10e-3, 8))
+    base = dict(n_samples=1024, n_frames=1, sampling_frequency=20e6, initial_time=0.0,
+                modulation_frequency=0.0, sound_speed=1540.0)
+
+    def create(seq=None, sc=None, **kw):
+        g, keep = build_geometry(probe, sc or scan, seq or [Wave.plane(0.0)], **dict(base, **kw))
+        h, st = ctypes.c_void_p(), L.Status()
+        rc = L.lib().ustbx_plan_create(ctypes.byref(g), ctypes.byref(h), ctypes.byref(st))
+        assert not h.value  # nothing created
+        return rc, st.message.decode()
+
+    # 128 channels x 2 waves x 2^20 samples x 8 B = 2 GiB
+    rc, msg = create(seq=[Wave.plane(0.0), Wave.plane(0.1)], n_samples=1 << 20)
+    assert rc == L.E_CAPACITY and "2 GiB" in msg
+    for kw in (dict(n_samples=1), dict(n_frames=0)):
+        rc, _ = create(**kw)
+        assert rc == L.E_DIMENSIONS, kw
+    rc, _ = create(sc=LinearScan(np.zeros(0), np.zeros(4)))
+    assert rc == L.E_DIMENSIONS
