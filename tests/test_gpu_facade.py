@@ -52,6 +52,28 @@ def test_hilbert_matches_scipy(cuda, L):
     assert np.max(np.abs(z - np.exp(2j * np.pi * k * n / L)[None])) <= 2e-5
 
 
+def test_hilbert_plan_cache_past_its_limit(cuda):
+    """The hipFFT path keeps at most 8 plans per device (one per length,
+    trace count and stream) and releases the least recently used one: 11
+    trace counts in a row, then the first again on a second stream, all
+    match scipy."""
+    L = 777
+    rng = np.random.default_rng(5)
+    s2 = torch.cuda.Stream(device=cuda)
+    for k, T in enumerate(list(range(1, 12)) + [1, 2]):
+        print(f"case hilbert cache T={T}", flush=True)
+        x = rng.standard_normal((T, L)).astype(np.float32)
+        xd = torch.from_numpy(x).to(cuda)
+        if k >= 11:
+            s2.wait_stream(torch.cuda.current_stream(cuda))
+            with torch.cuda.stream(s2):
+                got = hilbert(xd)
+            torch.cuda.current_stream(cuda).wait_stream(s2)
+        else:
+            got = hilbert(xd)
+        assert _rel(got.cpu().numpy(), O.hilbert(x.astype(np.float64), axis=-1)) <= TOL
+
+
 def _das(case, data, dim=Dimension.both):
     d = DAS()
     d.channel_data = ChannelData(probe=case["probe"], sequence=case["seq"], sampling_frequency=case["fs"],

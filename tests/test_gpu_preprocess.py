@@ -61,6 +61,22 @@ def test_python_demodulation_matches_oracle(cuda, fs, fc, fdown):
     assert _rel(got, ref) <= 1e-5
 
 
+def test_mixing_tables_past_the_cache_limit(cuda):
+    """The mixing-factor tables (one per device, length, fs, t0 and fmod) are
+    kept up to 64 per device, the least recently used released first: 70
+    initial times in a row, then the first again, each equal to the oracle."""
+    fs, fmod = 41.6e6, 5.2e6
+    rf = _rf(3, (1, 1, 2, 300), fs, fmod)
+    bd = matlab_lowpass(0.5 * fmod, fmod, fs)
+    Nd = int(math.floor(fs / (2 * fmod)))
+    x = torch.from_numpy(rf).to(cuda)
+    for k in list(range(70)) + [0, 1]:
+        t0 = 1e-6 + k * 1e-8
+        ref = O.fast_demodulation(rf, fs, t0, fmod, None)[0]
+        got = ops.demodulate(x, fs, t0, fmod, bd, Nd, L.FIR_CAUSAL).cpu().numpy()
+        assert _rel(got, ref) <= 1e-5, k
+
+
 def test_zero_phase_needs_more_samples_than_padlen(cuda):
     rf = torch.zeros((1, 1, 1, 60), device=cuda)
     with pytest.raises(L.USTBXError, match="padlen"):

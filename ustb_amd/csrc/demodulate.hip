@@ -174,22 +174,52 @@ __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
     if (jt + k < jn) o[k] = acc[k];
 }
 
+struct MixTable {
+  float2* tab = nullptr;
+  unsigned long long last_use = 0;
+};
+
 std::mutex g_mu;
-std::map<std::tuple<int, int, double, double, double>, float2*> g_mix;
+std::map<std::tuple<int, int, double, double, double>, MixTable> g_mix;
+unsigned long long g_mix_clock = 0;
+// tables kept per device; the least recently used one is released (after the
+// device has finished its queued work) to make room for a new one
+constexpr int kMaxMixTables = 64;
+
+hipError_t evict_mix(int dev) {
+  for (;;) {
+    int n = 0;
+    auto lru = g_mix.end();
+    for (auto it = g_mix.begin(); it != g_mix.end(); ++it) {
+      if (std::get<0>(it->first) != dev) continue;
+      ++n;
+      if (lru == g_mix.end() || it->second.last_use < lru->second.last_use) lru = it;
+    }
+    if (n < kMaxMixTables) return hipSuccess;
+    const hipError_t e = hipDeviceSynchronize();  // no queued work still reads it
+    if (e != hipSuccess) return e;
+    (void)hipFree(lru->second.tab);
+    g_mix.erase(lru);
+  }
+}
 
 // The mixing-factor table of (device, L, fs, t0, fmod), built on first use on
 // the caller's stream (and waited for, so that other streams may read it).
+// The caller holds g_mu until it has launched the kernel that reads the
+// table, so that no other call releases the table before that launch.
 int mix_table(int L, double fs, double t0, double fmod, hipStream_t s, const float2** out, ustbx_status* st) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_fail(st, e, "hipGetDevice");
-  std::lock_guard<std::mutex> lock(g_mu);
   const auto key = std::make_tuple(dev, L, fs, t0, fmod);
   auto it = g_mix.find(key);
   if (it != g_mix.end()) {
-    *out = it->second;
+    it->second.last_use = ++g_mix_clock;
+    *out = it->second.tab;
     return ok(st);
   }
+  e = evict_mix(dev);
+  if (e != hipSuccess) return hip_fail(st, e, "releasing a mixing-factor table");
   float2* tab = nullptr;
   e = hipMalloc(&tab, sizeof(float2) * (size_t)L);
   if (e != hipSuccess) return hip_fail(st, e, "hipMalloc (mixing factors)");
@@ -200,7 +230,7 @@ int mix_table(int L, double fs, double t0, double fmod, hipStream_t s, const flo
     (void)hipFree(tab);
     return hip_fail(st, e, "mixing factors");
   }
-  g_mix.emplace(key, tab);
+  g_mix.emplace(key, MixTable{tab, ++g_mix_clock});
   *out = tab;
   return ok(st);
 }
@@ -250,6 +280,7 @@ int ustbx_demodulate(const float* rf, void* iq, int32_t L, int64_t traces, doubl
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const float2* mix = nullptr;
+  std::lock_guard<std::mutex> lock(g_mu);  // held through the launch that reads the table
   const int rc = mix_table(L, fs, t0, fmod, s, &mix, st);
   if (rc != USTBX_OK) return rc;
   A.rf = rf;

@@ -27,12 +27,43 @@ namespace {
 struct FftPlans {
   hipfftHandle r2c = 0, c2c = 0;
   float2* spec = nullptr;  // [T][L/2+1]
+  unsigned long long last_use = 0;
 };
 
 std::mutex g_mu;
 // per (device, L, traces, stream): a plan's work areas and spectrum buffer are
-// never shared by calls that may run at the same time on different streams
+// never shared by calls that may run at the same time on different streams.
+// At most kMaxFftPlans per device are kept (each holds traces x (L/2+1)
+// complex values plus hipFFT's work areas); the least recently used one is
+// released, after the device has finished its queued work, to make room.
+constexpr int kMaxFftPlans = 8;
 std::map<std::tuple<int, int, long long, hipStream_t>, FftPlans> g_plans;
+unsigned long long g_plan_clock = 0;
+
+void release(FftPlans& p) {
+  if (p.r2c) (void)hipfftDestroy(p.r2c);
+  if (p.c2c) (void)hipfftDestroy(p.c2c);
+  if (p.spec) (void)hipFree(p.spec);
+  p = FftPlans{};
+}
+
+// Makes room for one more plan on device dev (the current device).
+hipError_t evict_plans(int dev) {
+  for (;;) {
+    int n = 0;
+    auto lru = g_plans.end();
+    for (auto it = g_plans.begin(); it != g_plans.end(); ++it) {
+      if (std::get<0>(it->first) != dev) continue;
+      ++n;
+      if (lru == g_plans.end() || it->second.last_use < lru->second.last_use) lru = it;
+    }
+    if (n < kMaxFftPlans) return hipSuccess;
+    const hipError_t e = hipDeviceSynchronize();  // no queued work still uses it
+    if (e != hipSuccess) return e;
+    release(lru->second);
+    g_plans.erase(lru);
+  }
+}
 
 int fail(ustbx_status* st, int code, const char* fmt, ...) {
   if (st) {
@@ -531,17 +562,26 @@ extern "C" int ustbx_hilbert(const float* rf, void* analytic, int32_t L, int64_t
   auto key = std::make_tuple(dev, (int)L, (long long)traces, s);
   auto it = g_plans.find(key);
   if (it == g_plans.end()) {
+    e = evict_plans(dev);
+    if (e != hipSuccess) return fail(st, USTBX_E_GPU, "HIP error: %s", hipGetErrorString(e));
     FftPlans p;
     int n[1] = {L};
     if (hipfftPlanMany(&p.r2c, 1, n, nullptr, 1, L, nullptr, 1, L / 2 + 1, HIPFFT_R2C, (int)traces) !=
             HIPFFT_SUCCESS ||
-        hipfftPlanMany(&p.c2c, 1, n, nullptr, 1, L, nullptr, 1, L, HIPFFT_C2C, (int)traces) != HIPFFT_SUCCESS)
+        hipfftPlanMany(&p.c2c, 1, n, nullptr, 1, L, nullptr, 1, L, HIPFFT_C2C, (int)traces) != HIPFFT_SUCCESS) {
+      release(p);
       return fail(st, USTBX_E_GPU, "hipfftPlanMany failed (L=%d, traces=%lld)", L, (long long)traces);
+    }
     e = hipMalloc(&p.spec, sizeof(float2) * (size_t)traces * (L / 2 + 1));
-    if (e != hipSuccess) return fail(st, USTBX_E_GPU, "HIP error: %s", hipGetErrorString(e));
+    if (e != hipSuccess) {
+      p.spec = nullptr;
+      release(p);
+      return fail(st, USTBX_E_GPU, "HIP error: %s", hipGetErrorString(e));
+    }
     it = g_plans.emplace(key, p).first;
   }
   FftPlans& p = it->second;
+  p.last_use = ++g_plan_clock;
   if (hipfftSetStream(p.r2c, s) != HIPFFT_SUCCESS || hipfftSetStream(p.c2c, s) != HIPFFT_SUCCESS)
     return fail(st, USTBX_E_GPU, "hipfftSetStream failed");
   if (hipfftExecR2C(p.r2c, const_cast<float*>(rf), reinterpret_cast<hipfftComplex*>(p.spec)) != HIPFFT_SUCCESS)
