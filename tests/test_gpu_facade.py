@@ -105,6 +105,25 @@ def test_go_iq_matches_oracle(cuda, name, dim):
     assert d.elapsed_time is not None and d.elapsed_time > 0
 
 
+def test_go_keeps_the_last_plans_only(cuda):
+    """go() over MAX_PLANS + 2 scans keeps the last MAX_PLANS plans (the
+    oldest are released) and re-running an evicted configuration builds it
+    again with the same image."""
+    from ustb_amd.uff import LinearScan
+
+    case = CASES["cpwc_tukey"]
+    ch = channel_noise(case, F=1, seed=9)
+    d = _das(case, np.ascontiguousarray(ch.transpose(3, 2, 1, 0)))
+    imgs = []
+    for k in range(DAS.MAX_PLANS + 2):
+        d.scan = LinearScan(np.linspace(-3e-3, 3e-3, 16), np.linspace(8e-3 + 1e-4 * k, 14e-3, 24))
+        imgs.append(d.go().data.cpu().numpy())
+        assert len(d._plans) == min(k + 1, DAS.MAX_PLANS)
+    d.scan = LinearScan(np.linspace(-3e-3, 3e-3, 16), np.linspace(8e-3, 14e-3, 24))  # evicted first
+    assert np.array_equal(d.go().data.cpu().numpy(), imgs[0])
+    assert len(d._plans) == DAS.MAX_PLANS
+
+
 def test_go_rf_goes_through_hilbert(cuda):
     case = CASES["pw_rf_hanning"]
     rng = np.random.default_rng(17)

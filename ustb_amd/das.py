@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import hashlib
 import time
+from collections import OrderedDict
 
 import numpy as np
 
@@ -24,7 +25,13 @@ from .uff import Apodization, BeamformedData, ChannelData, Scan
 
 
 class DAS:
-    """Generalized delay-and-sum beamformer (das.m:1-47; das.py:15-41)."""
+    """Generalized delay-and-sum beamformer (das.m:1-47; das.py:15-41).
+
+    go() keeps the device plans of its last MAX_PLANS configurations (probe,
+    scan, sequence, apodization, sampling, mode, GPU) and releases the least
+    recently used one beyond that."""
+
+    MAX_PLANS = 4
 
     def __init__(self):
         self.channel_data: ChannelData | None = None
@@ -42,7 +49,7 @@ class DAS:
         self.gpu_id = 0
         self.beamformed_data = None
         self.elapsed_time = None
-        self._plans: dict = {}
+        self._plans: OrderedDict = OrderedDict()  # most recently used last
 
     # ------------------------------------------------------------------
     def _plan_key(self, n_samples, n_frames):
@@ -55,7 +62,7 @@ class DAS:
         h.update(repr((n_samples, n_frames, cd.sampling_frequency, cd.initial_time, cd.modulation_frequency,
                        cd.sound_speed, int(self.dimension), int(self.interpolation), self.lens_delay,
                        self.pw_margin, self.blending_power, int(self.spherical_transmit_delay_model),
-                       self.phase, type(sc).__name__)).encode())
+                       self.phase, type(sc).__name__, int(self.gpu_id))).encode())
         return h.hexdigest()
 
     def _device_channel_data(self, torch, dev):
@@ -93,7 +100,13 @@ class DAS:
         F, W, N, Ls = ch.shape
         key = self._plan_key(Ls, F)
         plan = self._plans.get(key)
-        if plan is None:
+        if plan is not None:
+            self._plans.move_to_end(key)
+        else:
+            # keep at most MAX_PLANS configurations' device tables; go()
+            # synchronizes after each run, so a released plan has no work queued
+            while len(self._plans) >= self.MAX_PLANS:
+                self._plans.popitem(last=False)[1].close()
             with torch.cuda.device(dev):
                 plan = make_plan(cd.probe, self.scan, cd.sequence, n_samples=Ls, n_frames=F,
                                  sampling_frequency=cd.sampling_frequency, initial_time=cd.initial_time,
