@@ -114,7 +114,8 @@ int ustbx_das_run(const ustbx_das_desc* d, const void* ch, const float* tx_apo,
 
 /* Same contract on host buffers (what a MEX gateway holds): stages to the
  * device, runs and copies the result back; synchronous. `device` selects the
- * GPU like das_cuda's gpu_id (das_cuda.cu:161-162). */
+ * GPU like das_cuda's gpu_id (das_cuda.cu:161-162); the calling thread's
+ * current device is left as it was. */
 int ustbx_das_run_host(const ustbx_das_desc* d, const void* ch,
                        const float* tx_apo, const float* rx_apo,
                        const float* tx_delay, const float* rx_delay, void* out,

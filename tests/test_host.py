@@ -223,3 +223,17 @@ def test_plan_create_rejects_oversized_and_empty_geometry():
         assert rc == L.E_DIMENSIONS, kw
     rc, _ = create(sc=LinearScan(np.zeros(0), np.zeros(4)))
     assert rc == L.E_DIMENSIONS
+
+
+def test_host_entry_rejects_null_inputs_before_device_work():
+    """ustbx_das_run_host checks its host arrays before it stages anything:
+    a NULL input with a nonzero size is E_ARG, here even without a GPU."""
+    import ctypes
+
+    d = L.DasDesc(L=16, N=2, W=1, F=1, P=4, fs=1.0, t0=0.0, fd=0.0, dim=3, interp=L.INTERP_LINEAR,
+                  edge=L.EDGE_DASC, flags=0, split=0)
+    out = np.zeros(4, np.complex64)
+    st = L.Status()
+    rc = L.lib().ustbx_das_run_host(ctypes.byref(d), None, None, None, None, None, out.ctypes.data, 0,
+                                    ctypes.byref(st))
+    assert rc == L.E_ARG and b"NULL input" in st.message

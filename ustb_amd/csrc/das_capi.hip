@@ -299,15 +299,25 @@ int ustbx_das_run_host(const ustbx_das_desc* d, const void* ch, const float* tx_
   const size_t P = d->P, N = d->N, W = d->W, F = d->F, L = d->L;
   const size_t ne = out_elems(d->dim, P, N, W, F);
   if (empty_problem(d)) return ok(st);
+  const size_t bytes_ch = L * N * W * F * 8, bytes_tx = P * W * 4, bytes_rx = P * N * 4;
+  const size_t bytes_out = ne * 8;
+  if ((bytes_ch && !ch) || (bytes_tx && (!tx_apo || !tx_delay)) || (bytes_rx && (!rx_apo || !rx_delay)))
+    return set_status(st, USTBX_E_ARG, "NULL input array");
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev == 0) return set_status(st, USTBX_E_GPU, "no HIP device available");
   if (device < 0 || device >= ndev)
     return set_status(st, USTBX_E_GPU, "invalid gpu id %d (%d devices)", device, ndev);
+  // the caller's current device is restored on return
+  int prev = 0;
+  e = hipGetDevice(&prev);
+  if (e != hipSuccess) return hip_fail(st, e, "hipGetDevice");
   e = hipSetDevice(device);
   if (e != hipSuccess) return hip_fail(st, e, "hipSetDevice");
-  const size_t bytes_ch = L * N * W * F * 8, bytes_tx = P * W * 4, bytes_rx = P * N * 4;
-  const size_t bytes_out = ne * 8;
+  struct Restore {
+    int dev;
+    ~Restore() { (void)hipSetDevice(dev); }
+  } restore{prev};
   char* buf = nullptr;
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   const size_t total = al(bytes_ch) + 2 * al(bytes_tx) + 2 * al(bytes_rx) + al(bytes_out);
