@@ -49,6 +49,11 @@ def parse():
     ap.add_argument("--slice-ms", type=float, default=2.0, help="C5: estimated work per serving iteration")
     ap.add_argument("--long-slices", type=int, default=8,
                     help="C5: iteration length, in slices, while no small frame is pending")
+    ap.add_argument("--depth", type=int, default=2,
+                    help="C5: serving iterations queued on the GPU at once")
+    ap.add_argument("--idle", default="1,4,1",
+                    help="C5: slice ms, long slices and depth while the arrived work not yet submitted is "
+                         "below 250 ms ('none': the same as under a backlog)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--phase", default="auto", choices=["auto", "separable", "dasc"],
@@ -833,6 +838,14 @@ def run_batch(args, torch, dev):
     print(json.dumps(result))
 
 
+def idle_settings(spec: str) -> dict:
+    """--idle 'slice_ms,long_slices,depth' (or 'none') as Server keywords."""
+    if spec == "none":
+        return dict(idle_slice_ms=None, idle_long_slices=None, idle_depth=None)
+    sl, lg, dp = spec.split(",")
+    return dict(idle_slice_ms=float(sl), idle_long_slices=int(lg), idle_depth=int(dp))
+
+
 def run_serve(args, torch, dev):
     """C5: the continuous-batching server against the plain loop (one request
     at a time, one launch per frame) on the same Poisson trace."""
@@ -846,7 +859,7 @@ def run_serve(args, torch, dev):
     torch.cuda.synchronize()
     log(f"C5: {len(cfgs)} configurations, data and plans in {time.time() - t:.1f}s")
     srv = Server(plans, datasets, hbm_budget=64 << 30, slice_ms=args.slice_ms, tiles=(8, 4),
-                 long_slices=args.long_slices)
+                 long_slices=args.long_slices, depth=args.depth, **idle_settings(args.idle))
     log(f"calibrated ms per position {[round(x, 4) for x in srv.ms_per_pos]}, positions {srv.positions}")
     names = [c.name for c in cfgs]
     trace = poisson_trace(args.frames, rate=args.rate, n_configs=len(cfgs), n_data=len(datasets[0]), seed=5)
@@ -888,6 +901,7 @@ def run_serve(args, torch, dev):
         "data": "synthetic",
         "config": {"workload": f"C5 serving, Poisson {args.rate:g} frames/s, first {len(trace)} requests of the "
                                f"trace, uniform over {names}", "hbm_budget_GiB": 64, "slice_ms": args.slice_ms, "long_slices": args.long_slices,
+                   "depth": args.depth, "idle_slice_long_depth": args.idle,
                    "waves_per_split": srv.tiles[0], "splits": srv.tiles[1],
                    "units_per_frame": {n: c.units for n, c in zip(names, cfgs)}},
         "serving": res.summary(names),

@@ -48,6 +48,8 @@ def test_served_images_equal_alone(cuda, budget_frames, slice_ms):
     if budget_frames is not None:
         big = max(int(np.prod(p.out_shape)) * 8 + d[0].nbytes for p, d in zip(plans, datas))
         kw["hbm_budget"] = int(budget_frames * big)
+    if slice_ms < 1.0:  # tiny iterations whether or not a backlog waits
+        kw["idle_slice_ms"] = slice_ms
     srv = Server(plans, datas, slice_ms=slice_ms, **kw)
     trace = poisson_trace(60, rate=20000.0, n_configs=len(plans), n_data=2, seed=11)
     res = srv.run(trace, keep_outputs=True)

@@ -130,7 +130,16 @@ class Server:
     def __init__(self, plans: list[GeometryPlan], datasets: list[list[np.ndarray]], *,
                  hbm_budget: int = 64 << 30, slice_ms: float = 2.0, depth: int = 2,
                  tiles: tuple[int, int] = (8, 4), max_acqs: int = 4096, copy_window: int = 512 << 20,
-                 long_slices: int = 8):
+                 long_slices: int = 8, idle_slice_ms: float | None = 1.0, idle_long_slices: int | None = 4,
+                 idle_depth: int | None = 1, idle_below_ms: float = 250.0):
+        """slice_ms / long_slices / depth: iteration length (short while a
+        small frame is pending, long_slices times that otherwise) and the
+        number of iterations queued on the GPU under a backlog (throughput
+        first). The idle_* values apply while the work not yet submitted
+        (arrived requests' remaining positions at their calibrated cost) is
+        below idle_below_ms (latency first: shorter iterations and one queued
+        iteration, so a small frame that arrives waits behind less work).
+        None keeps the backlog value."""
         if not plans:
             raise ValueError("no configurations")
         interp = plans[0].interp
@@ -146,6 +155,13 @@ class Server:
         self.long_slices = int(long_slices)
         self.slice_ms = float(slice_ms)
         self.depth = int(depth)
+        self.idle_slice_ms = float(slice_ms if idle_slice_ms is None else idle_slice_ms)
+        self.idle_long_slices = int(long_slices if idle_long_slices is None else idle_long_slices)
+        self.idle_depth = int(depth if idle_depth is None else idle_depth)
+        self.idle_below_ms = float(idle_below_ms)
+        if min(self.depth, self.idle_depth, self.long_slices, self.idle_long_slices) < 1 or \
+                min(self.slice_ms, self.idle_slice_ms) < 0:
+            raise ValueError("depth and long slices must be positive, slices not negative")
         self.positions = [int(L.lib().ustbx_plan_positions(p.handle)) for p in plans]
         for k, n in enumerate(self.positions):
             if n < 0:
@@ -206,6 +222,7 @@ class Server:
         iters = 0
         t0 = time.perf_counter()
         last_done = 0.0
+        backlog_ms = 0.0  # arrived work not yet submitted, at the calibrated cost
         with torch.cuda.device(dev):
             while completed < n:
                 now = time.perf_counter() - t0
@@ -213,6 +230,7 @@ class Server:
                 while nxt < n and trace[nxt].arrival <= now:
                     r = trace[nxt]
                     heapq.heappush(waiting, (self.in_bytes[r.config], r.arrival, r.rid, r))
+                    backlog_ms += self.positions[r.config] * self.ms_per_pos[r.config]
                     nxt += 1
                 # admission: smallest input first, under the HBM budget, with
                 # at most copy_window bytes and one large frame's input queued
@@ -247,8 +265,10 @@ class Server:
                     copying_large -= 0 if self._small(lv.req.config) else 1
                     ready.append(lv)
                     progressed = True
-                # an iteration over the ready requests
-                if ready and len(running) < self.depth:
+                # an iteration over the ready requests; the settings follow the
+                # load: the arrived work not yet submitted
+                idle = backlog_ms < self.idle_below_ms
+                if ready and len(running) < (self.idle_depth if idle else self.depth):
                     plans, chs, outs, begin, end, finishing = [], [], [], [], [], []
                     # a short iteration while small requests are pending (their
                     # latency), a long one when only large frames are (fewer,
@@ -256,7 +276,8 @@ class Server:
                     small = any(self._small(lv.req.config) for lv in ready) or \
                         any(self._small(lv.req.config) for lv in staged) or \
                         (bool(waiting) and self._small(waiting[0][3].config))
-                    budget = self.slice_ms if small else self.slice_ms * self.long_slices
+                    sl, lg = (self.idle_slice_ms, self.idle_long_slices) if idle else (self.slice_ms, self.long_slices)
+                    budget = sl if small else sl * lg
                     items = 0
                     # shortest remaining work first (then arrival): small
                     # frames pass large ones, and the large frame nearest
@@ -282,6 +303,7 @@ class Server:
                         end.append(lv.next_pos + take)
                         lv.next_pos += take
                         budget -= self.ms_per_pos[k] * take
+                        backlog_ms -= self.ms_per_pos[k] * take
                         if lv.next_pos >= self.positions[k]:
                             finishing.append(lv)
                     self.batcher.submit(plans, chs, outs, begin, end, self.s_run)
