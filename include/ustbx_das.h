@@ -14,8 +14,8 @@
  *
  * Every entry point returns 0 on success or a negative USTBX_E* code; when
  * `st` is not NULL it also receives the code and a message. Nothing longjmps
- * and nothing aborts: a MEX gateway maps the codes to the reference's
- * "Toolbox:SRP_SRC:*" error identifiers (ustbx_status_id()).
+ * and nothing aborts: a MEX gateway maps the codes to das_c's error
+ * identifiers (ustbx_status_id()).
  */
 #ifndef USTBX_DAS_H
 #define USTBX_DAS_H
@@ -29,18 +29,22 @@ extern "C" {
 
 #define USTBX_ABI_VERSION 1
 
-/* error codes (mapped to the reference's mexErrMsgIdAndTxt identifiers,
- * +mex/source/das_c.cpp:53-252, +mex/source/das_cuda.cu:117-124) */
+/* error codes, with the identifier das_c raises each class with
+ * (ustbx_status_id; +mex/source/das_c.cpp:53-252, +mex/source/das_cuda.cu:122).
+ * "" = das_c raises it through mexErrMsgTxt, with no identifier. */
 #define USTBX_OK 0
-#define USTBX_E_NRHS (-1)       /* Toolbox:SRP_SRC:nrhs        */
-#define USTBX_E_NLHS (-2)       /* Toolbox:SRP_SRC:nlhs        */
-#define USTBX_E_DIMENSIONS (-3) /* Toolbox:SRP_SRC:Dimensions  */
-#define USTBX_E_FLOAT (-4)      /* Toolbox:SRP_SRC:Float       */
-#define USTBX_E_SCALAR (-5)     /* Toolbox:SRP_SRC:Scalar      */
-#define USTBX_E_GPU (-6)        /* Toolbox:SRP_SRC:GPU         */
-#define USTBX_E_ARG (-7)        /* invalid argument (sum mode, interp, NULL) */
-#define USTBX_E_CAPACITY (-8)   /* request exceeds a stated capacity */
-#define USTBX_E_UNSUPPORTED (-9)/* feature not available on the device path */
+#define USTBX_E_NRHS (-1)       /* Toolbox:SRP_SRC:nrhs        (das_c.cpp:53-54) */
+#define USTBX_E_NLHS (-2)       /* Toolbox:SRP_SRC:nlhs        (das_c.cpp:55) */
+#define USTBX_E_DIMENSIONS (-3) /* Toolbox:SRP_SRC:Dimensions  (das_c.cpp:88-199) */
+#define USTBX_E_FLOAT (-4)      /* "" (das_c.cpp:98, 145-236: not single precision) */
+#define USTBX_E_SCALAR (-5)     /* "" (das_c.cpp:208, 221, 234: not a scalar) */
+#define USTBX_E_GPU (-6)        /* Toolbox:SRP_SRC:GPU         (das_cuda.cu:122) */
+#define USTBX_E_ARG (-7)        /* "" (sum mode das_c.cpp:79, real data with fd
+                                   das_c.cpp:242; interp, NULL pointers) */
+#define USTBX_E_CAPACITY (-8)   /* ustbx:capacity: a request exceeds a stated
+                                   capacity (no reference counterpart) */
+#define USTBX_E_UNSUPPORTED (-9)/* ustbx:unsupported: feature not available on
+                                   the device path (no reference counterpart) */
 
 typedef struct ustbx_status {
   int32_t code;
@@ -69,7 +73,15 @@ typedef struct ustbx_status {
 #define USTBX_FLAG_SEPARABLE_PHASE 2u /* geometry path: exp(i wd (tx+rx)) taken as
                                          exp(i wd tx) * exp(i wd rx) */
 #define USTBX_FLAG_REF_PHASE 4u  /* geometry path: multiply by
-                                    exp(-1j*2*w0*r_ref/c) (das.m:306-309) */
+                                    exp(-1j*2*w0*r_ref/c) (das.m:306-309), the
+                                    argument formed in FP64 as python
+                                    das.py:169-171 forms it */
+#define USTBX_FLAG_REF_PHASE_SINGLE 16u /* with USTBX_FLAG_REF_PHASE: the
+   argument formed in single precision as das.m:308 evaluates it,
+   fl32(fl32(-2*w0 * single(r_ref)) / single(c)) (MATLAB's products of single
+   and double operands are single). The factor is the sine and cosine of the
+   argument rounded to FP32 in both flavours. At 55 mm and 5.2 MHz the two
+   differ by about 1e-4 (the FP32 rounding of a 2.3e3 rad argument). */
 #define USTBX_FLAG_PYTHON_ARITHMETIC 8u /* matrix ABI, with USTBX_EDGE_PYTHON and
    linear interpolation: the Python reference's DAS._beamform arithmetic
    (python das.py:217-250). `fs` holds the sample period dt [s] of the time
@@ -134,7 +146,8 @@ int ustbx_das_check_mex_args(int32_t nrhs, int32_t nlhs, int32_t ch_ndim,
                              int32_t ch_is_complex, float fd,
                              ustbx_das_desc* d, ustbx_status* st);
 
-/* Reference error identifier for a code ("Toolbox:SRP_SRC:Dimensions", ...). */
+/* Reference error identifier for a code ("Toolbox:SRP_SRC:Dimensions", ...;
+ * "" where das_c raises the error with no identifier). */
 const char* ustbx_status_id(int32_t code);
 
 /* ---------------------------------------------------------------------------
@@ -334,7 +347,9 @@ int ustbx_batcher_create(int32_t interp, int32_t max_acqs, int64_t max_items, in
 /* Uploads the submission's descriptors and work items (asynchronously, from
  * pinned staging memory; it first waits until the previous submission's
  * upload has left the staging memory) and launches the persistent kernel on
- * stream. Allocates nothing. Submissions run in stream order. */
+ * stream. Allocates nothing. Submissions run in submission order: one made on
+ * another stream than the previous one waits (hipStreamWaitEvent) for the
+ * previous submission's kernel, whose descriptor buffers it rewrites. */
 int ustbx_batcher_submit(ustbx_batcher* batcher, int32_t count, const ustbx_plan* const* plans,
                          const void* const* ch, void* const* out, const int32_t* pos_begin,
                          const int32_t* pos_end, void* stream, ustbx_status* st);

@@ -553,14 +553,29 @@ def tx_apod(px, py, pz, waves, window_kind, f_number=(1.0, 1.0), minimum_apertur
     return out
 
 
-def reference_phase(r_ref, fd, c):
+def reference_phase(r_ref, fd, c, flavour="python"):
     """exp(-1j*2*w0*r_ref/c) (das.m:306-309; das.py:164-173), complex128.
 
     w0 = 2*pi*single(fd) is a single-precision value in both references
-    (das.m:88-89; das.py:54-55 under NumPy's scalar promotion), equal to
-    das_c's wd."""
+    (das.m:88-89; das.py:54-55 under NumPy 2's scalar promotion), equal to
+    das_c's wd. The two references then differ:
+    - "python" (das.py:169-171): the complex64 factor -2j*w0 times the FP64
+      distance is complex128, so the argument is formed in FP64 and the
+      factor rounded to complex64 at the end;
+    - "matlab" (das.m:308): -1j*2*w0 is complex single, and MATLAB's products
+      and quotients of single with double operands are single, so the
+      argument is fl32(fl32(-2*w0 * single(r)) / single(c)) and exp is taken
+      in single precision (here: the exact FP64 exp of that FP32 argument,
+      rounded to complex64)."""
     w0 = dasc_wd(fd)
-    return np.exp(-1j * 2.0 * w0 * np.asarray(r_ref, float) / c)
+    r = np.asarray(r_ref, float)
+    if flavour == "python":
+        return np.exp(-1j * 2.0 * w0 * r / c)
+    if flavour != "matlab":
+        raise ValueError(f"unknown flavour {flavour!r}")
+    w32, r32, c32 = np.float32(w0), r.astype(np.float32), np.float32(c)
+    arg = np.float32(np.float32(np.float32(-2.0) * w32) * r32) / c32
+    return np.exp(1j * np.asarray(arg, np.float64)).astype(np.complex64).astype(np.complex128)
 
 
 def hilbert(x, axis=-1):

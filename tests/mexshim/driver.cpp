@@ -63,6 +63,7 @@ void mexErrMsgIdAndTxt(const char* id, const char* fmt, ...) {
   va_end(ap);
   throw MexError(id, buf);
 }
+void mexErrMsgTxt(const char* msg) { throw MexError("", msg); }
 
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]);
 extern "C" int oracle_das_c(const float* ch, int L, int N, int W, int F, int P, float Fs, float t0, float fd,
@@ -111,7 +112,7 @@ static const char* call(std::vector<mxArray*> prhs, int nlhs, mxArray** out) {
     if (out) *out = plhs[0];
     return "ok";
   } catch (const MexError& e) {
-    id = e.id;
+    id = e.id.empty() ? "(no-identifier)" : e.id;
     return id.c_str();
   }
 }

@@ -28,5 +28,6 @@ mxComplexSingle* mxGetComplexSingles(const mxArray*);
 int32_t* mxGetInt32s(const mxArray*);
 mxArray* mxCreateNumericArray(mwSize ndim, const mwSize* dims, mxClassID cls, mxComplexity cplx);
 void mexErrMsgIdAndTxt(const char* id, const char* fmt, ...);
+void mexErrMsgTxt(const char* msg);
 
 #endif

@@ -1,0 +1,451 @@
+"""Closed-form pins of the oracle's geometry branches that the reference's own
+tests leave open (CPU):
+
+- the hybrid and blended focused-transmit delay models (das.m:156-187), on
+  linear and sector scans, off axis;
+- the transmit apodization of spherical waves (apodization.m:489-552): the
+  source -> origin frame, the cone, the grating-lobe mask and the fallback
+  axis when the source sits on the origin;
+- the receive aperture's tilt (tools.rotate_points), the sector-scan origin
+  (apodization.m:394-409) and the minimum / maximum aperture clamps
+  (apodization.m:446-462);
+- the reference phase exp(-1j*2*w0*r/c) (das.m:306-309) in both reference
+  flavours.
+
+Every expected value comes from a geometric statement that does not retrace
+the reference's operation sequence: angles measured from a rotated axis, the
+perpendicular and axial distances of a pixel from the source axis, an aperture
+of known width, the phase at a quarter wavelength. Each check also runs on a
+copy of the oracle function with one planted mistake (a sign, an operator, a
+dropped factor, a swapped operand), which it must reject.
+"""
+import inspect
+import math
+import textwrap
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+C = 1540.0
+
+
+def mutant(fn, old, new):
+    """fn recompiled from its source with `old` replaced by `new` (once)."""
+    src = textwrap.dedent(inspect.getsource(fn))
+    assert src.count(old) == 1, (fn.__name__, old)
+    ns = dict(vars(O))
+    exec(compile(src.replace(old, new), f"<mutant of {fn.__name__}>", "exec"), ns)
+    return ns[fn.__name__]
+
+
+def rejects(check, fn):
+    try:
+        check(fn)
+    except AssertionError:
+        return True
+    return False
+
+
+def focused(sx, sz, sy=0.0, origin=(0.0, 0.0, 0.0)):
+    d = math.sqrt(sx * sx + sy * sy + sz * sz)
+    return dict(wavefront=O.WAVE_SPHERICAL, src=(sx, sy, sz), distance=d, azimuth=math.atan2(sx, sz),
+                elevation=math.asin(sy / d) if d else 0.0, sound_speed=C, delay=0.0, origin=origin)
+
+
+def grid(xs, zs):
+    X, Z = np.meshgrid(np.asarray(xs, float), np.asarray(zs, float))
+    return X.ravel(), np.zeros(X.size), Z.ravel()
+
+
+def hanning(r):
+    r = np.asarray(r, float)
+    return np.where(r <= 0.5, 0.5 + 0.5 * np.cos(2 * np.pi * r), 0.0)
+
+
+# ---------------------------------------------------------------------------
+# hybrid model (das.m:156-174)
+# ---------------------------------------------------------------------------
+SX, SZ, MARGIN = 4e-3, 20e-3, 1e-3
+# depths strictly inside the +-margin band, on its two edges, and outside
+Z_IN = [SZ - 0.6e-3, SZ, SZ + 0.4e-3]
+Z_EDGE = [SZ - MARGIN, SZ + MARGIN]
+Z_OUT = [10e-3, 25e-3]
+X_OFF = [-6e-3, 0.0, 3e-3, 4e-3, 9e-3]
+
+
+def check_hybrid_linear(tx_delay):
+    """Linear scan: within the band the delay is the plane wave through the
+    focus along z, (z - s_z) + |s| in distance units, at every lateral
+    position; on the band's edges and outside it the conventional virtual
+    source delay, -(1)^(z<s_z) |s - x| + |s|."""
+    wv = focused(SX, SZ)
+    dist = wv["distance"]
+    for zs, inside in ((Z_IN, True), (Z_EDGE, False), (Z_OUT, False)):
+        px, py, pz = grid(X_OFF, zs)
+        got = tx_delay(px, py, pz, [wv], "linear", O.MODEL_HYBRID, pw_margin=MARGIN)[0].astype(float) * C
+        if inside:
+            ref = (pz - SZ) + dist
+        else:
+            ref = np.where(pz < SZ, -1.0, 1.0) * np.hypot(px - SX, pz - SZ) + dist
+        np.testing.assert_allclose(got, ref, rtol=2e-7, atol=1e-12)
+
+
+def check_hybrid_sector(tx_delay):
+    """Sector scan: within the band the delay is the plane wave along the
+    source direction, the projection of x on (sin az, 0, cos az) (no distance
+    term); outside it the virtual source delay."""
+    az = 0.25
+    wv = focused(30e-3 * math.sin(az), 30e-3 * math.cos(az))
+    sz, dist = wv["src"][2], wv["distance"]
+    for zs, inside in (([sz - 0.5e-3, sz + 0.3e-3], True), ([15e-3, 40e-3], False)):
+        px, py, pz = grid([-5e-3, 0.0, 7.4e-3, 12e-3], zs)
+        got = tx_delay(px, py, pz, [wv], "sector", O.MODEL_HYBRID, pw_margin=MARGIN)[0].astype(float) * C
+        if inside:
+            ref = px * math.sin(az) + pz * math.cos(az)
+        else:
+            ref = np.where(pz < sz, -1.0, 1.0) * np.hypot(px - wv["src"][0], pz - sz) + dist
+        np.testing.assert_allclose(got, ref, rtol=2e-7, atol=1e-12)
+
+
+def test_hybrid_linear_off_axis():
+    check_hybrid_linear(O.tx_delay)
+
+
+def test_hybrid_sector_off_axis():
+    check_hybrid_sector(O.tx_delay)
+
+
+@pytest.mark.parametrize("check,old,new", [
+    (check_hybrid_linear, "(pz < (sz + pw_margin))", "(pz <= (sz + pw_margin))"),   # band edge included
+    (check_hybrid_linear, "pd = sgn * np.sqrt(dz * dz) + dist", "pd = np.sqrt(dz * dz) + dist"),  # sign dropped
+    (check_hybrid_linear, "pd = sgn * np.sqrt(dz * dz) + dist", "pd = sgn * np.sqrt(dx * dx + dz * dz) + dist"),
+    (check_hybrid_linear, "v = np.where(mask, pd, d + dist)", "v = np.where(mask, pd, d - dist)"),
+    (check_hybrid_sector, 'pd = plane_delay(px, py, pz, wv["azimuth"], wv["elevation"])',
+     'pd = plane_delay(px, py, pz, wv["azimuth"], wv["elevation"]) + dist'),
+    (check_hybrid_sector, 'pd = plane_delay(px, py, pz, wv["azimuth"], wv["elevation"])',
+     'pd = plane_delay(px, py, pz, -wv["azimuth"], wv["elevation"])'),
+])
+def test_hybrid_pins_reject_planted_mistakes(check, old, new):
+    assert rejects(check, mutant(O.tx_delay, old, new))
+
+
+# ---------------------------------------------------------------------------
+# blended model (das.m:175-187)
+# ---------------------------------------------------------------------------
+ANGLES = np.array([-0.5, -0.2, 0.45, 0.8])
+
+
+def check_blended(tx_delay):
+    """Linear scan, source (3, 0, 20) mm. With n = min(| |s| - |x| | / |s|, 1):
+    on the sphere |x| = |s| (n = 0) the delay is the plane delay
+    (z - s_z) + |s|; from |x| = 2|s| outwards (n = 1) the virtual source
+    delay; at |x| = 1.5 |s| and 0.25 |s| (n = 1/2 and 3/4) the fraction of the
+    way from the plane to the spherical delay is n to the blending power."""
+    wv = focused(3e-3, 20e-3)
+    s, dist = np.asarray(wv["src"]), wv["distance"]
+    for power in (0.5, 2.0):
+        for radius, n in ((1.0, 0.0), (2.0, 1.0), (2.5, 1.0), (1.5, 0.5), (0.25, 0.75)):
+            r = radius * dist
+            px, py, pz = r * np.sin(ANGLES), np.zeros(ANGLES.size), r * np.cos(ANGLES)
+            got = tx_delay(px, py, pz, [wv], "linear", O.MODEL_BLENDED, blending_power=power)[0].astype(float) * C
+            sph = np.where(pz < s[2], -1.0, 1.0) * np.hypot(px - s[0], pz - s[2]) + dist
+            plane = (pz - s[2]) + dist
+            f = n ** power
+            np.testing.assert_allclose(got, plane + f * (sph - plane), rtol=3e-7, atol=2e-11)
+            if n == 0.5:  # the two models differ by mm here, so the fraction is sharp
+                assert np.all(np.abs(sph - plane) > 1e-3)
+
+
+def test_blended_model():
+    check_blended(O.tx_delay)
+
+
+@pytest.mark.parametrize("old,new", [
+    ("nd = np.minimum(np.abs(dist - rr) / dist, 1.0)", "nd = np.minimum(np.abs(dist - rr), 1.0)"),
+    ("nd = np.minimum(np.abs(dist - rr) / dist, 1.0)", "nd = np.maximum(np.abs(dist - rr) / dist, 1.0)"),
+    ("npw = nd ** blending_power", "npw = nd * blending_power"),
+    ("v = (d + dist) * npw + pd * (1.0 - npw)", "v = (d + dist) * (1.0 - npw) + pd * npw"),
+])
+def test_blended_pins_reject_planted_mistakes(old, new):
+    assert rejects(check_blended, mutant(O.tx_delay, old, new))
+
+
+# ---------------------------------------------------------------------------
+# transmit apodization of spherical waves (apodization.m:489-552)
+# ---------------------------------------------------------------------------
+def axis_frame(src, origin):
+    """Unit vector of the source axis (origin -> source; +z when they coincide)."""
+    a = np.asarray(src, float) - np.asarray(origin, float)
+    n = np.linalg.norm(a)
+    return np.array([0.0, 0.0, 1.0]) if n <= np.finfo(float).eps else a / n
+
+
+def cone_ratio(px, py, pz, src, origin, f_number):
+    """F * (distance of the pixel from the source axis) / (its distance from
+    the source along the axis), in the azimuth plane (py = 0), scaled by
+    |cos theta|, theta the axis' angle from +z: apodization.m:520-521 takes
+    the lateral axes as cross(zu, [1 0 0]) and cross(zu, yu) without
+    normalising them, and in the azimuth plane both have length |zu_z|."""
+    u = axis_frame(src, origin)
+    v = np.stack([px - src[0], py - src[1], pz - src[2]], axis=1)
+    axial = v @ u
+    perp = np.linalg.norm(v - axial[:, None] * u[None, :], axis=1)
+    return f_number * abs(u[2]) * perp / np.abs(axial), axial
+
+
+def check_tx_apod_cone(tx_apod):
+    """Focused wave (source 8 mm off axis at 25 mm), Hanning, F# 1.7: a
+    pixel's weight is the window of F * perp / |axial| measured from the
+    source axis, 1 on the axis itself, 0 outside the cone
+    |perp| <= |axial| / (2F)."""
+    F = 1.7
+    wv = focused(8e-3, 25e-3)
+    src = np.asarray(wv["src"])
+    u = axis_frame(src, wv["origin"])
+    # pixels on the axis, before and beyond the focus (|axial| > the minimum aperture's reach)
+    t = np.array([5e-3, 12e-3, 40e-3, 55e-3])
+    on = t[:, None] * u[None, :]
+    a = tx_apod(on[:, 0], on[:, 1], on[:, 2], [wv], O.WIN_HANNING, (F, F))[0]
+    np.testing.assert_allclose(a, 1.0, atol=1e-7)
+    px, py, pz = grid(np.linspace(-10e-3, 20e-3, 31), [8e-3, 14e-3, 34e-3, 45e-3])
+    r, axial = cone_ratio(px, py, pz, src, wv["origin"], F)
+    assert np.all(np.abs(axial) > 1.7e-3) and np.any(r > 0.5) and np.any((r > 0.1) & (r < 0.5))
+    a = tx_apod(px, py, pz, [wv], O.WIN_HANNING, (F, F))[0]
+    np.testing.assert_allclose(a, hanning(r), atol=2e-7)
+    b = tx_apod(px, py, pz, [wv], O.WIN_BOXCAR, (F, F))[0]
+    np.testing.assert_array_equal(b != 0, r <= 0.5)
+
+
+def check_tx_apod_grating(tx_apod):
+    """Grating-lobe angle 0.6 rad: pixels whose azimuth seen from the wave's
+    origin differs from the source's by more than 0.3 rad get weight 0 (off
+    the axis); the others keep the cone's window."""
+    F, gla = 0.5, 0.6
+    wv = focused(5e-3, 25e-3)
+    src = np.asarray(wv["src"])
+    px, py, pz = grid(np.linspace(-20e-3, 25e-3, 46), [6e-3, 10e-3, 20e-3, 30e-3])
+    r, _ = cone_ratio(px, py, pz, src, wv["origin"], F)
+    dev = np.abs(np.arctan2(px, pz) - math.atan2(src[0], src[2]))
+    off = dev > 0.5 * gla
+    # pixels the mask zeroes that the cone alone keeps, some of them within
+    # the full grating-lobe angle (a mask at gla instead of gla / 2 keeps them)
+    assert (off & (r < 0.45) & (dev < gla)).sum() > 5 and (~off & (r < 0.5)).sum() > 5
+    a = tx_apod(px, py, pz, [wv], O.WIN_HANNING, (F, F), grating_lobe_angle=(gla, gla))[0]
+    np.testing.assert_array_equal(a[off], 0.0)
+    np.testing.assert_allclose(a[~off], hanning(r[~off]), atol=2e-7)
+
+
+def check_tx_apod_source_at_origin(tx_apod):
+    """Source on the wave's origin (an STA transmit from element x = 2.1 mm):
+    the axis falls back to +z through the source, so the weight is the window
+    of F |x - s_x| / |z - s_z|."""
+    F = 1.5
+    s = (2.1e-3, 0.0, 0.0)
+    wv = dict(focused(*s[::2]), origin=s)
+    wv["distance"] = 0.0
+    px, py, pz = grid(np.linspace(-8e-3, 12e-3, 41), [6e-3, 15e-3, 30e-3])
+    a = tx_apod(px, py, pz, [wv], O.WIN_HANNING, (F, F))[0]
+    np.testing.assert_allclose(a, hanning(F * np.abs(px - s[0]) / np.abs(pz - s[2])), atol=2e-7)
+
+
+def check_tx_apod_aperture_clamp(tx_apod):
+    """Near the source (|axial| <= minimum aperture * F) the cone is replaced
+    by an aperture of the minimum width: boxcar nonzero iff the distance from
+    the axis is at most half of it."""
+    F, amin = 2.0, 3e-3
+    s = (0.0, 0.0, 0.0)
+    wv = dict(focused(0.0, 0.0), origin=s)
+    xs = np.array([0.5e-3, 1.2e-3, 1.45e-3, 1.55e-3, 2.5e-3])
+    px, py, pz = grid(np.concatenate([-xs, xs]), [1e-3, 2.5e-3, 5.5e-3])
+    b = tx_apod(px, py, pz, [wv], O.WIN_BOXCAR, (F, F), minimum_aperture=(amin, amin))[0]
+    np.testing.assert_array_equal(b != 0, np.abs(px) <= amin / 2)
+
+
+def test_tx_apod_cone():
+    check_tx_apod_cone(O.tx_apod)
+
+
+def test_tx_apod_grating_lobe_mask():
+    check_tx_apod_grating(O.tx_apod)
+
+
+def test_tx_apod_source_at_origin():
+    check_tx_apod_source_at_origin(O.tx_apod)
+
+
+def test_tx_apod_minimum_aperture():
+    check_tx_apod_aperture_clamp(O.tx_apod)
+
+
+@pytest.mark.parametrize("check,old,new", [
+    (check_tx_apod_cone, "SP = (px - s[0], py - s[1], pz - s[2])", "SP = (px - o[0], py - o[1], pz - o[2])"),
+    (check_tx_apod_cone, "xd = (SP[0] * xu[0] + SP[1] * xu[1]) + SP[2] * xu[2]",
+     "xd = (SP[0] * yu[0] + SP[1] * yu[1]) + SP[2] * yu[2]"),
+    (check_tx_apod_cone, "OS = s - o", "OS = o - s + np.array([1e-3, 0.0, 0.0])"),
+    (check_tx_apod_cone, "xu = np.array(", "yu = yu / np.linalg.norm(yu)\n        xu = np.array("),
+    (check_tx_apod_grating, "> 0.5 * grating_lobe_angle[0]] = EPS", "> grating_lobe_angle[0]] = EPS"),
+    (check_tx_apod_grating, "zx[np.abs(np.arctan2(OP[0], OP[2])", "zx[np.abs(np.arctan2(OP[2], OP[0])"),
+    (check_tx_apod_source_at_origin, "OS = np.array([0.0, 0.0, 1.0])", "OS = np.array([1.0, 0.0, 0.0])"),
+    (check_tx_apod_aperture_clamp, "zx = _clamp(zd, minimum_aperture[0] * f_number[0]",
+     "zx = _clamp(zd, minimum_aperture[0]"),
+])
+def test_tx_apod_pins_reject_planted_mistakes(check, old, new):
+    assert rejects(check, mutant(O.tx_apod, old, new))
+
+
+# ---------------------------------------------------------------------------
+# receive aperture: tilt, sector-scan origin, minimum / maximum aperture
+# (apodization.m:369-468, tools/rotate_points.m)
+# ---------------------------------------------------------------------------
+EX = np.array([-3e-3, 0.0, 2.4e-3])
+
+
+def check_rx_tilt_azimuth(rx_apod):
+    """Azimuth tilt theta: a pixel seen from an element at angle alpha from +z
+    (towards +x) has ratio F |tan(alpha - theta)|, so the window peaks on the
+    ray alpha = theta."""
+    F, th = 1.2, 0.2
+    ez = np.zeros(EX.size)
+    alpha = np.linspace(-0.45, 0.75, 25)
+    for rng in (8e-3, 25e-3):
+        for n, e in enumerate(EX):
+            px, pz = e + rng * np.sin(alpha), rng * np.cos(alpha)
+            a = rx_apod(px, np.zeros(px.size), pz, EX, ez, ez, O.WIN_HANNING, (F, F), tilt=(th, 0.0))[n]
+            np.testing.assert_allclose(a, hanning(F * np.abs(np.tan(alpha - th))), atol=2e-7)
+
+
+def check_rx_tilt_elevation(rx_apod):
+    """Elevation tilt phi on a pixel straight in front of the element in x at
+    elevation angle beta: the azimuth ratio is 0 and the elevation ratio is
+    F |tan(beta - phi)|."""
+    F, ph = 1.0, -0.15
+    e = np.zeros(1)
+    beta = np.linspace(-0.6, 0.4, 21)
+    py, pz = 20e-3 * np.sin(beta), 20e-3 * np.cos(beta)
+    a = rx_apod(np.zeros(beta.size), py, pz, e, e, e, O.WIN_HANNING, (F, F), tilt=(0.0, ph))[0]
+    np.testing.assert_allclose(a, hanning(F * np.abs(np.tan(beta - ph))), atol=2e-7)
+
+
+def check_rx_sector_origin(rx_apod):
+    """Sector scan with its origin 4 mm behind the array: the aperture is
+    centred on the origin's x and sized by the pixel's distance rho from the
+    origin: weight = window(F |x_e - x_o| / rho), the same at every azimuth."""
+    F = 1.4
+    o = (0.5e-3, 0.0, -4e-3)
+    ex = np.linspace(-9.45e-3, 9.45e-3, 64)
+    ez = np.zeros(ex.size)
+    az = np.array([-0.6, -0.1, 0.3, 0.7])
+    for rho in (6e-3, 18e-3, 40e-3):
+        px, pz = o[0] + rho * np.sin(az), o[2] + rho * np.cos(az)
+        a = rx_apod(px, np.zeros(az.size), pz, ex, ez, ez, O.WIN_HANNING, (F, F), origin=o)
+        ref = hanning(F * np.abs(ex - o[0]) / rho)
+        for k in range(az.size):
+            np.testing.assert_allclose(a[:, k], ref, atol=2e-7)
+
+
+def check_rx_aperture_clamps(rx_apod):
+    """Minimum aperture 2 mm and maximum 6 mm at F# 2: shallower than 4 mm
+    the boxcar is nonzero iff |x_p - x_e| <= 1 mm, deeper than 12 mm iff
+    <= 3 mm, and in between iff <= z / (2F)."""
+    F = 2.0
+    e = np.zeros(1)
+    xs = np.array([0.3e-3, 0.9e-3, 1.1e-3, 2.0e-3, 2.9e-3, 3.1e-3, 4.0e-3, 6.0e-3])
+    for z, half in ((1e-3, 1e-3), (3.5e-3, 1e-3), (8e-3, 2e-3), (13e-3, 3e-3), (30e-3, 3e-3)):
+        px = np.concatenate([-xs, xs])
+        a = rx_apod(px, np.zeros(px.size), np.full(px.size, z), e, e, e, O.WIN_BOXCAR, (F, F),
+                    minimum_aperture=(2e-3, 2e-3), maximum_aperture=(6e-3, 6e-3))[0]
+        np.testing.assert_array_equal(a != 0, np.abs(px) <= half)
+
+
+def test_rx_tilt_azimuth():
+    check_rx_tilt_azimuth(O.rx_apod)
+
+
+def test_rx_tilt_elevation():
+    check_rx_tilt_elevation(O.rx_apod)
+
+
+def test_rx_sector_origin():
+    check_rx_sector_origin(O.rx_apod)
+
+
+def test_rx_aperture_clamps():
+    check_rx_aperture_clamps(O.rx_apod)
+
+
+@pytest.mark.parametrize("check,old,new", [
+    (check_rx_tilt_azimuth, "xd, zd = xd * ct - zd * st, xd * st + zd * ct",
+     "xd, zd = xd * ct + zd * st, -xd * st + zd * ct"),
+    (check_rx_tilt_elevation, "yd, zd = yd * ct - zd * st, yd * st + zd * ct", "yd, zd = yd * ct - zd * st, zd"),
+    (check_rx_sector_origin, "xd = np.broadcast_to(ex[None, :] - x0, (P, N)).copy()",
+     "xd = px[:, None] - ex[None, :]"),
+    (check_rx_sector_origin, "zd = np.broadcast_to(pdist[:, None], (P, N)).copy()",
+     "zd = np.broadcast_to(dz[:, None], (P, N)).copy()"),
+    (check_rx_aperture_clamps, "zx = _clamp(zd, minimum_aperture[0] * f_number[0], maximum_aperture[0] * f_number[0])",
+     "zx = _clamp(zd, minimum_aperture[0], maximum_aperture[0])"),
+])
+def test_rx_pins_reject_planted_mistakes(check, old, new):
+    assert rejects(check, mutant(O.rx_apod, old, new))
+
+
+def test_clamp_pin_rejects_a_one_sided_clamp():
+    bad = mutant(O._clamp, "m = np.abs(z) >= max_af", "m = np.abs(z) >= np.inf")
+    saved = O._clamp
+    try:
+        O._clamp = bad
+        assert rejects(check_rx_aperture_clamps, O.rx_apod)
+    finally:
+        O._clamp = saved
+
+
+# ---------------------------------------------------------------------------
+# reference phase (das.m:306-309; python das.py:164-173)
+# ---------------------------------------------------------------------------
+FD = 5.2e6
+
+
+def check_reference_phase_quarter_waves(reference_phase):
+    """At two-way path 2r = lambda/4, /2, 3/4 (r = c/(8 fd), c/(4 fd),
+    3c/(8 fd)) the factor is -j, -1, +j; at r = 0 it is 1."""
+    r = np.array([0.0, C / (8 * FD), C / (4 * FD), 3 * C / (8 * FD)])
+    for flavour in ("python", "matlab"):
+        ph = reference_phase(r, FD, C, flavour=flavour)
+        np.testing.assert_allclose(ph, [1.0, -1j, -1.0, 1j], atol=2e-6)
+
+
+def test_reference_phase_quarter_waves():
+    check_reference_phase_quarter_waves(O.reference_phase)
+
+
+@pytest.mark.parametrize("old,new", [
+    ("np.exp(-1j * 2.0 * w0 * r / c)", "np.exp(1j * 2.0 * w0 * r / c)"),
+    ("np.exp(-1j * 2.0 * w0 * r / c)", "np.exp(-1j * w0 * r / c)"),
+    ("arg = np.float32(np.float32(np.float32(-2.0) * w32) * r32) / c32",
+     "arg = np.float32(np.float32(np.float32(2.0) * w32) * r32) / c32"),
+])
+def test_reference_phase_pin_rejects_planted_mistakes(old, new):
+    assert rejects(check_reference_phase_quarter_waves, mutant(O.reference_phase, old, new))
+
+
+def test_reference_phase_flavours_differ_by_the_single_precision_argument():
+    """das.m evaluates the argument in single precision (w0 = 2*pi*single(fd)
+    is single, and MATLAB's mixed single/double products are single), das.py
+    in FP64. Where every FP32 step is exact (fd = 2^20 Hz, c = 1024 m/s,
+    r = 2^-k m, all scalings of w0 by powers of two) the two agree to FP32 rounding of the result; at
+    C2-like depths their difference is the FP32 rounding of an argument of
+    ~2e3 rad, about 1e-4."""
+    fd, c = float(2 ** 20), 1024.0
+    r = 2.0 ** -np.arange(1.0, 24.0)
+    a = O.reference_phase(r, fd, c, flavour="python")
+    b = O.reference_phase(r, fd, c, flavour="matlab")
+    np.testing.assert_allclose(a, b, atol=3e-7)
+    r = np.linspace(30e-3, 55e-3, 4001)
+    a = O.reference_phase(r, FD, C, flavour="python")
+    b = O.reference_phase(r, FD, C, flavour="matlab")
+    d = np.abs(a - b)
+    arg = 2 * O.dasc_wd(FD) * r / C
+    ulp = np.spacing(arg.astype(np.float32)).astype(float)
+    assert d.max() > 1e-5                      # the flavours do differ here
+    assert np.all(d <= 2.0 * ulp + 3e-7)       # by at most three FP32 roundings of the argument

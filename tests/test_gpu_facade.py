@@ -137,6 +137,21 @@ def test_go_rf_goes_through_hilbert(cuda):
     assert _rel(got, ref.transpose(3, 2, 1, 0)) <= TOL
 
 
+def test_go_complex_rf_device_data_goes_through_hilbert(cuda):
+    """Complex RF with modulation frequency 0, handed over as a device
+    tensor, is made analytic from its real part as host data is (das.m:212:
+    hilbert whenever |w0| < eps)."""
+    case = CASES["pw_rf_hanning"]
+    rng = np.random.default_rng(18)
+    F, W, N, L = 1, len(case["seq"]), case["probe"].N_elements, case["L"]
+    rf = rng.standard_normal((F, W, N, L)).astype(np.float32)
+    host = _das(case, np.ascontiguousarray(rf.transpose(3, 2, 1, 0))).go().data.cpu().numpy()
+    d = _das(case, None)
+    d.channel_data.device_data = torch.from_numpy(rf.astype(np.complex64)).to(cuda)
+    got = d.go().data.cpu().numpy()
+    np.testing.assert_array_equal(got, host)
+
+
 @pytest.mark.parametrize("dim", [0, 1, 2, 3])
 def test_beamform_static_matches_python_reference(cuda, dim):
     pb = matrix_problem(seed=21 + dim, P=300, F=2)

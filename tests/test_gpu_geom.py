@@ -155,6 +155,30 @@ def test_geometry_reference_phase(cuda, name):
     plan.close()
 
 
+@pytest.mark.parametrize("flavour", ["python", "matlab"])
+def test_reference_phase_flavours(cuda, flavour):
+    """The plan's reference-phase factor per pixel (the image with the factor
+    over the image without it) against the oracle's flavour: das.py's FP64
+    argument or das.m's single-precision one. The two flavours differ by up to
+    about 1e-4 on this scan (30 mm at 5.2 MHz), so each must match its own."""
+    case = CASES["cpwc_tukey"]
+    print(f"case reference phase {flavour}", flush=True)
+    ch = torch.from_numpy(channel_noise(case, seed=12)).to(cuda)
+    plain = _plan(case, dim=3, separable_phase=True)
+    with_ref = _plan(case, dim=3, separable_phase=True, reference_phase=flavour)
+    a = plain.run(ch).cpu().numpy()[0, 0, 0].astype(np.complex128)
+    b = with_ref.run(ch).cpu().numpy()[0, 0, 0].astype(np.complex128)
+    keep = np.abs(a) > 1e-3 * np.abs(a).max()
+    ratio = b[keep] / a[keep]
+    r = case["scan"].reference_distance[keep]
+    own = O.reference_phase(r, case["fd"], 1540.0, flavour=flavour)
+    other = O.reference_phase(r, case["fd"], 1540.0, flavour="matlab" if flavour == "python" else "python")
+    assert np.max(np.abs(ratio - own)) <= 1e-6
+    assert np.max(np.abs(own - other)) > 2e-5 and np.max(np.abs(ratio - other)) > 2e-5
+    plain.close()
+    with_ref.close()
+
+
 @pytest.mark.parametrize("variant,wc,split", [(0, 8, 1), (0, 8, 4), (0, 16, 1), (0, 16, 4), (1, 8, 4),
                                               (1, 10, 4), (1, 12, 8), (1, 16, 8), (2, 4, 4), (2, 6, 8),
                                               (2, 8, 8), (2, 10, 4), (2, 12, 8), (2, 16, 4), (2, 5, 5)])

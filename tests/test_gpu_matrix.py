@@ -138,3 +138,23 @@ def test_nonfinite_and_huge_delays_and_nan_weights(cuda, dim):
         assert np.array_equal(np.isnan(got), np.isnan(ref))
         ok = ~np.isnan(ref)
         assert float(np.max(np.abs(got[ok] - ref[ok]))) <= TOL * float(np.max(np.abs(ref[ok])))
+
+
+@pytest.mark.parametrize("dim", [0, 1, 2, 3])
+@pytest.mark.parametrize("fd", [0.0, 0.05])
+def test_torch_op_matches_dasc(cuda, dim, fd):
+    """torch.ops.ustbx.das (the dispatcher-registered operator over the C-ABI)
+    against the das_c oracle, within 1e-5 of the peak (the operator runs the
+    default channel split, not das_c's summation order)."""
+    from ustb_amd import torch_ops  # noqa: F401  (registers the operator)
+
+    pb = matrix_problem(seed=70 + dim)
+    print(f"case torch op dim={dim} fd={fd}", flush=True)
+    ref = O.das_c(pb["ch"], pb["fs"], pb["t0"], pb["tx_apo"], pb["rx_apo"], pb["tx_delay"], pb["rx_delay"], fd, dim)
+    d = _dev(pb, cuda)
+    got = torch.ops.ustbx.das(d["ch"], d["fs"], d["t0"], d["tx_apo"], d["rx_apo"], d["tx_delay"], d["rx_delay"],
+                              fd, dim)
+    torch.cuda.synchronize()
+    got = got.cpu().numpy()
+    assert got.shape == ref.shape
+    assert _rel(got, ref) <= TOL

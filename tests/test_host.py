@@ -67,9 +67,9 @@ def test_mex_args_accept_valid_call():
     (dict(nrhs=8), "Toolbox:SRP_SRC:nrhs", "Too few input arguments"),
     (dict(nrhs=11), "Toolbox:SRP_SRC:nrhs", "Too many input arguments"),
     (dict(nlhs=2), "Toolbox:SRP_SRC:nlhs", "Too many output arguments"),
-    (dict(mode=4), "Toolbox:SRP_SRC:Argument", "Unknown sum mode"),
+    (dict(mode=4), "", "Unknown sum mode"),
     (dict(ch_ndim=5), "Toolbox:SRP_SRC:Dimensions", "Unknown channel data format"),
-    (dict(ch_double=1), "Toolbox:SRP_SRC:Float", "channel data should be single"),
+    (dict(ch_double=1), "", "channel data should be single"),
     (dict(mat_ndim=(2, 2, 3, 2)), "Toolbox:SRP_SRC:Dimensions", "Unknown transmit delay format"),
     (dict(mat_dims=((100, 3), (100, 8), (100, 4), (100, 8))), "Toolbox:SRP_SRC:Dimensions",
      "Number of waves in channel data & transmit delays"),
@@ -81,11 +81,11 @@ def test_mex_args_accept_valid_call():
      "Number of pixels in transmit apodization"),
     (dict(mat_dims=((100, 3), (100, 7), (100, 3), (100, 8))), "Toolbox:SRP_SRC:Dimensions",
      "Number of channels in channel data & receive apodization"),
-    (dict(mat_double=(0, 0, 1, 0)), "Toolbox:SRP_SRC:Float", "transmiy delays"),
-    (dict(numel=(2, 1, 1)), "Toolbox:SRP_SRC:Scalar", "sampling frequency should be an escalar"),
-    (dict(sc_double=(0, 1, 0)), "Toolbox:SRP_SRC:Float", "initial time should be single"),
-    (dict(numel=(1, 1, 3)), "Toolbox:SRP_SRC:Scalar", "Modulation frequency 'fd' should be an escalar"),
-    (dict(cplx=0), "Toolbox:SRP_SRC:Argument", "input data is real"),
+    (dict(mat_double=(0, 0, 1, 0)), "", "transmiy delays"),
+    (dict(numel=(2, 1, 1)), "", "sampling frequency should be an escalar"),
+    (dict(sc_double=(0, 1, 0)), "", "initial time should be single"),
+    (dict(numel=(1, 1, 3)), "", "Modulation frequency 'fd' should be an escalar"),
+    (dict(cplx=0), "", "input data is real"),
 ])
 def test_mex_arg_errors(kw, ident, text):
     rc, got_ident, msg, _ = _check(**kw)
@@ -237,3 +237,65 @@ def test_host_entry_rejects_null_inputs_before_device_work():
     rc = L.lib().ustbx_das_run_host(ctypes.byref(d), None, None, None, None, None, out.ctypes.data, 0,
                                     ctypes.byref(st))
     assert rc == L.E_ARG and b"NULL input" in st.message
+
+
+def test_plan_key_sees_every_apodization_element():
+    """Two receive apodization vectors of 1024 elements that differ in one
+    middle element (NumPy's repr elides it) give different plan keys."""
+    import numpy as np
+
+    from ustb_amd.das import DAS
+    from ustb_amd.uff import Apodization, ChannelData, LinearArray, LinearScan, Wave
+
+    d = DAS()
+    prb = LinearArray(4, 0.3e-3)
+    d.channel_data = ChannelData(probe=prb, sequence=[Wave.plane(0.0)], sampling_frequency=1e7, initial_time=0.0,
+                                 modulation_frequency=0.0, sound_speed=1540.0, data=np.zeros((8, 4, 1, 1)))
+    d.scan = LinearScan(np.zeros(1), np.linspace(1e-3, 2e-3, 4))
+    v = np.ones(1024, np.float32)
+    d.receive_apodization = Apodization(apodization_vector=v.copy())
+    k1 = d._plan_key(8, 1)
+    v[511] = 0.5
+    d.receive_apodization = Apodization(apodization_vector=v.copy())
+    assert d._plan_key(8, 1) != k1
+    d.receive_apodization = Apodization(apodization_vector=v.copy())
+    assert d._plan_key(8, 1) == d._plan_key(8, 1)
+
+
+def test_hilbert_rule_follows_w0():
+    """RF is made analytic whenever |2*pi*single(fd)| < eps (das.m:212,
+    das.py:143), for host and device data alike."""
+    from ustb_amd.das import DAS
+    from ustb_amd.uff import ChannelData
+
+    d = DAS()
+    d.channel_data = ChannelData(probe=None, sequence=[], sampling_frequency=1e7, modulation_frequency=0.0)
+    assert d._needs_hilbert()
+    d.channel_data = ChannelData(probe=None, sequence=[], sampling_frequency=1e7, modulation_frequency=5.2e6)
+    assert not d._needs_hilbert()
+
+
+def test_torch_op_fake_kernel_shapes_and_errors():
+    """torch.ops.ustbx.das: the fake (meta) kernel gives das_c's output shape
+    per dimension and das_c's errors; CPU tensors have no kernel (no
+    fallback)."""
+    import torch
+
+    from ustb_amd import torch_ops  # noqa: F401  (registers the operator)
+
+    ch = torch.empty((2, 3, 4, 64), dtype=torch.complex64, device="meta")
+    ta, ra = torch.empty((3, 100), device="meta"), torch.empty((4, 100), device="meta")
+    shapes = {0: (2, 3, 4, 100), 1: (2, 3, 1, 100), 2: (2, 1, 4, 100), 3: (2, 1, 1, 100)}
+    for dim, shape in shapes.items():
+        out = torch.ops.ustbx.das(ch, 1.0, 0.0, ta, ra, ta, ra, 0.0, dim)
+        assert tuple(out.shape) == shape and out.dtype == torch.complex64
+    with pytest.raises(RuntimeError, match="Toolbox:SRP_SRC:Dimensions: Number of channels"):
+        torch.ops.ustbx.das(ch, 1.0, 0.0, ta, ra, ta, torch.empty((5, 100), device="meta"), 0.0, 3)
+    with pytest.raises(RuntimeError, match="^Unknown sum mode"):
+        torch.ops.ustbx.das(ch, 1.0, 0.0, ta, ra, ta, ra, 0.0, 4)
+    with pytest.raises(RuntimeError, match="single precision"):
+        torch.ops.ustbx.das(ch.to(torch.complex128), 1.0, 0.0, ta, ra, ta, ra, 0.0, 3)
+    c = torch.zeros((1, 1, 2, 8), dtype=torch.complex64)
+    a, b = torch.zeros((1, 5)), torch.zeros((2, 5))
+    with pytest.raises(NotImplementedError):
+        torch.ops.ustbx.das(c, 1.0, 0.0, a, b, a, b, 0.0, 3)

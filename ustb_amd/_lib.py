@@ -28,6 +28,7 @@ DIM_NONE, DIM_RX, DIM_TX, DIM_BOTH = 0, 1, 2, 3
 INTERP_LINEAR, INTERP_CUBIC = 0, 1
 EDGE_DASC, EDGE_PYTHON = 0, 1
 FLAG_DASC_ORDER, FLAG_SEPARABLE_PHASE, FLAG_REF_PHASE, FLAG_PYTHON_ARITHMETIC = 1, 2, 4, 8
+FLAG_REF_PHASE_SINGLE = 16
 CF_MATLAB, CF_PYTHON = 0, 1
 FIR_CAUSAL, FIR_ZERO_PHASE = 0, 1
 SCAN_LINEAR, SCAN_SECTOR, SCAN_XYZ = 0, 1, 2

@@ -416,6 +416,8 @@ struct ustbx_batcher {
   hipEvent_t staged[2];  // that buffer's last uploads have left it
   bool pending[2];
   int cur;               // the buffer the next submission fills
+  hipEvent_t launched;   // the last submission's kernel (it reads dev) is done
+  bool has_launched;
 };
 
 extern "C" {
@@ -456,6 +458,7 @@ int ustbx_batcher_create(int32_t interp, int32_t max_acqs, int64_t max_items, in
     if (e == hipSuccess) e = hipHostMalloc(&b->staging[i], b->b_qoff + b->b_acq + b_items, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged[i], hipEventDisableTiming);
   }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->launched, hipEventDisableTiming);
   if (e != hipSuccess) {
     ustbx_batcher_destroy(b);
     return hip_fail(st, e, "batcher setup");
@@ -527,6 +530,12 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
     acqs[i].out = reinterpret_cast<float2*>(out[i]);
   }
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // the device buffers are shared by all submissions: on another stream than
+  // the previous one, the upload must wait for that submission's kernel
+  if (b->has_launched) {
+    e = hipStreamWaitEvent(s, b->launched, 0);
+    if (e != hipSuccess) return hip_fail(st, e, "batcher stream order");
+  }
   e = hipMemcpyAsync(b->dev, stg, b->b_qoff + sizeof(RxAcq) * count, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && at > 0)
     e = hipMemcpyAsync(b->dev + b->b_qoff + b->b_acq, items, sizeof(int4) * at, hipMemcpyHostToDevice, s);
@@ -534,7 +543,12 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
   if (e != hipSuccess) return hip_fail(st, e, "batcher upload");
   b->pending[c] = true;
   b->cur = 1 - c;
-  if (at == 0) return ok(st);
+  if (at == 0) {
+    e = hipEventRecord(b->launched, s);
+    if (e != hipSuccess) return hip_fail(st, e, "batcher stream order");
+    b->has_launched = true;
+    return ok(st);
+  }
   int* d_ctr = reinterpret_cast<int*>(b->dev + b->b_qoff + b->b_acq + ((sizeof(int4) * b->max_items + 255) & ~size_t(255)));
   hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, d_ctr);
   e = hipGetLastError();
@@ -544,12 +558,18 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
                   d_ctr};
     e = dispatch(b->sm, b->wc, b->ws, LaunchFn{B, b->nblocks, s});
   }
+  if (e == hipSuccess) e = hipEventRecord(b->launched, s);
   if (e != hipSuccess) return hip_fail(st, e, "das_batch_kernel launch");
+  b->has_launched = true;
   return ok(st);
 }
 
 void ustbx_batcher_destroy(ustbx_batcher* b) {
   if (!b) return;
+  if (b->launched) {
+    if (b->has_launched) (void)hipEventSynchronize(b->launched);
+    (void)hipEventDestroy(b->launched);
+  }
   for (int i = 0; i < 2; ++i) {
     if (b->pending[i]) (void)hipEventSynchronize(b->staged[i]);
     if (b->staged[i]) (void)hipEventDestroy(b->staged[i]);

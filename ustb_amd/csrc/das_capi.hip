@@ -104,18 +104,23 @@ int32_t ustbx_device_count(void) {
   return n;
 }
 
+// The identifier das_c raises each class of error with: nrhs, nlhs and
+// Dimensions through mexErrMsgIdAndTxt (das_c.cpp:53-55, 88-199); the
+// single-precision, scalar, sum-mode and real-data checks through
+// mexErrMsgTxt, with no identifier (das_c.cpp:79, 98, 145-242): "". GPU
+// errors take das_cuda's identifier (das_cuda.cu:122). Capacity and
+// unsupported-feature errors have no reference counterpart and an identifier
+// of this library's own.
 const char* ustbx_status_id(int32_t code) {
   switch (code) {
     case USTBX_OK: return "";
     case USTBX_E_NRHS: return "Toolbox:SRP_SRC:nrhs";
     case USTBX_E_NLHS: return "Toolbox:SRP_SRC:nlhs";
     case USTBX_E_DIMENSIONS: return "Toolbox:SRP_SRC:Dimensions";
-    case USTBX_E_FLOAT: return "Toolbox:SRP_SRC:Float";
-    case USTBX_E_SCALAR: return "Toolbox:SRP_SRC:Scalar";
     case USTBX_E_GPU: return "Toolbox:SRP_SRC:GPU";
-    case USTBX_E_CAPACITY: return "Toolbox:SRP_SRC:Capacity";
-    case USTBX_E_UNSUPPORTED: return "Toolbox:SRP_SRC:Unsupported";
-    default: return "Toolbox:SRP_SRC:Argument";
+    case USTBX_E_CAPACITY: return "ustbx:capacity";
+    case USTBX_E_UNSUPPORTED: return "ustbx:unsupported";
+    default: return "";  // USTBX_E_FLOAT, USTBX_E_SCALAR, USTBX_E_ARG: mexErrMsgTxt in das_c
   }
 }
 
@@ -784,8 +789,11 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   G.separable = ((g->flags & USTBX_FLAG_SEPARABLE_PHASE) && !(g->flags & USTBX_FLAG_DASC_ORDER)) ? 1 : 0;
   G.ref_phase = ((g->flags & USTBX_FLAG_REF_PHASE) && G.wd != 0.0f) ? 1 : 0;
   // das.m:308: exp(-1j*2*w0*r_ref/c); w0 = 2*pi*single(fd) is the FP32 value
-  // das_c calls wd (das.m:88-89), the rest is evaluated in FP64 here
+  // das_c calls wd (das.m:88-89); the argument in FP64 (das.py) or in FP32
+  // (das.m) as the flags select
   G.w0_ref = 2.0 * (double)G.wd / g->sound_speed;
+  G.ref_single = (g->flags & USTBX_FLAG_REF_PHASE_SINGLE) ? 1 : 0;
+  G.c32 = (float)g->sound_speed;
   G.scan.kind = sc.kind;
   G.scan.n0 = sc.n0; G.scan.n1 = sc.n1; G.scan.n2 = sc.n2;
   G.scan.a0 = (const double*)(base + off_a0);

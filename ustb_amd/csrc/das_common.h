@@ -356,6 +356,24 @@ __device__ __forceinline__ void rotate_points(double& x, double& y, double& z,
   }
 }
 
+// Reference phase exp(-1j*2*w0*r/c) (das.m:306-309). single = 0: the
+// argument -(2*w0/c)*r in FP64, as python das.py:169-171 forms it; single = 1:
+// das.m's single-precision argument fl32(fl32(-2*w0 * fl32(r)) / fl32(c)).
+// The factor is the FP64 sine and cosine of the argument rounded to FP32.
+__device__ __forceinline__ float2 ref_phase_factor(double w0_ref, float wd, float c32, int single, double r) {
+  double arg;
+  if (single) {
+#pragma clang fp contract(off)
+    const float a = (-2.0f * wd) * (float)r;
+    arg = (double)(a / c32);
+  } else {
+    arg = -(w0_ref * r);
+  }
+  double sv, cv;
+  sincos(arg, &sv, &cv);
+  return make_float2((float)cv, (float)sv);
+}
+
 // receive_delay = single(sqrt(xm.^2+ym.^2+zm.^2)/c + lens_delay)  (das.m:103-106)
 __device__ __forceinline__ float rx_delay_f32(double px, double py, double pz,
                                               double ex, double ey, double ez,

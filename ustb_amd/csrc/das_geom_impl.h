@@ -184,11 +184,7 @@ void das_geom_kernel(GeomLaunch G) {
 
   // reference phase exp(-1j*2*w0*r_ref/c) (das.m:306-309), FP64 -> FP32
   float2 refp = make_float2(1.0f, 0.0f);
-  if (G.ref_phase && active) {
-    double sv, cv;
-    sincos(-(G.w0_ref * rref), &sv, &cv);
-    refp = make_float2((float)cv, (float)sv);
-  }
+  if (G.ref_phase && active) refp = ref_phase_factor(G.w0_ref, G.wd, G.c32, G.ref_single, rref);
 
   // byte strides of the channel data within one frame (< 2 GiB, checked when
   // the plan is created)
@@ -511,11 +507,7 @@ void das_both_shared_rx_kernel(GeomLaunch G) {
 #pragma unroll
     for (int k = 1; k < WS; ++k) cadd(v, red[k][lane]);
     float2 refp = make_float2(1.0f, 0.0f);
-    if (G.ref_phase) {
-      double sv, cv;
-      sincos(-(G.w0_ref * rref), &sv, &cv);
-      refp = make_float2((float)cv, (float)sv);
-    }
+    if (G.ref_phase) refp = ref_phase_factor(G.w0_ref, G.wd, G.c32, G.ref_single, rref);
     reinterpret_cast<float2*>(G.out)[p] = ref_apply(v, refp, G.ref_phase);
   }
 }

@@ -41,9 +41,7 @@ __global__ __launch_bounds__(kBlock) void geom_ref_table_kernel(GeomLaunch G, fl
   if (p >= G.P) return;
   double px, py, pz, rref;
   pixel_xyz(G.scan, p, px, py, pz, rref);
-  double sv, cv;
-  sincos(-(G.w0_ref * rref), &sv, &cv);
-  ref[p] = make_float2((float)cv, (float)sv);
+  ref[p] = ref_phase_factor(G.w0_ref, G.wd, G.c32, G.ref_single, rref);
 }
 
 // One wave64 per 64-pixel tile: the lists of waves and channels with a

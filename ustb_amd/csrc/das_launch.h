@@ -37,6 +37,8 @@ struct GeomLaunch {
   int separable;       // exp(i wd tx) * exp(i wd rx) instead of exp(i wd (tx+rx))
   int ref_phase;       // multiply by exp(-1j*2*w0*r_ref/c)
   double w0_ref;       // 2*w0/c for the reference phase, in FP64
+  int ref_single;      // the reference phase's argument in FP32 (das.m) instead of FP64 (das.py)
+  float c32;           // single(c) for it
   DevScan scan;
   const double* ex;    // element positions [N]
   const double* ey;

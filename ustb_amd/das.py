@@ -46,44 +46,66 @@ class DAS:
         self.code = Code.mex_hip
         self.interpolation = Interpolation.linear
         self.phase = "separable"       # or "dasc": das_c's per-term phase arithmetic
+        # the IQ reference phase exp(-1j*2*w0*r/c): its argument in FP64 as das.py
+        # forms it ("python") or in single precision as das.m does ("matlab")
+        self.reference_phase = "python"
         self.gpu_id = 0
         self.beamformed_data = None
         self.elapsed_time = None
         self._plans: OrderedDict = OrderedDict()  # most recently used last
 
     # ------------------------------------------------------------------
+    @staticmethod
+    def _apodization_key(h, apo: Apodization):
+        """Every field of an Apodization into the hash, arrays by their bytes
+        (a repr would elide the middle of long vectors)."""
+        h.update(repr((int(apo.window), apo.f_number, apo.minimum_aperture, apo.maximum_aperture, apo.tilt,
+                       apo.grating_lobe_angle, None if apo.origin is None else apo.origin.xyz)).encode())
+        v = apo.apodization_vector
+        if v is None:
+            h.update(b"no-vector")
+        else:
+            v = np.asarray(v)
+            h.update(repr(v.shape).encode())
+            h.update(np.ascontiguousarray(v, dtype=np.float64).tobytes())
+
     def _plan_key(self, n_samples, n_frames):
         cd, sc = self.channel_data, self.scan
         h = hashlib.sha1()
         for a in (cd.probe.x, cd.probe.y, cd.probe.z, sc.x, sc.y, sc.z):
             h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
         h.update(repr([(int(w.wavefront), w.source, w.origin, w.delay, w.sound_speed) for w in cd.sequence]).encode())
-        h.update(repr((self.receive_apodization, self.transmit_apodization)).encode())
+        self._apodization_key(h, self.receive_apodization)
+        self._apodization_key(h, self.transmit_apodization)
         h.update(repr((n_samples, n_frames, cd.sampling_frequency, cd.initial_time, cd.modulation_frequency,
                        cd.sound_speed, int(self.dimension), int(self.interpolation), self.lens_delay,
                        self.pw_margin, self.blending_power, int(self.spherical_transmit_delay_model),
-                       self.phase, type(sc).__name__, int(self.gpu_id))).encode())
+                       self.phase, self.reference_phase, type(sc).__name__, int(self.gpu_id))).encode())
         return h.hexdigest()
+
+    def _needs_hilbert(self) -> bool:
+        """das.m:212 / das.py:143: RF data (|w0| < eps, w0 = 2*pi*single(fd))
+        is made analytic, whatever its storage type."""
+        w0 = np.float32(2.0 * np.pi * np.float32(self.channel_data.modulation_frequency))
+        return abs(float(w0)) < float(np.finfo(np.float32).eps)
 
     def _device_channel_data(self, torch, dev):
         """Channel data as a complex64 (F, W, N, L) device tensor; RF data is
         made analytic first (das.m:211-214)."""
         cd = self.channel_data
         if cd.device_data is not None:
-            x = cd.device_data
-            if not x.is_complex():
-                from .ops import hilbert
-
-                x = hilbert(x.float().contiguous())
-            return x.to(dev).contiguous()
-        data = np.asarray(cd.data)
-        while data.ndim < 4:
-            data = data[..., None]
-        x = torch.from_numpy(np.ascontiguousarray(data.transpose(3, 2, 1, 0))).to(dev)
-        if abs(np.float32(cd.modulation_frequency)) < np.finfo(np.float32).eps:
+            x = cd.device_data.to(dev)
+        else:
+            data = np.asarray(cd.data)
+            while data.ndim < 4:
+                data = data[..., None]
+            x = torch.from_numpy(np.ascontiguousarray(data.transpose(3, 2, 1, 0))).to(dev)
+        if self._needs_hilbert():
             from .ops import hilbert
 
             x = hilbert(x.real.float().contiguous() if x.is_complex() else x.float().contiguous())
+        elif not x.is_complex():
+            raise ValueError("IQ channel data (modulation frequency > 0) must be complex")
         return x.to(torch.complex64).contiguous()
 
     def go(self) -> BeamformedData:
@@ -114,7 +136,8 @@ class DAS:
                                  receive_apodization=self.receive_apodization,
                                  transmit_apodization=self.transmit_apodization, dimension=self.dimension,
                                  interpolation=self.interpolation, separable_phase=(self.phase == "separable"),
-                                 reference_phase=True, lens_delay=self.lens_delay, pw_margin=self.pw_margin,
+                                 reference_phase=self.reference_phase, lens_delay=self.lens_delay,
+                                 pw_margin=self.pw_margin,
                                  blending_power=self.blending_power,
                                  tx_model=self.spherical_transmit_delay_model, device=dev)
             self._plans[key] = plan

@@ -58,12 +58,18 @@ def build_geometry(probe: Probe, scan: Scan, sequence: list, *, n_samples: int, 
                    transmit_apodization: Apodization | None = None,
                    dimension: int = Dimension.both, interpolation: int = Interpolation.linear,
                    edge: int = L.EDGE_DASC, separable_phase: bool = True,
-                   reference_phase: bool = False, dasc_order: bool = False,
+                   reference_phase: bool | str = False, dasc_order: bool = False,
                    lens_delay: float = 0.0, pw_margin: float = 1e-3, blending_power: float = 0.5,
                    tx_model: int = SphericalTransmitDelayModel.hybrid,
                    tx_delay_matrix=None, rx_apod_matrix=None, tx_apod_matrix=None):
     """Returns (Geometry struct, keepalive list). Matrices, when given, are
-    device tensors ([W, P] / [N, P] float32) that stay owned by the caller."""
+    device tensors ([W, P] / [N, P] float32) that stay owned by the caller.
+
+    reference_phase: False, True or "python" (the factor exp(-1j*2*w0*r/c)
+    with its argument in FP64, python das.py:169-171), or "matlab" (the
+    argument in single precision, das.m:308)."""
+    if reference_phase not in (False, True, "python", "matlab"):
+        raise ValueError(f"reference_phase must be a bool, 'python' or 'matlab', not {reference_phase!r}")
     keep: list = []
     g = L.Geometry()
     g.N = probe.N_elements
@@ -125,6 +131,7 @@ def build_geometry(probe: Probe, scan: Scan, sequence: list, *, n_samples: int, 
     g.edge = int(edge)
     g.flags = ((L.FLAG_SEPARABLE_PHASE if separable_phase else 0)
                | (L.FLAG_REF_PHASE if reference_phase else 0)
+               | (L.FLAG_REF_PHASE_SINGLE if reference_phase == "matlab" else 0)
                | (L.FLAG_DASC_ORDER if dasc_order else 0))
     return g, keep
 

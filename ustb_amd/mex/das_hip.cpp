@@ -10,8 +10,8 @@
 // das_cuda.cu:321-324). Output: complex single [pixel, N', W', frame].
 //
 // Argument checks follow das_c (+mex/source/das_c.cpp:53-252) through
-// ustbx_das_check_mex_args; errors are raised with the reference's
-// "Toolbox:SRP_SRC:*" identifiers. The 10th argument is das_cuda's gpu_id
+// ustbx_das_check_mex_args; errors are raised as das_c raises them, with its
+// "Toolbox:SRP_SRC:*" identifier or (das_c's mexErrMsgTxt cases) none. The 10th argument is das_cuda's gpu_id
 // (das_cuda.cu:161-162); das_c's (dead) verbose flag in that position is
 // not supported.
 //
@@ -25,8 +25,14 @@
 #include "mex.h"
 #include "ustbx_das.h"
 
+// das_c raises some errors with an identifier and the others through
+// mexErrMsgTxt (ustbx_status_id gives "" for those); so does the gateway.
 static void fail(int code, const char* msg) {
-  mexErrMsgIdAndTxt(ustbx_status_id(code), "%s", msg);
+  const char* id = ustbx_status_id(code);
+  if (id[0])
+    mexErrMsgIdAndTxt(id, "%s", msg);
+  else
+    mexErrMsgTxt(msg);
 }
 
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {

@@ -164,8 +164,13 @@ def hilbert(rf: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         raise TypeError("RF data must be float32")
     Ls = rf.shape[-1]
     traces = rf.numel() // max(Ls, 1)
+    if not rf.is_contiguous():
+        raise ValueError("RF data must be contiguous")
     if out is None:
         out = torch.empty(rf.shape, dtype=torch.complex64, device=rf.device)
+    elif (not isinstance(out, torch.Tensor) or out.dtype != torch.complex64 or tuple(out.shape) != tuple(rf.shape)
+          or not out.is_contiguous() or out.device != rf.device):
+        raise ValueError(f"out must be a contiguous complex64 tensor of shape {tuple(rf.shape)} on {rf.device}")
     st = L.Status()
     L.check(L.lib().ustbx_hilbert(rf.data_ptr(), out.data_ptr(), Ls, traces, _stream_ptr(rf.device),
                                   ctypes.byref(st)), st)
