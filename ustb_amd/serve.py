@@ -18,6 +18,23 @@ its latency is arrival -> result in host memory.
 
 `plain_loop` is the baseline: one request at a time, one plan launch per
 frame (copy in, beamform, copy out, wait).
+
+Every request ends with a status (ServeResult.status):
+- "ok": its image is in host memory (latency measured);
+- "rejected": its input and output alone exceed the HBM budget, so it is
+  never admitted;
+- "failed": the library refused its work (a C-ABI error on submission). A
+  submission that fails is retried one request at a time, so the error stays
+  with the request that caused it and the others of the iteration complete;
+- "cancelled": run()'s cancel callback returned true for it, or it was not
+  done within its Request.deadline_s of its arrival. Work already queued for
+  it finishes on the GPU before its buffers are released; nothing more is
+  queued for it.
+A fault of the device itself (a kernel that faults) ends the process's GPU
+context and cannot be confined to one request.
+
+roctx marks (ustb_amd.trace) name each admission, iteration and completion
+for rocprofv3 --marker-trace.
 """
 from __future__ import annotations
 
@@ -31,6 +48,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import trace as tr
 from .ops import GeometryPlan, _stream_ptr
 
 
@@ -40,6 +58,7 @@ class Request:
     config: int
     arrival: float          # seconds after the trace starts
     data: int               # dataset index of its configuration
+    deadline_s: float | None = None  # cancelled if not done this long after arrival
 
 
 def poisson_trace(n: int, rate: float, n_configs: int, n_data: int, seed: int = 5) -> list[Request]:
@@ -95,30 +114,44 @@ class _Live:
     next_pos: int = 0
     done: torch.cuda.Event | None = None
     host: torch.Tensor | None = None
+    last: torch.cuda.Event | None = None  # the last iteration that ran some of its work
 
 
 @dataclass
 class ServeResult:
-    latency_s: np.ndarray            # per request, in trace order
+    latency_s: np.ndarray            # per request, in trace order (NaN unless "ok")
     configs: np.ndarray              # configuration per request
     elapsed_s: float                 # first arrival -> last result
     iterations: int
     outputs: dict = field(default_factory=dict)   # rid -> host image (keep_outputs)
+    status: list = field(default_factory=list)    # per request: ok / rejected / failed / cancelled
+    errors: dict = field(default_factory=dict)    # rid -> message of a failed or rejected request
+
+    @property
+    def ok(self) -> np.ndarray:
+        if not self.status:
+            return np.ones(len(self.latency_s), bool)
+        return np.array([s == "ok" for s in self.status])
 
     @property
     def frames_per_s(self) -> float:
-        return len(self.latency_s) / self.elapsed_s
+        return int(self.ok.sum()) / self.elapsed_s
 
     def summary(self, names: list[str] | None = None) -> dict:
-        lat = self.latency_s * 1e3
+        ok = self.ok
+        lat = self.latency_s[ok] * 1e3
+        cfg = self.configs[ok]
         res = {"frames": int(lat.size), "frames_per_s": round(self.frames_per_s, 2),
                "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                "elapsed_s": round(self.elapsed_s, 3)}
+        if self.status:
+            res["status"] = {k: int(sum(s == k for s in self.status)) for k in ("ok", "rejected", "failed",
+                                                                              "cancelled")}
         if self.iterations:
             res["iterations"] = self.iterations
         per = {}
-        for k in np.unique(self.configs):
-            m = self.configs == k
+        for k in np.unique(cfg):
+            m = cfg == k
             key = names[k] if names else str(int(k))
             per[key] = {"frames": int(m.sum()), "p50_ms": round(float(np.percentile(lat[m], 50)), 3),
                         "p99_ms": round(float(np.percentile(lat[m], 99)), 3)}
@@ -204,10 +237,17 @@ class Server:
                 del ch, out
         return res
 
-    def run(self, trace: list[Request], keep_outputs: bool = False) -> ServeResult:
+    def _submit(self, plans, chs, outs, begin, end):
+        self.batcher.submit(plans, chs, outs, begin, end, self.s_run)
+
+    def run(self, trace: list[Request], keep_outputs: bool = False, cancel=None) -> ServeResult:
+        """Serve the trace; cancel(rid) -> bool is asked about every request
+        not yet done on every pass of the loop (a true answer cancels it)."""
         dev = self.device
         n = len(trace)
-        lat = np.zeros(n)
+        lat = np.full(n, np.nan)
+        status = ["ok"] * n
+        errors = {}
         waiting: list = []  # heap of (input bytes, arrival, rid, request)
         staged: deque[_Live] = deque()
         copying = 0  # input bytes queued on the copy stream
@@ -215,6 +255,7 @@ class Server:
         ready: list[_Live] = []
         running: deque[tuple[torch.cuda.Event, list[_Live]]] = deque()
         draining: deque[_Live] = deque()
+        releasing: list[_Live] = []  # cancelled or failed, waiting for their queued work
         outputs = {}
         in_flight = 0
         completed = 0
@@ -223,15 +264,61 @@ class Server:
         t0 = time.perf_counter()
         last_done = 0.0
         backlog_ms = 0.0  # arrived work not yet submitted, at the calibrated cost
+
+        def stop(r: Request, why: str) -> bool:
+            """Whether request r is to be cancelled now."""
+            if cancel is not None and cancel(r.rid):
+                return True
+            return r.deadline_s is not None and (time.perf_counter() - t0) - r.arrival > r.deadline_s
+
+        def release(lv: _Live, st: str, msg: str | None = None):
+            """End a request that did not complete: its queued work (if any)
+            finishes before its buffers go."""
+            nonlocal backlog_ms
+            status[lv.req.rid] = st
+            if msg:
+                errors[lv.req.rid] = msg
+            backlog_ms -= (self.positions[lv.req.config] - lv.next_pos) * self.ms_per_pos[lv.req.config]
+            lv.next_pos = self.positions[lv.req.config]
+            releasing.append(lv)
+            tr.mark(f"{st} rid={lv.req.rid}")
+
         with torch.cuda.device(dev):
             while completed < n:
                 now = time.perf_counter() - t0
                 progressed = False
                 while nxt < n and trace[nxt].arrival <= now:
                     r = trace[nxt]
+                    nxt += 1
+                    need = self.in_bytes[r.config] + self.out_bytes[r.config]
+                    if need > self.budget:  # can never be admitted
+                        status[r.rid] = "rejected"
+                        errors[r.rid] = f"needs {need} bytes of the {self.budget}-byte HBM budget"
+                        tr.mark(f"rejected rid={r.rid}")
+                        completed += 1
+                        continue
                     heapq.heappush(waiting, (self.in_bytes[r.config], r.arrival, r.rid, r))
                     backlog_ms += self.positions[r.config] * self.ms_per_pos[r.config]
-                    nxt += 1
+                # cancellations: not yet admitted, being copied in, or ready
+                if waiting and (cancel is not None or any(w[3].deadline_s is not None for w in waiting)):
+                    keep = []
+                    for w in waiting:
+                        if stop(w[3], "cancelled"):
+                            status[w[3].rid] = "cancelled"
+                            backlog_ms -= self.positions[w[3].config] * self.ms_per_pos[w[3].config]
+                            tr.mark(f"cancelled rid={w[3].rid}")
+                            completed += 1
+                            progressed = True
+                        else:
+                            keep.append(w)
+                    if len(keep) != len(waiting):
+                        waiting = keep
+                        heapq.heapify(waiting)
+                for lv in list(ready):
+                    if stop(lv.req, "cancelled"):
+                        ready.remove(lv)
+                        release(lv, "cancelled")
+                        progressed = True
                 # admission: smallest input first, under the HBM budget, with
                 # at most copy_window bytes and one large frame's input queued
                 # on the copy stream (so a small frame's copy waits behind at
@@ -239,7 +326,7 @@ class Server:
                 while waiting:
                     r = waiting[0][3]
                     need = self.in_bytes[r.config] + self.out_bytes[r.config]
-                    if in_flight + need > self.budget and in_flight > 0:
+                    if in_flight + need > self.budget:
                         break
                     if copying + self.in_bytes[r.config] > self.copy_window and copying > 0:
                         break
@@ -258,18 +345,22 @@ class Server:
                         ev.record(self.s_in)
                     staged.append(_Live(r, need, ch, out, ev))
                     in_flight += need
+                    tr.mark(f"admit rid={r.rid} config={r.config}")
                     progressed = True
                 while staged and staged[0].ready.query():
                     lv = staged.popleft()
                     copying -= self.in_bytes[lv.req.config]
                     copying_large -= 0 if self._small(lv.req.config) else 1
-                    ready.append(lv)
+                    if stop(lv.req, "cancelled"):
+                        release(lv, "cancelled")
+                    else:
+                        ready.append(lv)
                     progressed = True
                 # an iteration over the ready requests; the settings follow the
                 # load: the arrived work not yet submitted
                 idle = backlog_ms < self.idle_below_ms
                 if ready and len(running) < (self.idle_depth if idle else self.depth):
-                    plans, chs, outs, begin, end, finishing = [], [], [], [], [], []
+                    group, begin, end = [], [], []
                     # a short iteration while small requests are pending (their
                     # latency), a long one when only large frames are (fewer,
                     # fuller launches)
@@ -285,7 +376,7 @@ class Server:
                     ready.sort(key=lambda lv: ((self.positions[lv.req.config] - lv.next_pos) *
                                                self.ms_per_pos[lv.req.config], lv.req.arrival))
                     for lv in ready:
-                        if plans and (budget <= 0 or len(plans) == self.max_acqs):
+                        if group and (budget <= 0 or len(group) == self.max_acqs):
                             break  # (the oldest ready request always gets a slice)
                         k = lv.req.config
                         left = self.positions[k] - lv.next_pos
@@ -296,19 +387,37 @@ class Server:
                             if take <= 0:
                                 break
                         items += take * per_pos
-                        plans.append(self.plans[k])
-                        chs.append(lv.ch)
-                        outs.append(lv.out)
+                        group.append(lv)
                         begin.append(lv.next_pos)
                         end.append(lv.next_pos + take)
-                        lv.next_pos += take
                         budget -= self.ms_per_pos[k] * take
+                    with tr.range_(f"iteration {iters}: {len(group)} requests"):
+                        try:
+                            self._submit([self.plans[lv.req.config] for lv in group], [lv.ch for lv in group],
+                                         [lv.out for lv in group], begin, end)
+                            done_ok = list(range(len(group)))
+                        except L.USTBXError:
+                            # one request at a time: the error stays with its request
+                            done_ok = []
+                            for i, lv in enumerate(group):
+                                try:
+                                    self._submit([self.plans[lv.req.config]], [lv.ch], [lv.out], [begin[i]],
+                                                 [end[i]])
+                                    done_ok.append(i)
+                                except L.USTBXError as e:
+                                    release(lv, "failed", str(e))
+                    ev = torch.cuda.Event()
+                    ev.record(self.s_run)
+                    finishing = []
+                    for i in done_ok:
+                        lv = group[i]
+                        k = lv.req.config
+                        take = end[i] - begin[i]
+                        lv.next_pos += take
+                        lv.last = ev
                         backlog_ms -= self.ms_per_pos[k] * take
                         if lv.next_pos >= self.positions[k]:
                             finishing.append(lv)
-                    self.batcher.submit(plans, chs, outs, begin, end, self.s_run)
-                    ev = torch.cuda.Event()
-                    ev.record(self.s_run)
                     running.append((ev, finishing))
                     ready = [lv for lv in ready if lv.next_pos < self.positions[lv.req.config]]
                     iters += 1
@@ -335,11 +444,22 @@ class Server:
                     in_flight -= lv.bytes
                     lv.ch = lv.out = lv.host = None
                     completed += 1
+                    tr.mark(f"complete rid={lv.req.rid}")
                     progressed = True
+                # cancelled or failed requests: buffers go once their queued work is done
+                for lv in list(releasing):
+                    if lv.last is None or lv.last.query():
+                        releasing.remove(lv)
+                        in_flight -= lv.bytes
+                        lv.ch = lv.out = lv.host = None
+                        completed += 1
+                        progressed = True
                 if not progressed:
                     time.sleep(20e-6)
         cfgs = np.array([r.config for r in trace])
-        return ServeResult(lat, cfgs, last_done - trace[0].arrival, iters, outputs)
+        ok = [status[r.rid] == "ok" for r in trace]
+        first = min((r.arrival for r, o in zip(trace, ok) if o), default=0.0)
+        return ServeResult(lat, cfgs, max(last_done - first, 1e-9), iters, outputs, status, errors)
 
     def plain_loop(self, trace: list[Request], keep_outputs: bool = False) -> ServeResult:
         """One request at a time: copy in, one launch of its plan, copy out, wait."""
