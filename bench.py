@@ -68,6 +68,8 @@ def parse():
     ap.add_argument("--wave-chunk", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant (default: the plan's choice)")
+    ap.add_argument("--stage", type=int, default=1, choices=[0, 1],
+                    help="dimension transmit: stage sample windows through LDS (1) or gather (0)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a 1 GiB device-to-device copy (achievable HBM bandwidth; also the "
                          "known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
@@ -170,6 +172,7 @@ def main():
         plan.set_variant(args.variant)
     if args.wave_chunk or args.split:
         plan.set_tiles(args.wave_chunk, args.split)
+    plan.set_staging(bool(args.stage))
     variant, wave_chunk, split = plan.variant
     out = torch.empty(plan.out_shape, dtype=torch.complex64, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -224,7 +227,8 @@ def main():
                    "dimension": int(acq.dimension), "phase": args.phase,
                    "geometry": "on-device, resident per-configuration tables",
                    "plan_build_ms": round(plan_ms, 2),
-                   "kernel_variant": variant, "waves_per_split": wave_chunk, "splits": split},
+                   "kernel_variant": variant, "waves_per_split": wave_chunk, "splits": split,
+                   "lds_staging": bool(args.stage) if int(acq.dimension) == 2 else None},
         "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": VALU_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / VALU_PEAK_TFLOPS, 4),
                      "traffic": traffic,
@@ -241,7 +245,8 @@ def main():
     if not args.no_check:
         result["errors"] = check_errors(acq, plan, out, ch, torch)
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(acq, plan, torch)
+        lin = gpu_linear_rate(acq, ch, args, torch, dev) if int(acq.interpolation) != 0 else None
+        result["cpu_baseline"] = cpu_baseline(acq, plan, torch, gpu_linear=lin)
     plan.close()
     print(json.dumps(result))
 
@@ -953,17 +958,47 @@ def check_errors(acq, plan, out, ch, torch):
     return res
 
 
-def cpu_baseline(acq, plan, torch):
+def host_cpus() -> dict:
+    """The host CPUs this process may use: the affinity mask, capped by the
+    cgroup's CPU quota where one is set (a share of a larger machine), with
+    nproc and the CPU model for the record."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(per)))
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": aff, "cgroup_quota": quota,
+            "threads": min(aff, quota) if quota else aff, "model": model}
+
+
+def cpu_baseline(acq, plan, torch, gpu_linear=None):
     """The das_c restatement (C, pthreads over pixels like das_c's
     tbb::parallel_for) timed on the host on a pixel sample of the same
-    workload, linear interpolation (das_c has no cubic)."""
+    workload, linear interpolation (das_c has no cubic), with every host CPU
+    this process may use; median of 5 runs after 1 warm-up. gpu_linear: the
+    GPU's rate on the same workload with linear interpolation, when the
+    headline runs another interpolation, so that the two can be compared."""
     from oracle import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    cpus = host_cpus()
+    threads = cpus["threads"]
     ta, ra, td, rd = [x.cpu().numpy() for x in plan.materialize()]
-    # bounded sample: every k-th pixel column so that one run is ~2-5 s
+    # bounded sample: every k-th pixel column so that one run is ~2-4 s
     per_unit_s = 1.0 / (60e6 * threads)                   # rough prior
-    target_s = 3.0
+    target_s = 2.5
     frac = min(1.0, target_s / max(acq.units * per_unit_s, 1e-9))
     k = max(1, int(round(acq.P * frac)))
     pix = _sample_pixels(acq.P, k, seed=7)
@@ -971,15 +1006,43 @@ def cpu_baseline(acq, plan, torch):
             np.ascontiguousarray(td[:, pix]), np.ascontiguousarray(rd[:, pix]), acq.fd, int(acq.dimension))
     O.das_c(*args, nthreads=threads)                      # warm-up
     times = []
-    for _ in range(3):
+    for _ in range(5):
         t = time.perf_counter()
         O.das_c(*args, nthreads=threads)
         times.append(time.perf_counter() - t)
     med = float(np.median(times))
     units = pix.size * acq.probe.N_elements * len(acq.sequence) * acq.n_frames
-    return {"value": round(units / med / 1e9, 4), "unit": "Gsamples/s", "cores": threads, "kind": "port",
-            "sample": f"{pix.size} of {acq.P} pixels (all channels, waves, frames), linear interpolation, "
-                      f"median of 3 runs after 1 warm-up, {med:.2f} s per run"}
+    res = {"value": round(units / med / 1e9, 4), "unit": "Gsamples/s", "cores": threads, "kind": "port",
+           "sample": f"{pix.size} of {acq.P} pixels (all channels, waves, frames), linear interpolation, "
+                     f"median of 5 runs after 1 warm-up, {med:.2f} s per run",
+           "host": cpus}
+    if gpu_linear is not None:
+        res["gpu_linear_value"] = gpu_linear
+        res["gpu_linear_over_cpu"] = round(gpu_linear / res["value"], 1)
+    return res
+
+
+def gpu_linear_rate(acq, ch, args, torch, dev):
+    """Gsamples/s of the geometry plan with linear interpolation on the same
+    workload (kernel time by HIP events over the steps)."""
+    from ustb_amd.geometry import make_plan
+
+    lin = make_plan(acq.probe, acq.scan, acq.sequence, separable_phase=(args.phase == "separable"),
+                    **acq.plan_kwargs(interpolation=0))
+    out = torch.empty(lin.out_shape, dtype=torch.complex64, device=dev)
+    for _ in range(max(1, args.warmup)):
+        lin.run(ch, out)
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        lin.run(ch, out)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    lin.close()
+    return round(acq.units / (ms * 1e-3) / 1e9, 3)
 
 
 if __name__ == "__main__":

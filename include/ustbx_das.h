@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define USTBX_ABI_VERSION 1
+#define USTBX_ABI_VERSION 2
 
 /* error codes, with the identifier das_c raises each class with
  * (ustbx_status_id; +mex/source/das_c.cpp:53-252, +mex/source/das_cuda.cu:122).
@@ -184,7 +184,7 @@ const char* ustbx_status_id(int32_t code);
 #define USTBX_WIN_TUKEY50 5
 #define USTBX_WIN_TUKEY75 6
 #define USTBX_WIN_STA 7      /* host: reduces to a per-element vector */
-#define USTBX_WIN_SCANLINE 8 /* host: pass a matrix */
+#define USTBX_WIN_SCANLINE 8 /* tx: MLA scan-line blocks (apodization.m:265-306) */
 #define USTBX_WIN_TRIANGLE 9
 
 typedef struct ustbx_scan {
@@ -225,6 +225,11 @@ typedef struct ustbx_apod {
                            (apodization_vector, window.sta); host pointer */
   const float* matrix;  /* optional explicit matrix ([N][P] rx, [W][P] tx);
                            DEVICE pointer, used as is */
+  int32_t mla[2];       /* tx, USTBX_WIN_SCANLINE: scan lines per wave in
+                           x / azimuth and y / elevation (apodization.m:47;
+                           0 = 1) */
+  int32_t mla_overlap[2]; /* lines each wave's block is widened by, as a box
+                           filter (apodization.m:48) */
 } ustbx_apod;
 
 typedef struct ustbx_geometry {
@@ -290,6 +295,11 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st);
  * scan lines (the default; it keeps the sample windows that co-resident
  * blocks read in L2). Results do not depend on it. */
 int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st);
+/* Dimension transmit on variant 2 (no reference counterpart): 1 (the
+ * default) stages each 64-pixel tile's sample window of every (channel,
+ * wave) through LDS with contiguous loads, 0 gathers each term's samples.
+ * Results do not depend on it. */
+int ustbx_plan_set_staging(ustbx_plan* plan, int32_t on, ustbx_status* st);
 /* Current variant (or -1 for NULL) and its tile shape. */
 int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split);
 
@@ -450,6 +460,18 @@ int ustbx_plan_active_elements(const ustbx_plan* plan, int32_t dim, double crite
  * frequency (fs / down). No allocation; asynchronous on stream. */
 int ustbx_demodulate(const float* rf, void* iq, int32_t L, int64_t traces, double fs, double t0, double fmod,
                      const double* b, int32_t ntaps, int32_t down, int32_t mode, void* stream, ustbx_status* st);
+
+/* The modulation frequency preprocess.fast_demodulation estimates when none
+ * is set (fast_demodulation.m:57-73): rf [traces][L] float32 (device; the
+ * first min(100, F) frames of the channel data, F*W*N traces in the MEX
+ * layout), fs [Hz]. The power spectrum averaged over the traces
+ * (tools.power_spectrum, power_spectrum.m:1-22; fftshifted, normalised to its
+ * maximum) is written to pw (host [L], NULL: not written); fmod (host, NULL:
+ * not written) receives -fc, fc the centre of the -6 dB points around the
+ * spectrum's first maximum (MATLAB interp1 over pw(1:ic) and
+ * pw(ic:round(L/2))). Synchronous on stream; allocates its FFT workspace. */
+int ustbx_estimate_modulation_frequency(const float* rf, int32_t L, int64_t traces, double fs, double* fmod,
+                                        double* pw, void* stream, ustbx_status* st);
 
 #ifdef __cplusplus
 }

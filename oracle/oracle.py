@@ -15,7 +15,8 @@ Contents
   ``rx_delay`` (das.m:103-106), ``tx_delay`` (das.m:110-206) with
   ``unified_delay`` (calculate_unified_delay_model.m:14-100),
   ``window`` (apodization.m:172-186), ``rx_apod`` (apodization.m:323-468),
-  ``tx_apod`` (apodization.m:249-320, 472-555), pixel grids
+  ``tx_apod`` (apodization.m:249-320, 472-555), ``scanline_apod``
+  (apodization.m:265-306) with ``filter2_same``, pixel grids
   (linear_scan.m:64-82, sector_scan.m:73-121), element positions
   (linear_array.m:56-61, matrix_array.m:60-70), ``reference_phase``
   (das.m:306-309).
@@ -24,6 +25,8 @@ Contents
   (coherence_factor.m:58-151; python/src/ustb/postprocess/coherence_factor.py:19-50).
 - RF -> IQ in FP64: ``fast_demodulation`` with ``kaiser_lowpass`` and
   ``fir_causal`` (fast_demodulation.m:58-183, 267-275),
+  ``estimate_modulation_frequency`` with ``power_spectrum``
+  (fast_demodulation.m:57-73, power_spectrum.m:1-22),
   ``python_fast_demodulation`` with ``fir_zero_phase``
   (python/src/ustb/preprocess/fast_demodulation.py:24-85).
 
@@ -553,6 +556,43 @@ def tx_apod(px, py, pz, waves, window_kind, f_number=(1.0, 1.0), minimum_apertur
     return out
 
 
+def filter2_same(h, x):
+    """MATLAB filter2(h, x, 'same') = conv2(x, rot90(h, 2), 'same'): the full
+    2-D convolution with zero padding, central part starting at offset
+    floor(size(h)/2) (MATLAB's centring, which differs from scipy's 'same'
+    for even kernel sizes)."""
+    from scipy.signal import convolve2d
+
+    h = np.asarray(h, float)
+    full = convolve2d(np.asarray(x, float), h[::-1, ::-1], mode="full")
+    o0, o1 = h.shape[0] // 2, h.shape[1] // 2
+    return full[o0:o0 + x.shape[0], o1:o1 + x.shape[1]]
+
+
+def scanline_apod(n_depth, n_lines, n_lines_y, n_waves, mla=(1, 1), mla_overlap=(0, 0)):
+    """Scanline (MLA) transmit apodization (apodization.m:265-306) of a linear
+    scan ([N_z, N_x, N_y]) or sector scan ([N_depth, N_az, N_el]): wave
+    i + j*N_x/MLA(1) covers the MLA(1) x MLA(2) block of scan lines starting
+    at ((i-1) MLA(1), (j-1) MLA(2)) (circshift), smoothed by filter2 with a
+    box of (MLA_overlap+1) elements on each axis, and is constant along
+    depth. Returns (W, P) float32 in the scan's pixel order (depth fastest)."""
+    m1, m2 = int(mla[0]), int(mla[1])
+    if n_waves * m1 * m2 != n_lines * n_lines_y:
+        raise ValueError("The number of waves in the sequence does not match with the number of scanlines and set "
+                         "MLA.")
+    A = np.zeros((n_lines, n_lines_y))
+    A[:m1, :m2] = 1.0
+    k = (int(mla_overlap[0]) + 1, int(mla_overlap[1]) + 1)
+    h = np.ones(k) / (k[0] * k[1])
+    B = np.zeros((n_waves, n_lines, n_lines_y))
+    for i in range(n_lines // m1):
+        for j in range(n_lines_y // m2):
+            B[i + j * (n_lines // m1)] = filter2_same(h, np.roll(A, (i * m1, j * m2), axis=(0, 1)))
+    # reshape(permute(B .* ones(1,1,1,N_z), [4 2 3 1]), [P, W]): depth fastest, then x, then y
+    out = np.repeat(B.transpose(0, 2, 1).reshape(n_waves, n_lines * n_lines_y)[:, :, None], n_depth, axis=2)
+    return np.ascontiguousarray(out.reshape(n_waves, -1).astype(np.float32))
+
+
 def reference_phase(r_ref, fd, c, flavour="python"):
     """exp(-1j*2*w0*r_ref/c) (das.m:306-309; das.py:164-173), complex128.
 
@@ -763,6 +803,53 @@ def fast_demodulation(rf, fs, t0, fmod, fdown=None, lowpass=(0.5, 1.0)):
     x = np.asarray(rf, dtype=np.float64) * np.exp(-1j * 2.0 * np.pi * fmod * t)
     y = fir_causal(b, x)
     return y[..., ::Ndown], fs / Ndown, t0 - delay, b
+
+
+def power_spectrum(data, fs, normalised=True, n_frames=None):
+    """tools.power_spectrum (power_spectrum.m:1-22): the mean over channels,
+    waves and the first min(100, F) frames of |fftshift(fft(trace))|^2,
+    divided by its maximum; fx = linspace(-fs/2, fs/2, L+1) without its last
+    point. data: (F, W, N, L) real."""
+    x = np.asarray(data, dtype=np.float64)
+    F = x.shape[0]
+    n = min(100, F) if n_frames is None else n_frames
+    X = np.fft.fftshift(np.fft.fft(x[:n], axis=-1), axes=-1)
+    pw = np.mean(np.abs(X) ** 2, axis=(0, 1, 2))
+    if normalised:
+        pw = pw / np.max(pw)
+    L = x.shape[-1]
+    fx = np.linspace(-fs / 2, fs / 2, L + 1)[:-1]
+    return fx, pw
+
+
+def _interp1_unsorted(x, v, xq):
+    """MATLAB interp1(x, v, xq) for distinct, unsorted sample points x: the
+    points are sorted first, then linear interpolation (NaN outside)."""
+    x = np.asarray(x, float)
+    v = np.asarray(v, float)
+    o = np.argsort(x, kind="stable")
+    xs, vs = x[o], v[o]
+    if np.any(np.diff(xs) == 0):
+        raise ValueError("The sample points must be unique.")
+    if xq < xs[0] or xq > xs[-1]:
+        return float("nan")
+    return float(np.interp(xq, xs, vs))
+
+
+def estimate_modulation_frequency(data, fs):
+    """The modulation frequency fast_demodulation estimates when none is set
+    (fast_demodulation.m:57-73): the power spectrum's maximum (the first, in
+    the negative half), the -6 dB points below it (interp1 over pw(1:ic)) and
+    above it up to the middle (pw(ic:round(end/2))), their centre fc, and
+    modulation_frequency = -fc."""
+    fx, pw = power_spectrum(data, fs)
+    ic = int(np.argmax(pw))                      # [dc, ic] = max(pw) (first maximum)
+    dc = pw[ic]
+    L = pw.size
+    mid = int(math.floor(L / 2 + 0.5))           # round(end/2), 1-based
+    lo = _interp1_unsorted(pw[:ic + 1], fx[:ic + 1], dc / 2)
+    up = _interp1_unsorted(pw[ic:mid], fx[ic:mid], dc / 2)
+    return -(lo + up) / 2
 
 
 def python_fast_demodulation(rf, fs, t0, fc, fdown=None, lowpass=(0.5, 1.0)):

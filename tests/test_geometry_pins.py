@@ -449,3 +449,78 @@ def test_reference_phase_flavours_differ_by_the_single_precision_argument():
     ulp = np.spacing(arg.astype(np.float32)).astype(float)
     assert d.max() > 1e-5                      # the flavours do differ here
     assert np.all(d <= 2.0 * ulp + 3e-7)       # by at most three FP32 roundings of the argument
+
+
+# ---------------------------------------------------------------------------
+# scanline (MLA) transmit apodization (apodization.m:265-306)
+# ---------------------------------------------------------------------------
+def test_filter2_same_centring():
+    """MATLAB's 'same' part of a convolution starts at offset floor(k/2):
+    filter2(ones(2)/4, eye(3), 'same') is [.5 .25 0; .25 .5 .25; 0 .25 .25]
+    (rows and columns 2..4 of the full 4 x 4 result), not scipy's rows 1..3;
+    for an odd, symmetric kernel it is scipy's correlate2d 'same'."""
+    from scipy.signal import correlate2d
+
+    got = O.filter2_same(np.ones((2, 2)) / 4, np.eye(3))
+    np.testing.assert_allclose(got, [[0.5, 0.25, 0.0], [0.25, 0.5, 0.25], [0.0, 0.25, 0.25]], atol=1e-15)
+    x = np.random.default_rng(0).standard_normal((7, 5))
+    h = np.ones((3, 3)) / 9
+    np.testing.assert_allclose(O.filter2_same(h, x), correlate2d(x, h, mode="same"), atol=1e-14)
+
+
+def check_scanline(scanline_apod):
+    """Closed forms, depth-constant weights in the scan's pixel order:
+    - MLA 1, no overlap: wave w is 1 on scan line w, 0 elsewhere;
+    - MLA 2: wave w is 1 on lines 2w and 2w+1 (each pixel in exactly one
+      wave);
+    - MLA 2 with overlap 1 (a 2-line box filter, 'same' from offset 1): the
+      weight of line i in wave w is |{2w, 2w+1} and {i, i+1}| / 2;
+    - 2 x 2 MLA on a 3-D scan (4 x 4 lines): wave i + 2j covers lines
+      (2i..2i+1) x (2j..2j+1)."""
+    nz = 3
+    a = scanline_apod(nz, 5, 1, 5).reshape(5, 5, nz)
+    np.testing.assert_array_equal(a, np.repeat(np.eye(5)[:, :, None], nz, axis=2))
+    b = scanline_apod(nz, 8, 1, 4, mla=(2, 1)).reshape(4, 8, nz)
+    lines = np.arange(8)
+    for w in range(4):
+        np.testing.assert_array_equal(b[w, :, 0], (lines // 2 == w).astype(float))
+    assert np.all(b.sum(axis=0) == 1.0)
+    c = scanline_apod(nz, 8, 1, 4, mla=(2, 1), mla_overlap=(1, 0)).reshape(4, 8, nz)
+    for w in range(4):
+        cover = np.array([len({2 * w, 2 * w + 1} & {i, i + 1}) / 2 for i in lines])
+        np.testing.assert_array_equal(c[w, :, 0], cover)
+        assert np.all(c[w] == c[w, :, :1])
+    d = scanline_apod(2, 4, 4, 4, mla=(2, 2)).reshape(4, 4, 4, 2)  # (w, y, x, z)
+    for i in range(2):
+        for j in range(2):
+            want = np.zeros((4, 4))
+            want[2 * j:2 * j + 2, 2 * i:2 * i + 2] = 1.0
+            np.testing.assert_array_equal(d[i + 2 * j, :, :, 0], want)
+
+
+def test_scanline_apodization():
+    check_scanline(O.scanline_apod)
+    with pytest.raises(ValueError, match="does not match with the number of scanlines"):
+        O.scanline_apod(4, 8, 1, 3, mla=(2, 1))
+
+
+@pytest.mark.parametrize("old,new", [
+    ("o0, o1 = h.shape[0] // 2, h.shape[1] // 2", "o0, o1 = (h.shape[0] - 1) // 2, (h.shape[1] - 1) // 2"),
+])
+def test_scanline_pins_reject_planted_mistakes(old, new):
+    bad = mutant(O.filter2_same, old, new)
+    saved = O.filter2_same
+    try:
+        O.filter2_same = bad
+        assert rejects(check_scanline, O.scanline_apod)
+    finally:
+        O.filter2_same = saved
+
+
+@pytest.mark.parametrize("old,new", [
+    ("B[i + j * (n_lines // m1)] =", "B[j + i * (n_lines_y // m2)] ="),
+    ("np.roll(A, (i * m1, j * m2), axis=(0, 1))", "np.roll(A, (i, j), axis=(0, 1))"),
+    (".transpose(0, 2, 1)", ".transpose(0, 1, 2)"),
+])
+def test_scanline_pins_reject_planted_mistakes_in_the_layout(old, new):
+    assert rejects(check_scanline, mutant(O.scanline_apod, old, new))

@@ -346,3 +346,114 @@ def test_receive_table_rejected_modes(cuda):
     plan = _plan(case, dim=3, separable_phase=False)
     assert not plan.has_variant(2)
     plan.close()
+
+
+@pytest.mark.parametrize("name", ["dw_sector", "cpwc_tukey", "pw_rf_hanning", "focused_hybrid", "cpwc_depth0"])
+@pytest.mark.parametrize("separable", [True, False])
+@pytest.mark.parametrize("interp", [Interpolation.linear, Interpolation.cubic])
+def test_transmit_staging_equals_gathers(cuda, name, separable, interp):
+    """Dimension transmit: sample windows staged through LDS give the same
+    per-channel images as one gather per term, bit for bit, for every tile
+    shape (frames x channels per block), 9 frames (a partial frame chunk);
+    NaN-weight tiles (cpwc_depth0, focused_hybrid) run the gather body."""
+    case = CASES[name]
+    F = 9
+    print(f"case staging {name} separable={separable} interp={int(interp)}", flush=True)
+    plan = _plan(case, dim=2, F=F, separable_phase=separable, interpolation=interp)
+    assert plan.variant[0] == 2
+    ch = torch.from_numpy(channel_noise(case, F=F, seed=21)).to(cuda)
+    for fc, cs in ((8, 8), (4, 4), (8, 4), (4, 8)):
+        plan.set_tiles(fc, cs)
+        plan.set_staging(False)
+        a = plan.run(ch).cpu().numpy()
+        plan.set_staging(True)
+        b = plan.run(ch).cpu().numpy()
+        assert np.array_equal(a, b, equal_nan=True), (fc, cs, np.nanmax(np.abs(a - b)))
+    plan.close()
+
+
+@pytest.mark.parametrize("name", ["cpwc_tukey", "dw_sector", "pw_rf_hanning", "cpwc_depth0", "matrix3d_dw",
+                                  "focused_hybrid", "sta_hybrid"])
+@pytest.mark.parametrize("interp", [Interpolation.linear, Interpolation.cubic])
+def test_receive_state_on_the_fly(cuda, name, interp):
+    """Variant 4 (receive delays and weights computed in the kernel, no
+    receive table: the FP64 delay das.m rounds, the window in FP32) against
+    variant 2 (the resident table of das.m's FP32 matrices) within 2e-6 of the
+    peak with identical NaN positions, and against the oracle's FP64
+    restatement on its matrices (the separable phase's reference; das_c's
+    own FP32 phase rounding on noise data is larger) within 1e-5; F = 3."""
+    case = CASES[name]
+    F = 3
+    print(f"case otf {name} interp={int(interp)}", flush=True)
+    plan = _plan(case, dim=3, F=F, separable_phase=True, interpolation=interp)
+    if any(abs(t) > 0 for t in case["rx"].tilt):  # the table only
+        assert not plan.has_variant(4)
+        plan.close()
+        return
+    ch = channel_noise(case, F=F, seed=31)
+    chd = torch.from_numpy(ch).to(cuda)
+    plan.set_variant(2)
+    assert plan.variant[0] == 2
+    tab = plan.run(chd).cpu().numpy()
+    plan.set_variant(4)
+    assert plan.variant[0] == 4
+    otf = plan.run(chd).cpu().numpy()
+    assert _rel(otf, tab) <= 2e-6
+    ta, ra, td, rd = oracle_matrices(case)
+    ref = O.beamform_fp64(ch, case["fs"], 0.0, ta, ra, td, rd, case["fd"], 3,
+                          interp="cubic" if interp == Interpolation.cubic else "linear", w0=O.dasc_wd(case["fd"]),
+                          index="fp32")
+    assert _rel(otf, ref) <= TOL
+    plan.set_variant(2)
+    assert np.array_equal(plan.run(chd).cpu().numpy(), tab, equal_nan=True)  # back to the table
+    plan.close()
+
+
+@pytest.mark.parametrize("kind,mla,overlap", [("linear", (1, 1), (0, 0)), ("linear", (2, 1), (1, 0)),
+                                               ("sector", (4, 1), (2, 0))])
+def test_scanline_transmit_apodization(cuda, kind, mla, overlap):
+    """Scanline (MLA) transmit apodization built by the plan (apodization.m:
+    265-306): the plan's FP32 transmit weights equal the oracle's
+    scanline_apod exactly, and the image equals das_c on the oracle's
+    matrices within 1e-5 (RF, das_c's order: bit for bit)."""
+    from ustb_amd.enums import Window
+    from ustb_amd.uff import Apodization, LinearArray, LinearScan, SectorScan, Wave
+
+    print(f"case scanline {kind} mla={mla} overlap={overlap}", flush=True)
+    prb = LinearArray(16, 0.3e-3)
+    if kind == "linear":
+        scan = LinearScan(np.linspace(-3e-3, 3e-3, 12), np.linspace(4e-3, 20e-3, 40))
+        seq = [Wave.plane(0.0) for _ in range(12 // mla[0])]
+        case = dict(probe=prb, scan=scan, seq=seq, model=3, scan_kind="linear")
+    else:
+        scan = SectorScan(np.deg2rad(np.linspace(-20, 20, 16)), np.linspace(3e-3, 25e-3, 30))
+        seq = [Wave.spherical(0.0, 0.0, -5e-3) for _ in range(16 // mla[0])]
+        case = dict(probe=prb, scan=scan, seq=seq, model=3, scan_kind="sector")
+    case.update(rx=Apodization(Window.hanning, f_number=1.5), tx=Apodization(), fs=20e6, fd=0.0, L=500)
+    tx = Apodization(Window.scanline, MLA=mla, MLA_overlap=overlap)
+    plan = make_plan(prb, scan, seq, n_samples=case["L"], n_frames=1, sampling_frequency=case["fs"],
+                     initial_time=0.0, modulation_frequency=0.0, sound_speed=1540.0,
+                     receive_apodization=case["rx"], transmit_apodization=tx, dimension=3, dasc_order=True)
+    ta, ra, td, rd = [t.cpu().numpy() for t in plan.materialize()]
+    nz, nl = scan.shape[0], scan.shape[1]
+    want = O.scanline_apod(nz, nl, 1, len(seq), mla, overlap)
+    np.testing.assert_array_equal(ta, want)
+    _, ora, otd, ord_ = oracle_matrices(case)
+    assert np.array_equal(td, otd) and np.array_equal(rd, ord_)
+    ch = channel_noise(case, seed=41)
+    got = plan.run(torch.from_numpy(ch).to(cuda)).cpu().numpy()
+    ref = O.das_c(ch, case["fs"], 0.0, want, ora, otd, ord_, 0.0, 3)
+    assert np.array_equal(got, ref, equal_nan=True)
+    plan.close()
+
+
+def test_scanline_wave_count_mismatch_is_refused(cuda):
+    from ustb_amd import _lib as L
+    from ustb_amd.enums import Window
+    from ustb_amd.uff import Apodization, LinearArray, LinearScan, Wave
+
+    with pytest.raises(L.USTBXError, match="does not match with the number of scanlines"):
+        make_plan(LinearArray(8, 0.3e-3), LinearScan(np.linspace(-1e-3, 1e-3, 8), np.linspace(5e-3, 9e-3, 16)),
+                  [Wave.plane(0.0)] * 3, n_samples=200, n_frames=1, sampling_frequency=20e6, initial_time=0.0,
+                  modulation_frequency=0.0, sound_speed=1540.0,
+                  transmit_apodization=Apodization(Window.scanline, MLA=(2, 1)))

@@ -18,6 +18,7 @@ from oracle import oracle as O
 from ustb_amd import _lib as L
 from ustb_amd import ops
 from ustb_amd.preprocess import FastDemodulation, matlab_lowpass, python_lowpass
+from ustb_amd.uff import ChannelData
 
 pytestmark = pytest.mark.gpu
 
@@ -113,3 +114,54 @@ def test_fast_demodulation_facade_then_das_focuses(cuda):
     img = d.go().data.cpu().numpy()[:, 0, 0, 0]
     k = int(np.argmax(np.abs(img)))
     assert abs(d.scan.x[k] - 1e-3) <= 2e-3 and abs(d.scan.z[k] - 20e-3) <= 2e-3
+
+
+def _pulses(f0, fs, L, seed, shape=(3, 4, 8), sigma=0.4e-6):
+    rng = np.random.default_rng(seed)
+    t = np.arange(L) / fs
+    tau = rng.uniform(0.2 * L / fs, 0.6 * L / fs, shape)[..., None]
+    x = np.exp(-((t - tau) / sigma) ** 2) * np.cos(2 * np.pi * f0 * (t - tau))
+    return (x + 0.01 * rng.standard_normal(x.shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("f0,fs,L", [(5.2e6, 41.6e6, 1024), (3.1e6, 25e6, 1000), (7.0e6, 50e6, 777)])
+def test_modulation_frequency_estimate_matches_oracle(cuda, f0, fs, L):
+    """ustbx_estimate_modulation_frequency against the oracle's restatement
+    (fast_demodulation.m:57-73, power_spectrum.m): the normalised power
+    spectrum within 1e-5 (FP32 FFT against FP64), the estimate within a
+    hundredth of a bin, and within half a bin of the pulses' centre
+    frequency (the linear interpolation of the -6 dB points between bins of
+    a Gaussian spectrum; odd and even L)."""
+    print(f"case f0 estimate f0={f0} fs={fs} L={L}", flush=True)
+    rf = _pulses(f0, fs, L, seed=int(f0) % 97)
+    est, fx, pw = ops.estimate_modulation_frequency(torch.from_numpy(rf).to(cuda), fs, return_spectrum=True)
+    ofx, opw = O.power_spectrum(rf.astype(np.float64), fs)
+    np.testing.assert_allclose(pw, opw, atol=1e-5)
+    np.testing.assert_allclose(fx, ofx, rtol=0, atol=1e-6 * fs)
+    ref = O.estimate_modulation_frequency(rf.astype(np.float64), fs)
+    assert abs(est - ref) < 0.01 * fs / L, (est, ref)
+    assert abs(est - f0) < 0.5 * fs / L, (est, f0)
+
+
+def test_fast_demodulation_estimates_a_missing_modulation_frequency(cuda):
+    """FastDemodulation without a modulation frequency takes the estimate
+    (warning as the MATLAB class does) and then demodulates exactly as with
+    that frequency set."""
+    fs, f0 = 41.6e6, 5.2e6
+    rf = _pulses(f0, fs, 1024, seed=3)
+    cd = ChannelData(probe=None, sequence=[None] * rf.shape[1], sampling_frequency=fs,
+                     device_data=torch.from_numpy(rf).to(cuda))
+    a = FastDemodulation()
+    a.input = cd
+    a.downsample_frequency = 10.4e6
+    with pytest.warns(UserWarning, match="modulation frequency is not specified"):
+        out_a = a.go()
+    est = a.modulation_frequency
+    assert abs(est - f0) < 0.5 * fs / 1024
+    b = FastDemodulation()
+    b.input = cd
+    b.downsample_frequency = 10.4e6
+    b.modulation_frequency = est
+    out_b = b.go()
+    assert torch.equal(out_a.device_data, out_b.device_data)
+    assert out_a.modulation_frequency == est

@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_device_count():
-    assert L.lib().ustbx_abi_version() == 1
+    assert L.lib().ustbx_abi_version() == 2
     assert L.lib().ustbx_device_count() >= 0
 
 

@@ -65,7 +65,7 @@ class Apod(ctypes.Structure):
                 ("minimum_aperture", c_double * 2), ("maximum_aperture", c_double * 2),
                 ("tilt", c_double * 2), ("grating_lobe_angle", c_double * 2),
                 ("has_origin", c_int32), ("_pad2", c_int32), ("origin", c_double * 3),
-                ("vector", P_float), ("matrix", P_float)]
+                ("vector", P_float), ("matrix", P_float), ("mla", c_int32 * 2), ("mla_overlap", c_int32 * 2)]
 
 
 class Geometry(ctypes.Structure):
@@ -107,6 +107,7 @@ EXPORTS = {
     "ustbx_plan_set_variant": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
     "ustbx_plan_variant": (c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32)]),
     "ustbx_plan_set_order": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
+    "ustbx_plan_set_staging": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
     "ustbx_plan_materialize": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                               c_void_p, ctypes.POINTER(Status)]),
     "ustbx_plan_active_terms": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(Status)]),
@@ -144,6 +145,8 @@ EXPORTS = {
                                          c_int64, c_void_p, ctypes.POINTER(Status)]),
     "ustbx_plan_active_elements": (ctypes.c_int, [c_void_p, c_int32, ctypes.c_double, c_void_p, c_void_p,
                                                   ctypes.POINTER(Status)]),
+    "ustbx_estimate_modulation_frequency": (ctypes.c_int, [c_void_p, c_int32, c_int64, ctypes.c_double,
+                                                           c_void_p, c_void_p, c_void_p, ctypes.POINTER(Status)]),
     "ustbx_demodulate": (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, c_void_p, c_int32, c_int32, c_int32, c_void_p,
                                         ctypes.POINTER(Status)]),
@@ -156,7 +159,7 @@ class USTBXError(RuntimeError):
     """An error returned by the C-ABI, carrying the reference identifier."""
 
     def __init__(self, code: int, identifier: str, message: str):
-        super().__init__(f"{identifier}: {message}")
+        super().__init__(f"{identifier}: {message}" if identifier else message)
         self.code = code
         self.identifier = identifier
 

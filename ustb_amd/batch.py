@@ -23,7 +23,7 @@ class Batch:
             raise ValueError("a batch needs at least one plan")
         for p in plans:
             if p.variant[0] < 2:
-                p.set_variant(2)
+                p.set_variant(2)  # variant 4 (the receive state on the fly) is kept
         self.plans = list(plans)
         self.device = plans[0].device
         arr = (ctypes.c_void_p * len(plans))(*[p.handle.value for p in plans])

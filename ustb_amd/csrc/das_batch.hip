@@ -61,6 +61,7 @@ __global__ void batch_reset_kernel(int* ctr) {
 template <int SM, int WC, int WS>
 __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
   __shared__ float2 red[WS][64];
+  __shared__ float4 rxs[kOtfChunk][64];  // receive entries of on-the-fly acquisitions
   __shared__ int claimed;
   const int home = (int)(blockIdx.x % kQueues);  // blocks reach the XCDs round-robin
   for (;;) {
@@ -83,10 +84,17 @@ __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
     if (it < 0) break;  // every queue is empty
     const __attribute__((address_space(4))) int* w = (const __attribute__((address_space(4))) int*)B.items + 4 * it;
     const int acq = w[0], tile = w[1], frame = w[2];
-    if (w[3])  // a nonzero weight or its delay not finite on this tile
+    const __attribute__((address_space(4))) RxAcq* ad = (const __attribute__((address_space(4))) RxAcq*)(B.acqs + acq);
+    if (ad->rx_otf) {  // the receive state computed in the block
+      if (w[3])
+        rxotf_tile<SM, WC, WS, true>(load_constant(B.acqs + acq), tile, frame, red, rxs);
+      else
+        rxotf_tile<SM, WC, WS, false>(load_constant(B.acqs + acq), tile, frame, red, rxs);
+    } else if (w[3]) {  // a nonzero weight or its delay not finite on this tile
       rxtab_tile<SM, WC, WS, true>(load_constant(B.acqs + acq), tile, frame, red);
-    else
+    } else {
       rxtab_tile<SM, WC, WS, false>(load_constant(B.acqs + acq), tile, frame, red);
+    }
     __syncthreads();  // all threads are past this item before the next claim
   }
 }

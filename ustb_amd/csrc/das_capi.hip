@@ -387,7 +387,28 @@ std::vector<int> band_major_order(const DevScan& sc, int tiles, int shares) {
   return order;
 }
 
-bool rxtab_applies(const ustbx_plan* pl) {
+// Receive tables above this size are replaced by the on-the-fly receive
+// state by default (dimension both, where it applies).
+constexpr size_t kRxTableOtfBytes = (size_t)2 << 30;
+
+// The on-the-fly receive state covers dimension both with a windowed (or no)
+// receive apodization without tilt; variant 3 (coefficients) keeps the table.
+bool otf_applies(const ustbx_plan* pl) {
+  const GeomLaunch& G = pl->G;
+  if (G.dim != 3) return false;
+  const DevRxApod& r = G.rx;
+  if (r.kind == RXAP_ONES) return true;
+  if (r.kind != RXAP_WINDOW || r.tilt != 0) return false;
+  switch (r.window) {
+    case WIN_BOXCAR: case WIN_HANNING: case WIN_HAMMING: case WIN_TUKEY25: case WIN_TUKEY50: case WIN_TUKEY75:
+    case WIN_TRIANGLE:
+      return true;
+    default:
+      return false;
+  }
+}
+
+bool rxtab_applies(const ustbx_plan* pl, bool otf) {
   const GeomLaunch& G = pl->G;
   if (!(G.dim == 3 || G.dim == 2)) return false;
   // dimension both needs the separable (or RF) phase outside das_c's order;
@@ -396,31 +417,60 @@ bool rxtab_applies(const ustbx_plan* pl) {
   if (G.edge == USTBX_EDGE_PYTHON && G.interp == USTBX_INTERP_LINEAR) return false;
   // dimension transmit gathers up to 8 frames through one buffer resource
   if (G.dim == 2 && (int64_t)G.L * G.N * G.W * 8 * 8 > 0x7FFFFE00) return false;
+  if (otf && !otf_applies(pl)) return false;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
-  if ((size_t)G.N * (size_t)G.P * sizeof(float4) > (size_t)(kRxTableMaxFraction * (double)total_b)) return false;
+  if (!otf && (size_t)G.N * (size_t)G.P * sizeof(float4) > (size_t)(kRxTableMaxFraction * (double)total_b))
+    return false;
   return (int64_t)G.L * G.N * G.W * 8 <= 0x7FFFFE00;
+}
+
+// Whether a new plan takes the on-the-fly receive state: where it applies
+// and the receive table would exceed kRxTableOtfBytes.
+bool otf_default(const ustbx_plan* pl) {
+  const GeomLaunch& G = pl->G;
+  return otf_applies(pl) && (size_t)G.N * (size_t)G.P * sizeof(float4) > kRxTableOtfBytes;
 }
 
 // Variant 3 adds a coefficient buffer of 32 bytes per channel sample (cubic).
 bool coef_applies(const ustbx_plan* pl) {
   const GeomLaunch& G = pl->G;
-  return G.dim == 3 && G.interp == USTBX_INTERP_CUBIC && (int64_t)G.L * G.N * G.W * 32 <= 0x7FFFFE00;
+  return G.dim == 3 && G.interp == USTBX_INTERP_CUBIC && !G.rx_otf && (int64_t)G.L * G.N * G.W * 32 <= 0x7FFFFE00;
 }
 
-// Builds the variant-2 state once; returns false (and frees it) when the
-// variant does not apply. Tiles where a nonzero weight or its delay is not
-// finite go last in both visiting orders, for the kernels' SAFE form.
-bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
+void rxtab_free(ustbx_plan* pl) {
+  (void)hipFree(pl->rxtab);
+  pl->rxtab = nullptr;
+  pl->rxtab_bytes = 0;
+  pl->rxtab_state = 0;
+  GeomLaunch& G = pl->G;
+  G.rx_table = nullptr;
+  G.tile_nc = G.tile_ncf = nullptr;
+  G.nan_mask = nullptr;
+  G.tile_chl = G.tile_waves = G.tile_nw = nullptr;
+  G.ref_table = nullptr;
+  G.tile_order = nullptr;
+  pl->tile_order = pl->tile_natural = nullptr;
+}
+
+// Builds the variant-2 state (otf: with the receive state on the fly instead
+// of the receive table) once, or again in the other receive mode; returns
+// false (and frees it) when that form does not apply. Tiles where a nonzero
+// weight or its delay is not finite go last in both visiting orders, for the
+// kernels' SAFE form.
+bool rxtab_build(ustbx_plan* pl, hipError_t& err, bool otf) {
   err = hipSuccess;
-  if (pl->rxtab_state != 0) return pl->rxtab_state > 0;
+  if (pl->rxtab_state > 0 && (pl->G.rx_otf != 0) == otf) return true;
+  if (pl->rxtab_state > 0 || (pl->rxtab_state < 0 && (pl->G.rx_otf != 0) != otf)) rxtab_free(pl);
+  if (pl->rxtab_state != 0) return false;
   pl->rxtab_state = -1;
-  if (!rxtab_applies(pl)) return false;
+  pl->G.rx_otf = otf ? 1 : 0;
+  if (!rxtab_applies(pl, otf)) return false;
   GeomLaunch& G = pl->G;
   const size_t P = (size_t)G.P, tiles = (P + 63) / 64;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   G.rx_compact = G.wd == 0.0f ? 1 : 0;  // RF: {rd, ra} (the phase factor is (ra, 0))
-  const size_t b_rx = al((G.rx_compact ? sizeof(float2) : sizeof(float4)) * (size_t)G.N * P);
+  const size_t b_rx = otf ? 0 : al((G.rx_compact ? sizeof(float2) : sizeof(float4)) * (size_t)G.N * P);
   const size_t b_nc = al(sizeof(int) * 2 * tiles);  // channel counts, then counts of the non-NaN ones
   const size_t b_cl = al(sizeof(int) * tiles * (size_t)G.N);
   const size_t b_wl = al(sizeof(int) * tiles * (size_t)G.W);
@@ -429,7 +479,8 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   const size_t b_ref = G.ref_phase ? al(sizeof(float2) * P) : 0;
   const size_t b_ord = al(sizeof(int) * tiles);
   const size_t b_mask = al(sizeof(unsigned long long) * tiles);
-  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + 2 * b_ord + b_mask;
+  const size_t b_otf = otf ? al(sizeof(RxOtf)) : 0;
+  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + 2 * b_ord + b_mask + b_otf;
   char* m = nullptr;
   err = hipMalloc(&m, total);
   if (err != hipSuccess) return false;
@@ -439,7 +490,7 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
     off += b;
     return q;
   };
-  float4* rx = (float4*)take(b_rx);
+  float4* rx = b_rx ? (float4*)take(b_rx) : nullptr;
   int* tnc = (int*)take(b_nc);
   int* tcl = (int*)take(b_cl);
   int* twl = (int*)take(b_wl);
@@ -449,7 +500,10 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err) {
   int* ord = (int*)take(b_ord);
   int* nat = (int*)take(b_ord);
   unsigned long long* nmask = (unsigned long long*)take(b_mask);
-  err = launch_rxtab_tables(G, rx, ref, tnc, tcl, twl, tnw, tbad, nmask, nullptr);
+  RxOtf* dotf = b_otf ? (RxOtf*)take(b_otf) : nullptr;
+  if (dotf) err = hipMemcpy(dotf, &G.otf, sizeof(RxOtf), hipMemcpyHostToDevice);
+  G.d_otf = dotf;
+  if (err == hipSuccess) err = launch_rxtab_tables(G, rx, ref, tnc, tcl, twl, tnw, tbad, nmask, nullptr);
   if (err == hipSuccess) err = hipDeviceSynchronize();
   std::vector<int> nw(tiles), nc(tiles), bad(tiles);
   if (err == hipSuccess) err = hipMemcpy(nw.data(), tnw, sizeof(int) * tiles, hipMemcpyDeviceToHost);
@@ -527,6 +581,62 @@ struct Arena {
     return off;
   }
 };
+
+// MATLAB's filter2(h, x, 'same') for a box h of k0 x k1 ones / (k0 k1) on an
+// n0 x n1 array (column-major): the full correlation with zero padding, its
+// central part from offset floor(k/2) on each axis.
+std::vector<double> box_filter_same(const std::vector<double>& x, int n0, int n1, int k0, int k1) {
+  std::vector<double> y((size_t)n0 * n1, 0.0);
+  const double w = 1.0 / ((double)k0 * k1);
+  for (int j = 0; j < n1; ++j)
+    for (int i = 0; i < n0; ++i) {
+      double acc = 0.0;
+      // conv2(x, rot90(h, 2), 'same')(i, j) = sum of x over rows
+      // i + floor(k0/2) - k0 + 1 .. i + floor(k0/2), likewise columns
+      for (int a = i + k0 / 2 - k0 + 1; a <= i + k0 / 2; ++a)
+        for (int b = j + k1 / 2 - k1 + 1; b <= j + k1 / 2; ++b)
+          if (a >= 0 && a < n0 && b >= 0 && b < n1) acc += x[(size_t)a + (size_t)n0 * b];
+      y[(size_t)i + (size_t)n0 * j] = acc * w;
+    }
+  return y;
+}
+
+// Scanline (MLA) transmit apodization (apodization.m:265-306), [W][P] in the
+// scan's pixel order: wave i + j * n1 / MLA(1) holds the MLA(1) x MLA(2)
+// block of scan lines from ((i-1) MLA(1), (j-1) MLA(2)), smoothed by
+// filter2 with a box of MLA_overlap + 1 lines per axis, along all depths.
+// Linear scans: lines (x, y); sector scans: (azimuth, elevation).
+int scanline_matrix(const ustbx_scan& sc, int W, const int32_t mla[2], const int32_t ov[2],
+                    std::vector<float>& out, ustbx_status* st) {
+  if (sc.kind != USTBX_SCAN_LINEAR && sc.kind != USTBX_SCAN_SECTOR)
+    return set_status(st, USTBX_E_ARG,
+                      "The scan class does not support scanline based beamforming. This must be done manually, "
+                      "defining several scan and setting the apodization to uff.window.none.");
+  const int m1 = mla[0] > 0 ? mla[0] : 1, m2 = mla[1] > 0 ? mla[1] : 1;
+  const int k0 = (ov[0] > 0 ? ov[0] : 0) + 1, k1 = (ov[1] > 0 ? ov[1] : 0) + 1;
+  const int nz = sc.n0, nx = sc.n1, ny = sc.n2;
+  if ((int64_t)W * m1 * m2 != (int64_t)nx * ny || nx % m1 || ny % m2)
+    return set_status(st, USTBX_E_DIMENSIONS,
+                      "The number of waves in the sequence does not match with the number of scanlines and set MLA.");
+  const size_t P = (size_t)nz * nx * ny;
+  out.assign((size_t)W * P, 0.0f);
+  std::vector<double> a((size_t)nx * ny);
+  for (int i = 0; i < nx / m1; ++i)
+    for (int j = 0; j < ny / m2; ++j) {
+      // circshift of the block at (0, 0) by ((i-1) MLA(1), (j-1) MLA(2))
+      std::fill(a.begin(), a.end(), 0.0);
+      for (int u = 0; u < m1; ++u)
+        for (int v = 0; v < m2; ++v) a[(size_t)((i * m1 + u) % nx) + (size_t)nx * ((j * m2 + v) % ny)] = 1.0;
+      const std::vector<double> b = box_filter_same(a, nx, ny, k0, k1);
+      float* o = out.data() + (size_t)(i + j * (nx / m1)) * P;
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          const float v = (float)b[(size_t)x + (size_t)nx * y];
+          for (int z = 0; z < nz; ++z) o[(size_t)z + (size_t)nz * (x + (size_t)nx * y)] = v;
+        }
+    }
+  return USTBX_OK;
+}
 
 int check_apod(const ustbx_apod& a, const char* which, ustbx_status* st) {
   if (a.window < 0 || a.window > 9)
@@ -613,8 +723,8 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     rx.kind = RXAP_VECTOR;
     rvec.resize(N);
     for (int n = 0; n < N; ++n) rvec[n] = dist[n] == mn ? 1.0f : 0.0f;
-  } else if (ra.window == USTBX_WIN_SCANLINE) {
-    return set_status(st, USTBX_E_UNSUPPORTED, "Unknown apodization type!");
+  } else if (ra.window == USTBX_WIN_SCANLINE) {  // a wave apodization only (apodization.m:192-220)
+    return set_status(st, USTBX_E_ARG, "Unknown apodization type!");
   } else {
     rx.kind = RXAP_WINDOW;
     rx.window = ra.window;
@@ -652,8 +762,11 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   tx.gl0 = 0.5 * ta.grating_lobe_angle[0];
   tx.gl1 = 0.5 * ta.grating_lobe_angle[1];
   tx.mat = ta.matrix;
-  if (!ta.matrix && ta.window == USTBX_WIN_SCANLINE)
-    return set_status(st, USTBX_E_UNSUPPORTED, "scanline transmit apodization needs an explicit matrix");
+  std::vector<float> scanline;  // the scanline apodization, computed here and uploaded with the plan
+  if (!ta.matrix && ta.window == USTBX_WIN_SCANLINE) {
+    rc = scanline_matrix(sc, W, ta.mla, ta.mla_overlap, scanline, st);
+    if (rc) return rc;
+  }
   if (!ta.matrix && ta.window == USTBX_WIN_STA)
     return set_status(st, USTBX_E_UNSUPPORTED, "Unknown apodization type!");
 
@@ -734,7 +847,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
       d.os_el = atan2(OS[1], OS[2]);
     }
     // apodization kind
-    if (ta.matrix) {
+    if (ta.matrix || !scanline.empty()) {
       d.apk = TXAP_MATRIX;
     } else if (ta.vector) {
       d.apk = TXAP_CONST;
@@ -754,6 +867,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     }
   }
   const size_t off_waves = ar.add(waves.data(), sizeof(DevWave) * W);
+  const size_t off_tmat = scanline.empty() ? SIZE_MAX : ar.add(scanline.data(), sizeof(float) * scanline.size());
 
   // ---- upload ----
   ustbx_plan* pl = new (std::nothrow) ustbx_plan();  // value-initialized
@@ -806,6 +920,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   G.ez = (const double*)(base + off_ez);
   G.waves = (const DevWave*)(base + off_waves);
   if (off_rvec != SIZE_MAX) rx.vec = (const float*)(base + off_rvec);
+  if (off_tmat != SIZE_MAX) tx.mat = (const float*)(base + off_tmat);
   G.rx = rx;
   G.tx = tx;
   G.tx_delay_mat = g->tx_delay_matrix;
@@ -817,6 +932,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   G.wave_chunk = dasc_order ? 1 : (G.separable ? (W >= 16 ? 16 : 8) : 8);
   G.split = dasc_order ? 1 : ((G.dim == 3 || G.dim == 2) ? auto_split(P) : 1);
   G.variant = 0;
+  G.stage = 1;
   if (!dasc_order && shared_rx_applies(G)) {
     G.variant = 1;
     shared_rx_default_tiles(W, G.wave_chunk, G.split);
@@ -882,9 +998,34 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     delete pl;
     return hip_fail(st, e, "transmit table");
   }
-  // the receive-table kernel where it applies
+  // the receive-table kernel where it applies, with the receive state on the
+  // fly where the table would be large
   pl->dasc_order = dasc_order;
-  if (rxtab_build(pl, e)) {
+  G.otf.scan = G.scan;
+  G.otf.ex = G.ex;
+  G.otf.ey = G.ey;
+  G.otf.ez = G.ez;
+  G.otf.inv_c = 1.0 / G.c;
+  G.otf.lens = G.lens;
+  G.otf.wd = G.wd;
+  G.otf.compact = G.wd == 0.0f ? 1 : 0;
+  {
+    RxOtfApod& a = G.otf.ap;
+    a.kind = G.rx.kind == RXAP_ONES ? RXAP_ONES : RXAP_WINDOW;
+    a.window = G.rx.window;
+    a.geo = G.rx.geo;
+    a.f0 = G.rx.f0;
+    a.f1 = G.rx.f1;
+    a.min0 = G.rx.min0;
+    a.min1 = G.rx.min1;
+    a.max0 = G.rx.max0;
+    a.max1 = G.rx.max1;
+    a.ox = G.rx.ox;
+    a.oy = G.rx.oy;
+    a.oz = G.rx.oz;
+    a.win0 = (float)G.rx.win0;
+  }
+  if (rxtab_build(pl, e, otf_default(pl))) {
     G.variant = 2;
     rxtab_defaults(G, pl->max_nw, G.wave_chunk, G.split);
   } else if (e != hipSuccess) {
@@ -900,13 +1041,28 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
 
 int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st) {
   if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
-  if (variant < 0 || variant > 3) return set_status(st, USTBX_E_ARG, "variant must be 0, 1, 2 or 3");
+  if (variant < 0 || variant > 4) return set_status(st, USTBX_E_ARG, "variant must be 0, 1, 2, 3 or 4");
+  if (variant == 4) {
+    // variant 2 with the receive state on the fly
+    hipError_t e;
+    if (!rxtab_build(plan, e, true)) {
+      if (e != hipSuccess) return hip_fail(st, e, "receive-state tables");
+      return set_status(st, USTBX_E_UNSUPPORTED,
+                        "the on-the-fly receive state needs dimension both with the separable (or RF) phase, and a "
+                        "windowed receive apodization without tilt");
+    }
+    if (plan->G.variant != 2) {
+      plan->G.variant = 2;
+      rxtab_defaults(plan->G, plan->max_nw, plan->G.wave_chunk, plan->G.split);
+    }
+    return ok(st);
+  }
   if (variant == 1 && (plan->dasc_order || !shared_rx_applies(plan->G)))
     return set_status(st, USTBX_E_UNSUPPORTED,
                       "the shared-receive kernel needs dimension both, IQ data, the separable phase and one frame");
   if (variant >= 2) {
     hipError_t e;
-    if (!rxtab_build(plan, e)) {
+    if (!rxtab_build(plan, e, false)) {
       if (e != hipSuccess) return hip_fail(st, e, "receive table");
       return set_status(st, USTBX_E_UNSUPPORTED,
                         "the receive-table kernels need dimension both with the separable (or RF) phase outside das_c's "
@@ -1075,11 +1231,18 @@ int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st) {
   return ok(st);
 }
 
+int ustbx_plan_set_staging(ustbx_plan* plan, int32_t on, ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  if (on != 0 && on != 1) return set_status(st, USTBX_E_ARG, "staging must be 0 or 1");
+  plan->G.stage = on;
+  return ok(st);
+}
+
 int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split) {
   if (!plan) return -1;
   if (wave_chunk) *wave_chunk = plan->G.wave_chunk;
   if (split) *split = plan->G.split;
-  return plan->G.variant;
+  return (plan->G.variant == 2 && plan->G.rx_otf) ? 4 : plan->G.variant;
 }
 
 void ustbx_plan_destroy(ustbx_plan* plan) {

@@ -118,6 +118,23 @@ __device__ __forceinline__ float4 buf_load16(__amdgpu_buffer_rsrc_t r, int byte_
   return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Minimum (MAX = false) or maximum of v over the 64 lanes of a wave, every
+// lane active: an inclusive scan within each row of 16 lanes (DPP row_shr 1,
+// 2, 4, 8), then row_bcast15 and row_bcast31 carry the row results up, which
+// leaves the wave's result in lane 63; returned wave-uniform.
+template <bool MAX>
+__device__ __forceinline__ unsigned wave_reduce_u32(unsigned v) {
+  constexpr int kId = MAX ? 0 : -1;  // identity: 0 or 0xFFFFFFFF
+  auto step = [&](unsigned t) { v = MAX ? (v > t ? v : t) : (v < t ? v : t); };
+  step((unsigned)__builtin_amdgcn_update_dpp(kId, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
+  step((unsigned)__builtin_amdgcn_update_dpp(kId, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
+  step((unsigned)__builtin_amdgcn_update_dpp(kId, (int)v, 0x114, 0xf, 0xf, false));  // row_shr:4
+  step((unsigned)__builtin_amdgcn_update_dpp(kId, (int)v, 0x118, 0xf, 0xf, false));  // row_shr:8
+  step((unsigned)__builtin_amdgcn_update_dpp(kId, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  step((unsigned)__builtin_amdgcn_update_dpp(kId, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // x0 = X[n0], x1 = X[n0+1] as one 16-byte load (8-byte aligned).
 __device__ __forceinline__ float4 load_pair(const float2* __restrict__ trace, int n0) {
   const f4a8 v = *reinterpret_cast<const f4a8*>(trace + n0);
@@ -412,6 +429,123 @@ __device__ __forceinline__ float rx_apod_f32(const DevRxApod& ap, double px,
                         ? ap.win0
                         : win_eval(ap.window, fabs(ap.f1 * (yd / zy)));
   return (float)(win_eval(ap.window, rt) * wp);
+}
+
+// ---------------------------------------------------------------------------
+// Receive state computed on the fly (no [N][P] table): per (pixel, element)
+// the delay das.m forms (FP64, rounded to FP32: rx_delay_f32, the table's
+// value bit for bit) and the receive weight from the same formulas as
+// rx_apod_f32 (aperture clamps and ratios in FP64, the pixel- or
+// origin-centred aperture; no tilt), the window evaluated in FP32 with the
+// hardware cosine. Weights agree with rx_apod_f32 to FP32 rounding.
+// ---------------------------------------------------------------------------
+struct RxOtfApod {
+  int kind;    // RXAP_ONES or RXAP_WINDOW
+  int window;  // WIN_*
+  int geo;     // RXGEO_*
+  float win0;  // window(0)
+  double f0, f1;
+  double min0, min1, max0, max1;  // minimum / maximum aperture .* f_number
+  double ox, oy, oz;
+};
+
+// The window of ratio r in FP32 (apodization.m:172-186); v_cos_f32 takes
+// revolutions. Non-finite r: 0 for the boxcar, NaN for the others (MATLAB's
+// logical mask times NaN), as win_eval.
+__device__ __forceinline__ float win_eval_f32(int win, float r) {
+  if (!(r <= 3.402823466e38f)) return win == WIN_BOXCAR ? 0.0f : __builtin_nanf("");
+  const bool m = r <= 0.5f;
+  switch (win) {
+    case WIN_BOXCAR: return m ? 1.0f : 0.0f;
+    case WIN_HANNING: return m ? 0.5f + 0.5f * __builtin_amdgcn_cosf(r) : 0.0f;
+    case WIN_HAMMING: return m ? 0.53836f + 0.46164f * __builtin_amdgcn_cosf(r) : 0.0f;
+    case WIN_TUKEY25:
+    case WIN_TUKEY50:
+    case WIN_TUKEY75: {
+      const float roll = win == WIN_TUKEY25 ? 0.25f : (win == WIN_TUKEY50 ? 0.5f : 0.75f);
+      if (r <= 0.5f * (1.0f - roll)) return 1.0f;
+      if (!(r < 0.5f)) return 0.0f;
+      // 0.5 (1 + cos(2 pi / roll (r - roll/2 - 1/2)))
+      return 0.5f * (1.0f + __builtin_amdgcn_cosf((r - (0.5f + 0.5f * roll)) / roll));
+    }
+    case WIN_TRIANGLE: return m ? 1.0f - 2.0f * r : 0.0f;
+    default: return 1.0f;
+  }
+}
+
+__device__ __forceinline__ float aperture_clamp_f32(float z, float min_af, float max_af) {
+  float zd = z;
+  const float sg = z > 0.0f ? 1.0f : (z < 0.0f ? -1.0f : (z == 0.0f ? 0.0f : z));  // MATLAB sign()
+  if (fabsf(z) <= min_af) zd = sg * min_af;
+  if (fabsf(z) >= max_af) zd = sg * max_af;
+  return zd;
+}
+
+// |f * (x / z)|: in FP32 (the hardware reciprocal), except within 1e-5 of
+// 1/2 for the windows that jump there (boxcar, Hamming), where the FP64
+// quotient rx_apod_f32 forms decides the side (a ratio exactly on the edge
+// occurs for aligned pixel and element grids).
+__device__ __forceinline__ float otf_ratio(int window, double f, double x, double z) {
+  const float r = fabsf((float)f * ((float)x * __builtin_amdgcn_rcpf((float)z)));
+  if ((window == WIN_BOXCAR || window == WIN_HAMMING) && fabsf(r - 0.5f) < 1e-5f) {
+#pragma clang fp contract(off)
+    const double r64 = fabs(f * (x / z));
+    return r64 <= 0.5 ? fminf(r, 0.5f) : fmaxf(r, nextafterf(0.5f, 1.0f));
+  }
+  return r;
+}
+
+// Receive weight of distances (xd, yd, zd) (apodization.m:415-467 without
+// the tilt): the clamps in FP64, the ratios by otf_ratio, the window in FP32.
+__device__ __forceinline__ float rx_apod_otf(const RxOtfApod& a, double xd, double yd, double zd) {
+#pragma clang fp contract(off)
+  if (a.kind == RXAP_ONES) return 1.0f;
+  const double zx = aperture_clamp(zd, a.min0, a.max0);
+  const double zy = aperture_clamp(zd, a.min1, a.max1);
+  const float rt = otf_ratio(a.window, a.f0, xd, zx);
+  const float wp = (yd == 0.0 && fabs(zy) > 0.0 && fabs(zy) < INFINITY) ? a.win0
+                                                                         : win_eval_f32(a.window, otf_ratio(a.window, a.f1, yd, zy));
+  return win_eval_f32(a.window, rt) * wp;
+}
+
+// Everything the on-the-fly receive state needs (device pointers).
+struct RxOtf {
+  DevScan scan;
+  const double* ex;
+  const double* ey;
+  const double* ez;
+  double inv_c, lens;
+  float wd;     // das_c's wd (the separable receive factor)
+  int compact;  // RF (wd == 0): the factor is (ra, 0), as the compact table
+  RxOtfApod ap;
+};
+
+// {rd, ra, ra*cos(wd rd), ra*sin(wd rd)} of (pixel (px, py, pz), element n):
+// the receive-table entry computed in place, with das.m's delay
+// single(sqrt(xm^2+ym^2+zm^2)/c + lens) in FP64 (the quotient by c taken as
+// a product with 1/c, which can move the FP32 rounding by one ulp in rare
+// ties) and the weight of rx_apod_otf.
+__device__ __forceinline__ float4 rx_otf_entry(const RxOtf& o, double px, double py, double pz, int n) {
+#pragma clang fp contract(off)
+  const double ex = o.ex[n], ey = o.ey[n], ez = o.ez[n];
+  const double xm = ex - px, ym = ey - py, zm = ez - pz;
+  const float rd = (float)(sqrt((xm * xm + ym * ym) + zm * zm) * o.inv_c + o.lens);
+  double xd, yd, zd;
+  if (o.ap.geo == RXGEO_FLAT) {
+    xd = px - ex;
+    yd = py - ey;
+    zd = pz - ez;
+  } else {
+    const double dx = px - o.ap.ox, dy = py - o.ap.oy, dz = pz - o.ap.oz;
+    xd = ex - o.ap.ox;
+    yd = ey - o.ap.oy;
+    zd = sqrt((dx * dx + dy * dy) + dz * dz);
+  }
+  const float ra = rx_apod_otf(o.ap, xd, yd, zd);
+  if (o.compact) return make_float4(rd, ra, ra, ra * 0.0f);
+  float sv, cv;
+  sincos_wd_hw(o.wd, rd, sv, cv);
+  return make_float4(rd, ra, ra * cv, ra * sv);
 }
 
 // Transmit side. Kinds of tx delay per wave (das.m:110-204).

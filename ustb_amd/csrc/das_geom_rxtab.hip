@@ -44,6 +44,15 @@ __global__ __launch_bounds__(kBlock) void geom_ref_table_kernel(GeomLaunch G, fl
   ref[p] = ref_phase_factor(G.w0_ref, G.wd, G.c32, G.ref_single, rref);
 }
 
+// Receive entry of (pixel p, channel n) for the plan-time scans: from the
+// table, or computed in place with the on-the-fly receive state (the entries
+// the tile body will use, so both agree on which weights are nonzero).
+__device__ __forceinline__ float4 rx_at(const GeomLaunch& G, const float4* __restrict__ rxt, int n, int p,
+                                        double px, double py, double pz) {
+  if (G.rx_otf) return rx_otf_entry(G.otf, px, py, pz, n);
+  return rx_entry(rxt, G.rx_compact, (size_t)n * G.P + p);
+}
+
 // One wave64 per 64-pixel tile: the lists of waves and channels with a
 // nonzero weight on some pixel of the tile, and whether a nonzero weight or
 // its delay is not finite. The channel list holds first the channels with a
@@ -57,6 +66,8 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
   const int p = tile * 64 + (int)threadIdx.x;
   const bool active = p < G.P;
   const size_t PS = (size_t)G.P;
+  double px = 0.0, py = 0.0, pz = 0.0, rref = 0.0;
+  if (G.rx_otf) pixel_xyz(G.scan, active ? p : 0, px, py, pz, rref);
   bool bad = false;
   int cnt = 0;
   for (int w = 0; w < G.W; ++w) {
@@ -70,7 +81,7 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
   }
   int nc = 0;
   for (int n = 0; n < G.N; ++n) {
-    const float4 v = active ? rx_entry(rxt, G.rx_compact, (size_t)n * PS + p) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 v = active ? rx_at(G, rxt, n, p, px, py, pz) : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool on = v.y != 0.0f;
     if (on && !(isfinite(v.x) && isfinite(v.z) && isfinite(v.w))) bad = true;
     if (__any(on && v.y == v.y)) {
@@ -80,7 +91,7 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
   }
   const int nfin = nc;
   for (int n = 0; n < G.N; ++n) {
-    const float4 v = active ? rx_entry(rxt, G.rx_compact, (size_t)n * PS + p) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 v = active ? rx_at(G, rxt, n, p, px, py, pz) : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool on = v.y != 0.0f;
     if (__any(on) && !__any(on && v.y == v.y)) {
       if (threadIdx.x == 0) tile_chl[(size_t)tile * G.N + nc] = n;
@@ -111,12 +122,14 @@ __global__ __launch_bounds__(64) void geom_nan_mask_kernel(GeomLaunch G, const f
   const int p = tile * 64 + (int)threadIdx.x;
   const bool active = p < G.P;
   const size_t PS = (size_t)G.P;
+  double px = 0.0, py = 0.0, pz = 0.0, rref = 0.0;
+  if (G.rx_otf) pixel_xyz(G.scan, active ? p : 0, px, py, pz, rref);
   const int nc = tile_nc[tile], nfin = tile_nc[gridDim.x + tile], nw = tile_nw[tile];
   const float lim = (float)(G.interp == INTERP_LINEAR ? G.L - 1 : G.L - 2);
   bool any = false;
   for (int k = nfin; k < nc; ++k) {
     const int n = tile_chl[(size_t)tile * G.N + k];
-    const float4 rv = active ? rx_entry(rxt, G.rx_compact, (size_t)n * PS + p) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 rv = active ? rx_at(G, rxt, n, p, px, py, pz) : make_float4(0.f, 0.f, 0.f, 0.f);
     if (rv.y == 0.0f) continue;
     for (int i = 0; i < nw && !any; ++i) {
       const int w = tile_waves[(size_t)tile * G.W + i];
@@ -189,6 +202,39 @@ __global__ __launch_bounds__(64 * WS) void das_both_rxtab_mixed_kernel(RxAcq A, 
   }
 }
 
+// dimension.both with the receive state on the fly (variant 4): one block per
+// (tile, frame). Blocks 0 .. bad - 1 take the tiles with a non-finite nonzero
+// weight or delay (the visiting order's entries good ..) with the SAFE body,
+// blocks bad .. bad8 - 1 none, the rest the others through the XCD remap.
+template <int SM, int WC, int WS>
+__global__ __launch_bounds__(64 * WS) void das_both_rxotf_kernel(RxAcq A, const int* __restrict__ order, int good,
+                                                                 int bad, int bad8) {
+  __shared__ float2 red[WS][64];
+  __shared__ float4 rxs[kOtfChunk][64];
+  const int b = (int)blockIdx.x;
+  if (b < bad8) {
+    if (b >= bad) return;  // whole block, before any barrier
+    const int tile = __builtin_amdgcn_readfirstlane(order[good + b]);
+    rxotf_tile<SM, WC, WS, true>(A, tile, (int)blockIdx.y, red, rxs);
+  } else {
+    int tile = xcd_tile(b - bad8, good);
+    if (order) tile = __builtin_amdgcn_readfirstlane(order[tile]);
+    rxotf_tile<SM, WC, WS, false>(A, tile, (int)blockIdx.y, red, rxs);
+  }
+}
+
+template <int SM, int WC, int WS>
+static hipError_t launch_otf_t(const GeomLaunch& G, hipStream_t stream) {
+  const int tiles = (G.P + 63) / 64;
+  const int good = G.tile_good, bad = tiles - good;
+  if (bad > 0 && !G.tile_order) return hipErrorInvalidValue;
+  const int bad8 = bad > 0 ? (bad + 7) & ~7 : 0;
+  const RxAcq A = rx_acq(G, SM);
+  hipLaunchKernelGGL((das_both_rxotf_kernel<SM, WC, WS>), dim3((unsigned)(bad8 + good), (unsigned)G.F),
+                     dim3(64 * WS), 0, stream, A, G.tile_order, good, bad, bad8);
+  return hipGetLastError();
+}
+
 // Plans whose weights are all finite launch the branch-free kernel; the
 // others the mixed one (same tile shape for both bodies, so that a batch,
 // which runs either body per tile, computes the same sums).
@@ -211,6 +257,7 @@ static hipError_t launch_rxtab_l(const GeomLaunch& G, hipStream_t stream) {
 
 template <int SM, int WC, int WS>
 static hipError_t launch_rxtab_t(const GeomLaunch& G, hipStream_t stream) {
+  if (SM != SM_COEF && G.rx_otf) return launch_otf_t<SM == SM_COEF ? SM_LINEAR : SM, WC, WS>(G, stream);
   return G.rx_compact ? launch_rxtab_l<SM, WC, WS, RXL_COMPACT>(G, stream)
                       : launch_rxtab_l<SM, WC, WS, RXL_FULL>(G, stream);
 }
@@ -257,10 +304,15 @@ hipError_t launch_both_rxtab(const GeomLaunch& G, hipStream_t stream) {
 hipError_t launch_rxtab_tables(const GeomLaunch& G, float4* rx_table, float2* ref_table, int* tile_nc,
                                int* tile_chl, int* tile_waves, int* tile_nw, int* tile_bad,
                                unsigned long long* nan_mask, hipStream_t stream) {
-  const dim3 rgrid((unsigned)((G.P + kBlock - 1) / kBlock), (unsigned)G.N);
-  hipLaunchKernelGGL(geom_rx_table_kernel, rgrid, dim3(kBlock), 0, stream, G, rx_table);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  hipError_t e = hipSuccess;
+  if (rx_table) {
+    const dim3 rgrid((unsigned)((G.P + kBlock - 1) / kBlock), (unsigned)G.N);
+    hipLaunchKernelGGL(geom_rx_table_kernel, rgrid, dim3(kBlock), 0, stream, G, rx_table);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  } else if (!G.rx_otf) {
+    return hipErrorInvalidValue;
+  }
   if (ref_table) {
     hipLaunchKernelGGL(geom_ref_table_kernel, dim3((unsigned)((G.P + kBlock - 1) / kBlock)), dim3(kBlock), 0,
                        stream, G, ref_table);

@@ -38,9 +38,21 @@ namespace ustbx {
 // nonzero weight or its delay is not finite): a term enters the sum only when
 // it is in the window, and the separable receive factor multiplies the sum
 // only when some term did, as das_c skips the others before multiplying.
-template <int SM, int FC, int PH, bool SAFE>
+// Samples per frame of a wave's LDS window (STAGE): 64 lanes x 2.
+constexpr int kWin = 128;
+
+// STAGE (branch-free tiles only): per (channel, wave) the tile's samples of
+// each frame are staged through LDS. The wave-uniform window [s0, s0 + span)
+// covers every in-window term's samples (the lanes' sample indices reduced
+// to their minimum and maximum); lane j loads samples s0 + 2j and s0 + 2j + 1
+// of each frame with one 16-byte load (the loads of a wave are contiguous),
+// writes them to the wave's LDS region win [FC][kWin], and every lane then
+// reads its samples from LDS. A window wider than kWin samples falls back to
+// one gather per term and frame. The arithmetic is the gather path's, so the
+// two give the same result bit for bit.
+template <int SM, int FC, int PH, bool SAFE, bool STAGE>
 __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, int n, int f0, int nf, int lane,
-                                           float2 (&v)[FC]) {
+                                           float2 (&v)[FC], float2* __restrict__ win) {
   using M = SampleMode<SM>;
   constexpr int TP = 64;
   constexpr int ST = M::kStride;
@@ -137,14 +149,49 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
     const float b = __builtin_amdgcn_fractf(d);
     // the first gather's offset folded into the wave-uniform part (n0 >= 1
     // wherever ok holds, so the sum stays non-negative)
-    const unsigned off =
-        ok ? (unsigned)(n0 * ST) + (nb + (unsigned)(wc * wst) + (unsigned)M::kLo) : (unsigned)kOOB;
     float4 q[FC][M::kLoads];
+    bool staged = false;
+    if (STAGE) {
+      const unsigned lo = wave_reduce_u32<false>(ok ? (unsigned)n0 : 0xFFFFFFFFu);
+      if (lo == 0xFFFFFFFFu) continue;  // no term of this wave in the window: nothing to add
+      const unsigned hi = wave_reduce_u32<true>(ok ? (unsigned)n0 : 0u);
+      const int s0 = (int)lo + (SM == SM_CUBIC ? -1 : 0);  // first sample of the window (>= 0)
+      const int span = (int)(hi - lo) + (SM == SM_CUBIC ? 4 : 2);
+      if (span <= kWin) {
+        staged = true;
+        if (lane < ((span + 1) >> 1)) {
+          const unsigned go = (unsigned)((s0 + 2 * lane) * 8) + nb + (unsigned)(wc * wst);
+          float4 st[FC];
 #pragma unroll
-    for (int f = 0; f < FC; ++f) {
-      const unsigned o = off + (unsigned)(f * fb);
-      q[f][0] = buf_load16(rs, (int)o);
-      if (M::kLoads == 2) q[f][M::kLoads - 1] = buf_load16(rs, (int)(o + (unsigned)(M::kHi - M::kLo)));
+          for (int f = 0; f < FC; ++f) st[f] = buf_load16(rs, (int)(go + (unsigned)(f * fb)));
+#pragma unroll
+          for (int f = 0; f < FC; ++f) reinterpret_cast<float4*>(win + f * kWin)[lane] = st[f];
+        }
+        // this lane's first sample in the window (n0 - 1 for cubic), kept
+        // inside the wave's region for the lanes outside the window
+        const int li = min(max(ok ? n0 - (int)lo : 0, 0), kWin - (SM == SM_CUBIC ? 4 : 2));
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int f = 0; f < FC; ++f) {
+          const float2* w = win + f * kWin + li;
+          const float2 a = w[0], b1 = w[1];
+          q[f][0] = ok ? make_float4(a.x, a.y, b1.x, b1.y) : z4;
+          if (M::kLoads == 2) {
+            const float2 c = w[2], d2 = w[3];
+            q[f][M::kLoads - 1] = ok ? make_float4(c.x, c.y, d2.x, d2.y) : z4;
+          }
+        }
+      }
+    }
+    if (!staged) {
+      const unsigned off =
+          ok ? (unsigned)(n0 * ST) + (nb + (unsigned)(wc * wst) + (unsigned)M::kLo) : (unsigned)kOOB;
+#pragma unroll
+      for (int f = 0; f < FC; ++f) {
+        const unsigned o = off + (unsigned)(f * fb);
+        q[f][0] = buf_load16(rs, (int)o);
+        if (M::kLoads == 2) q[f][M::kLoads - 1] = buf_load16(rs, (int)(o + (unsigned)(M::kHi - M::kLo)));
+      }
     }
     __builtin_amdgcn_sched_barrier(0);
     float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f;
@@ -190,10 +237,11 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
 
 // Blocks cover slabs x tiles entries of the visiting order (order[0 ..
 // tiles), or the tiles themselves where order is null).
-template <int SM, int FC, int CS, int PH, bool SAFE>
+template <int SM, int FC, int CS, int PH, bool SAFE, bool STAGE>
 __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, float wd,
                                                                const int* __restrict__ order, int tiles) {
   constexpr int TP = 64;
+  __shared__ __align__(16) float2 win[STAGE ? CS * FC * kWin : 1];
   const int lane = threadIdx.x & (TP - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
   const int chunks = (F + FC - 1) / FC;
@@ -207,7 +255,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, f
   const int f0 = (slab - group * chunks) * FC;
   const int nf = min(FC, F - f0);
   float2 v[FC];
-  tx_channel<SM, FC, PH, SAFE>(A, wd, tile, n, f0, nf, lane, v);
+  tx_channel<SM, FC, PH, SAFE, STAGE>(A, wd, tile, n, f0, nf, lane, v, win + (STAGE ? s * FC * kWin : 0));
   const int p = tile * TP + lane;
   if (p < A.P) {
     const size_t PS = (size_t)A.P;
@@ -224,12 +272,13 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, f
 // its CS channels in channel order: S_g = sum v, E_g = sum |v|^2
 // (coherence_factor.m:126-127), through LDS. part is [G][F][P] float4
 // {S.x, S.y, E, 0}, G = ceil(N / CS).
-template <int SM, int FC, int CS, int PH, bool SAFE>
+template <int SM, int FC, int CS, int PH, bool SAFE, bool STAGE>
 __global__ __launch_bounds__(64 * CS) void das_tx_cf_part_kernel(RxAcq A, int F, float wd,
                                                                  const int* __restrict__ order, int tiles,
                                                                  float4* __restrict__ part) {
   constexpr int TP = 64;
   __shared__ float2 red[CS][TP];
+  __shared__ __align__(16) float2 win[STAGE ? CS * FC * kWin : 1];
   const int lane = threadIdx.x & (TP - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
   const int chunks = (F + FC - 1) / FC;
@@ -243,7 +292,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_cf_part_kernel(RxAcq A, int F,
   const int nf = min(FC, F - f0);
   float2 v[FC];
   if (n < A.N) {
-    tx_channel<SM, FC, PH, SAFE>(A, wd, tile, n, f0, nf, lane, v);
+    tx_channel<SM, FC, PH, SAFE, STAGE>(A, wd, tile, n, f0, nf, lane, v, win + (STAGE ? s * FC * kWin : 0));
   } else {
 #pragma unroll
     for (int f = 0; f < FC; ++f) v[f] = make_float2(0.0f, 0.0f);
@@ -300,14 +349,18 @@ static hipError_t launch_tx_t(const GeomLaunch& G, hipStream_t stream) {
   if ((int64_t)tiles * slabs > 0x7FFFFFFF) return hipErrorInvalidValue;
   const RxAcq A = rx_acq(G, SM);
   if (good > 0) {
-    hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, FC, CS, PH, false>), dim3((unsigned)(good * slabs)), dim3(64 * CS),
-                       0, stream, A, G.F, G.wd, G.tile_order, good);
+    if (G.stage)
+      hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, FC, CS, PH, false, true>), dim3((unsigned)(good * slabs)),
+                         dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order, good);
+    else
+      hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, FC, CS, PH, false, false>), dim3((unsigned)(good * slabs)),
+                         dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order, good);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if (good < tiles) {
     const int64_t bslabs = (int64_t)((G.F + 7) / 8) * ((G.N + CS - 1) / CS);
-    hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, 8, CS, PH, true>), dim3((unsigned)((tiles - good) * bslabs)),
+    hipLaunchKernelGGL((das_tx_rxtab_kernel<SM, 8, CS, PH, true, false>), dim3((unsigned)((tiles - good) * bslabs)),
                        dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order + good, tiles - good);
     return hipGetLastError();
   }
@@ -337,15 +390,20 @@ static hipError_t launch_cf_t(const GeomLaunch& G, float4* part, hipStream_t str
   if ((int64_t)tiles * slabs > 0x7FFFFFFF) return hipErrorInvalidValue;
   const RxAcq A = rx_acq(G, SM);
   if (good > 0) {
-    hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, FC, CS, PH, false>), dim3((unsigned)(good * slabs)),
-                       dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order, good, part);
+    if (G.stage)
+      hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, FC, CS, PH, false, true>), dim3((unsigned)(good * slabs)),
+                         dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order, good, part);
+    else
+      hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, FC, CS, PH, false, false>), dim3((unsigned)(good * slabs)),
+                         dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order, good, part);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if (good < tiles) {
     const int64_t bslabs = (int64_t)((G.F + 7) / 8) * ((G.N + CS - 1) / CS);
-    hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, 8, CS, PH, true>), dim3((unsigned)((tiles - good) * bslabs)),
-                       dim3(64 * CS), 0, stream, A, G.F, G.wd, G.tile_order + good, tiles - good, part);
+    hipLaunchKernelGGL((das_tx_cf_part_kernel<SM, 8, CS, PH, true, false>),
+                       dim3((unsigned)((tiles - good) * bslabs)), dim3(64 * CS), 0, stream, A, G.F, G.wd,
+                       G.tile_order + good, tiles - good, part);
     return hipGetLastError();
   }
   return hipSuccess;

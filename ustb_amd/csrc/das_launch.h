@@ -70,6 +70,11 @@ struct GeomLaunch {
   // coefficients [F][W][N][L] of two float4.
   const float4* rx_table;
   int rx_compact;       // RF plans: the receive table holds float2 {rd, ra}
+  // Receive state on the fly (variant 4): no receive table; the blocks
+  // compute each (pixel, channel) entry from otf (rx_otf_entry).
+  int rx_otf;
+  RxOtf otf;
+  const RxOtf* d_otf;  // its device copy (the tile bodies read it from there)
   const int* tile_nc;
   const int* tile_ncf;  // [tiles] leading entries of the channel list with a weight other than NaN
   const unsigned long long* nan_mask;  // [tiles] lanes whose sum the NaN-only channels make NaN
@@ -84,6 +89,9 @@ struct GeomLaunch {
   // are all finite; the order is non-null when tile_good < tiles.
   const int* tile_order;
   int tile_good;
+  // Dimension transmit (variant 2): stage each tile's sample windows through
+  // LDS (1) or gather every term's samples (0). Results do not depend on it.
+  int stage;
 };
 
 // Builds the transmit table of a plan (one lane per (wave, pixel)).

@@ -70,6 +70,8 @@ struct RxAcq {
   float t0, fs;
   int frame_bytes;         // W * N * L * stride (< 0x7FFFFE00)
   int rx_compact;          // RF plans (wd = 0): the receive table holds float2 {rd, ra}
+  int rx_otf;              // no receive table: entries computed in the block (rxotf_tile)
+  const RxOtf* otf;        // device copy of the on-the-fly receive state's parameters
 };
 
 // Receive-table entry idx as {rd, ra, ra*cos(wd rd), ra*sin(wd rd)}: RF plans
@@ -317,6 +319,113 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
   }
 }
 
+// The same tile as rxtab_tile with the receive state computed in the block
+// (RxAcq::rx_otf, no receive table): the tile's channel list is taken kOtfChunk
+// entries at a time; the block's waves compute the entries {rd, ra, R} of
+// the chunk for the tile's 64 pixels (wave s every WS-th one, rx_otf_entry)
+// into LDS, and after a barrier every split folds the chunk's channels of its
+// group from there, in list order, as rxtab_channels does from the table. All
+// waves run every round and chunk (the barriers); a split with no waves in a
+// round only computes. Given the same entries the sums are rxtab_tile's, term
+// for term.
+constexpr int kOtfChunk = 16;
+
+template <int SM, int WC, int WS, bool SAFE>
+__device__ __forceinline__ void rxotf_tile(const RxAcq& A, int tile, int f, float2 (*red)[64],
+                                           float4 (*rxs)[64]) {
+  constexpr int TP = 64;
+  constexpr int CC = kOtfChunk;
+  constexpr int ST = SampleMode<SM>::kStride;
+  const int lane = threadIdx.x & (TP - 1);
+  const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);  // wave-uniform
+  const int p = tile * TP + lane;
+  const bool active = p < A.P;
+  const size_t PS = (size_t)A.P;
+  const int nc = __builtin_amdgcn_readfirstlane(SAFE ? A.tile_ncf[tile] : A.tile_nc[tile]);
+  const int nw = __builtin_amdgcn_readfirstlane(A.tile_nw[tile]);
+  const int* __restrict__ wl = A.tile_waves + (size_t)tile * A.W;
+  const int* __restrict__ chl = A.tile_chl + (size_t)tile * A.N;
+  const float4* __restrict__ txt = A.tx;
+  const int lst = A.L * ST;
+  const int wst = lst * A.N;
+  const __amdgpu_buffer_rsrc_t rs =
+      make_rsrc((const char*)A.data + (size_t)f * A.frame_bytes, A.frame_bytes);
+  const float lim = (float)(SM == SM_LINEAR ? A.L - 1 : A.L - 2);
+
+  // waves and channel groups as rxtab_tile
+  const int wneed = (nw + WC - 1) / WC;
+  const int cg = (wneed > 0 && wneed < WS) ? WS / wneed : 1;
+  const int wsp = cg > 1 ? wneed : WS;
+  const int g = s / wsp;
+  const int sw = s - g * wsp;
+  const int rounds = nc > 0 ? (nw + wsp * WC - 1) / (wsp * WC) : 0;  // block-uniform
+
+  float2 acc = make_float2(0.0f, 0.0f);
+  for (int r = 0; r < rounds; ++r) {
+    const int k0 = r * wsp * WC;
+    const int nj = g < cg ? min(WC, max(0, (nw - k0 - sw + wsp - 1) / wsp)) : 0;
+    float td[WC];
+    float2 T[WC];
+    int wb[WC];
+#pragma unroll
+    for (int j = 0; j < WC; ++j) {
+      td[j] = kFar;
+      T[j] = make_float2(0.0f, 0.0f);
+      wb[j] = 0;
+      if (j < nj) {
+        const int w = __builtin_amdgcn_readfirstlane(wl[k0 + sw + wsp * j]);
+        wb[j] = w * wst;
+        if (active) {
+          const float4 v = txt[(size_t)w * PS + p];
+          if (v.y != 0.0f) {
+            td[j] = v.x;
+            T[j] = make_float2(v.z, v.w);
+          }
+        }
+      }
+    }
+    for (int c0 = 0; c0 < nc; c0 += CC) {
+      const int cn = min(CC, nc - c0);
+      {
+        // the parameters and the pixel's coordinates are read again for every
+        // chunk (scalar loads; the empty asm keeps them from being hoisted),
+        // so that none of it stays live across the term loop
+        const RxOtf* op = A.otf;
+        asm volatile("" : "+s"(op));
+        double px = 0.0, py = 0.0, pz = 0.0, rref = 0.0;
+        pixel_xyz(op->scan, active ? p : 0, px, py, pz, rref);
+        for (int e = s; e < cn; e += WS) {
+          const int n = __builtin_amdgcn_readfirstlane(chl[c0 + e]);
+          rxs[e][lane] = active ? rx_otf_entry(*op, px, py, pz, n) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+      __syncthreads();
+      if (nj > 0) {
+        // this group's list entries k = g, g + cg, ... within the chunk
+        for (int e = ((g - c0 % cg) % cg + cg) % cg; e < cn; e += cg) {
+          const int nb = __builtin_amdgcn_readfirstlane(chl[c0 + e]) * lst;
+          const float4 rv = rxs[e][lane];
+          if (nj == WC)
+            rxtab_fold<SM, WC, false, SAFE>(rs, td, T, wb, nj, rv, nb, A.t0, A.fs, lim, acc);
+          else
+            rxtab_fold<SM, WC, true, SAFE>(rs, td, T, wb, nj, rv, nb, A.t0, A.fs, lim, acc);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  red[s][lane] = acc;
+  __syncthreads();
+  if (s == 0 && active) {
+    float2 v = red[0][lane];
+#pragma unroll
+    for (int k = 1; k < WS; ++k) cadd(v, red[k][lane]);
+    if (A.ref) v = cmul(v, A.ref[p]);
+    if (SAFE && ((A.nan_mask[tile] >> lane) & 1ull)) v = make_float2(__builtin_nanf(""), __builtin_nanf(""));
+    A.out[(size_t)f * PS + p] = v;
+  }
+}
+
 // Descriptor of a plan's variant-2/3 tables for the tile body.
 inline RxAcq rx_acq(const GeomLaunch& G, int sm) {
   RxAcq A;
@@ -339,6 +448,8 @@ inline RxAcq rx_acq(const GeomLaunch& G, int sm) {
   A.fs = G.fs;
   A.frame_bytes = G.L * G.N * G.W * (sm == SM_COEF ? 32 : 8);
   A.rx_compact = G.rx_compact;
+  A.rx_otf = G.rx_otf;
+  A.otf = G.d_otf;
   return A;
 }
 

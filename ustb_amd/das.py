@@ -8,8 +8,8 @@ data, and the static ``_beamform`` with the reference's signature.
 ``go()`` computes delays and apodization on the device (a resident geometry
 plan, cached per configuration, including the unified transmit delay model)
 and fuses the IQ reference phase (das.m:306-309) into the kernel epilogue.
-Scanline apodization has no device form: the plan raises
-USTBX_E_UNSUPPORTED for it.
+Scanline (MLA) transmit apodization is built by the plan from the scan's
+lines (apodization.m:265-306).
 """
 from __future__ import annotations
 
@@ -60,7 +60,8 @@ class DAS:
         """Every field of an Apodization into the hash, arrays by their bytes
         (a repr would elide the middle of long vectors)."""
         h.update(repr((int(apo.window), apo.f_number, apo.minimum_aperture, apo.maximum_aperture, apo.tilt,
-                       apo.grating_lobe_angle, None if apo.origin is None else apo.origin.xyz)).encode())
+                       apo.grating_lobe_angle, None if apo.origin is None else apo.origin.xyz, apo.MLA,
+                       apo.MLA_overlap)).encode())
         v = apo.apodization_vector
         if v is None:
             h.update(b"no-vector")

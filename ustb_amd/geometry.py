@@ -24,6 +24,8 @@ def _apod_struct(a: Apodization, keep: list) -> L.Apod:
     s.maximum_aperture[:] = a.maximum_aperture
     s.tilt[:] = a.tilt
     s.grating_lobe_angle[:] = a.grating_lobe_angle
+    s.mla[:] = a.MLA
+    s.mla_overlap[:] = a.MLA_overlap
     if a.origin is not None:
         s.has_origin = 1
         s.origin[:] = a.origin.xyz

@@ -156,6 +156,30 @@ def demodulate(rf: torch.Tensor, fs: float, t0: float, fmod: float, b, down: int
     return out
 
 
+def estimate_modulation_frequency(rf: torch.Tensor, fs: float, return_spectrum: bool = False):
+    """The modulation frequency preprocess.fast_demodulation estimates from
+    RF data (F, W, N, L) float32 on the device when none is set
+    (fast_demodulation.m:57-73): the power spectrum averaged over channels,
+    waves and the first min(100, F) frames, the centre of its -6 dB points
+    around the maximum, negated. return_spectrum: also (fx, pw) as
+    tools.power_spectrum gives them."""
+    _require_cuda(rf)
+    if rf.dtype != torch.float32 or rf.dim() != 4:
+        raise ValueError("RF data must be a (F, W, N, L) float32 tensor")
+    F, W, N, Ls = rf.shape
+    traces = min(100, F) * W * N
+    fmod = ctypes.c_double(0.0)
+    pw = np.zeros(Ls, dtype=np.float64)
+    st = L.Status()
+    L.check(L.lib().ustbx_estimate_modulation_frequency(rf.data_ptr(), Ls, traces, float(fs), ctypes.byref(fmod),
+                                                        pw.ctypes.data, _stream_ptr(rf.device), ctypes.byref(st)),
+            st)
+    if return_spectrum:
+        fx = np.linspace(-fs / 2, fs / 2, Ls + 1)[:-1]
+        return fmod.value, fx, pw
+    return fmod.value
+
+
 def hilbert(rf: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
     """Analytic signal along the last axis (MATLAB/scipy hilbert) of a
     float32 device tensor; returns complex64 of the same shape."""
@@ -246,6 +270,12 @@ class GeometryPlan:
         per XCD (default)."""
         st = L.Status()
         L.check(L.lib().ustbx_plan_set_order(self.handle, int(order), ctypes.byref(st)), st)
+
+    def set_staging(self, on: bool):
+        """Dimension transmit: stage the tiles' sample windows through LDS
+        (default) or gather each term's samples; results are the same."""
+        st = L.Status()
+        L.check(L.lib().ustbx_plan_set_staging(self.handle, int(bool(on)), ctypes.byref(st)), st)
 
     def set_tiles(self, wave_chunk: int = 0, split: int = 0):
         st = L.Status()

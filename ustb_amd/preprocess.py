@@ -7,7 +7,9 @@ data whose IQ samples stay on the device. ``flavour = "python"`` follows the
 Python reference's FastDemodulation instead
 (python/src/ustb/preprocess/fast_demodulation.py). The FIR taps are designed
 on the host (scipy.signal, the published Kaiser and windowed-sinc designs);
-mixing, filtering and decimation run in ustbx_demodulate.
+mixing, filtering and decimation run in ustbx_demodulate. A missing
+modulation frequency is estimated from the RF power spectrum
+(ustbx_estimate_modulation_frequency, fast_demodulation.m:57-73).
 """
 from __future__ import annotations
 
@@ -83,8 +85,9 @@ class FastDemodulation:
             fs_out, t0_out, fmod = fs / dec, t0, fc
         else:
             if self.modulation_frequency is None:
-                raise ValueError("modulation_frequency must be set (its estimate from the power spectrum, "
-                                 "fast_demodulation.m:63-73, is not on the device)")
+                warnings.warn("The modulation frequency is not specified. The estimated central frequency will be "
+                              "used.")
+                self.modulation_frequency = ops.estimate_modulation_frequency(x, fs)
             fmod = float(self.modulation_frequency)
             fdown = self.downsample_frequency
             if fdown is None:

@@ -209,6 +209,8 @@ class Apodization:
     grating_lobe_angle: tuple = (math.pi, math.pi)
     apodization_vector: np.ndarray | None = None
     origin: Point | None = None
+    MLA: tuple = (1, 1)             # scan lines per wave (window.scanline)
+    MLA_overlap: tuple = (0, 0)     # lines each wave's block is widened by (window.scanline)
 
     def __post_init__(self):
         def pair(v):
@@ -221,6 +223,9 @@ class Apodization:
         t = np.atleast_1d(np.asarray(self.tilt, dtype=np.float64))
         self.tilt = (float(t[0]), 0.0) if t.size == 1 else (float(t[0]), float(t[1]))
         self.grating_lobe_angle = pair(self.grating_lobe_angle)
+        ipair = lambda v: tuple(int(x) for x in (np.resize(np.atleast_1d(v), 2)))
+        self.MLA = ipair(self.MLA)
+        self.MLA_overlap = ipair(self.MLA_overlap)
 
 
 @dataclass
