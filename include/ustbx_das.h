@@ -438,6 +438,18 @@ int ustbx_coherence_factor(const void* in, void* image, float* cf, const float* 
 int64_t ustbx_plan_cf_workspace_bytes(const ustbx_plan* plan);
 int ustbx_plan_run_cf(const ustbx_plan* plan, const void* ch, void* image, float* cf, const float* M,
                       int32_t flavour, void* workspace, int64_t workspace_bytes, void* stream, ustbx_status* st);
+/* ustbx_plan_run_cf without a workspace argument and without allocating: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc). The partial sums go to a device array of the library's own,
+ * ustbx_scratch_bytes() (64 MiB) per GPU, held from library load to process
+ * end; a plan whose ustbx_plan_cf_workspace_bytes exceeds it is
+ * USTBX_E_CAPACITY. The results equal ustbx_plan_run_cf's bit for bit. Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (on
+ * different streams, or from different threads) race. */
+int64_t ustbx_scratch_bytes(void);
+int ustbx_plan_run_cf_noalloc(const ustbx_plan* plan, const void* ch, void* image, float* cf, const float* M,
+                              int32_t flavour, void* stream, ustbx_status* st);
 /* M[p] from a plan's FP32 apodization (as ustbx_plan_materialize gives it),
  * for ustbx_plan_run_cf and ustbx_coherence_factor. */
 int ustbx_plan_active_elements(const ustbx_plan* plan, int32_t dim, double criterium, float* M, void* stream,

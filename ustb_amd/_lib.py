@@ -143,6 +143,9 @@ EXPORTS = {
     "ustbx_plan_cf_workspace_bytes": (c_int64, [c_void_p]),
     "ustbx_plan_run_cf": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p,
                                          c_int64, c_void_p, ctypes.POINTER(Status)]),
+    "ustbx_scratch_bytes": (c_int64, []),
+    "ustbx_plan_run_cf_noalloc": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int32,
+                                                 c_void_p, ctypes.POINTER(Status)]),
     "ustbx_plan_active_elements": (ctypes.c_int, [c_void_p, c_int32, ctypes.c_double, c_void_p, c_void_p,
                                                   ctypes.POINTER(Status)]),
     "ustbx_estimate_modulation_frequency": (ctypes.c_int, [c_void_p, c_int32, c_int64, ctypes.c_double,

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1175,6 +1176,70 @@ int ustbx_plan_run_cf(const ustbx_plan* plan, const void* ch, void* image, float
                               flavour, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(st, e, "das_tx_cf kernels launch");
   return ok(st);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of ustbx_plan_run_cf: the per-group partial sums go to
+// a file-scope device array instead of a caller workspace. Each GPU holds its
+// own copy of the array from library load to process end; host code reaches
+// it only through the address hipGetSymbolAddress gives on that device.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices];
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller's current device is restored).
+hipError_t scratch_address(int device, void** ptr) {
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int prev = 0;
+    hipError_t e = hipGetDevice(&prev);
+    if (e != hipSuccess) return e;
+    if (prev != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    const hipError_t e2 = prev != device ? hipSetDevice(prev) : hipSuccess;
+    if (e != hipSuccess) return e;
+    if (e2 != hipSuccess) return e2;
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ustbx_scratch_bytes(void) { return (int64_t)kScratchBytes; }
+
+int ustbx_plan_run_cf_noalloc(const ustbx_plan* plan, const void* ch, void* image, float* cf, const float* M,
+                              int32_t flavour, void* stream, ustbx_status* st) {
+  if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
+  const GeomLaunch& P0 = plan->G;
+  if (P0.dim != USTBX_DIM_TX || P0.variant != 2)
+    return set_status(st, USTBX_E_UNSUPPORTED,
+                      "the fused coherence factor needs a dimension-transmit plan on the receive-table kernel "
+                      "(variant 2)");
+  const size_t need = tx_cf_workspace_bytes(P0);
+  if (need > kScratchBytes)
+    return set_status(st, USTBX_E_CAPACITY,
+                      "the partial sums need %zu bytes, more than the %zu-byte device scratch of the allocation-free "
+                      "entry; use ustbx_plan_run_cf with a workspace",
+                      need, kScratchBytes);
+  void* ws = nullptr;
+  const hipError_t e = scratch_address(plan->device, &ws);
+  if (e != hipSuccess) return hip_fail(st, e, "hipGetSymbolAddress (scratch)");
+  return ustbx_plan_run_cf(plan, ch, image, cf, M, flavour, ws, (int64_t)kScratchBytes, stream, st);
 }
 
 int ustbx_plan_active_elements(const ustbx_plan* plan, int32_t dim, double criterium, float* M, void* stream,
