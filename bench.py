@@ -68,8 +68,9 @@ def parse():
     ap.add_argument("--wave-chunk", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant (default: the plan's choice)")
-    ap.add_argument("--stage", type=int, default=1, choices=[0, 1],
-                    help="dimension transmit: stage sample windows through LDS (1) or gather (0)")
+    ap.add_argument("--stage", type=int, default=-1, choices=[-1, 0, 1],
+                    help="stage sample windows through LDS (1) or gather (0); default (-1): the plan's "
+                         "choice (on for dimension both, off for dimension transmit)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only time a 1 GiB device-to-device copy (achievable HBM bandwidth; also the "
                          "known-size kernel that calibrates the FETCH_SIZE/WRITE_SIZE counters)")
@@ -172,7 +173,10 @@ def main():
         plan.set_variant(args.variant)
     if args.wave_chunk or args.split:
         plan.set_tiles(args.wave_chunk, args.split)
-    plan.set_staging(bool(args.stage))
+    if args.stage >= 0:
+        plan.set_staging(bool(args.stage))
+    staging = bool(args.stage) if args.stage >= 0 else int(acq.dimension) == 3
+    staged_pairs = plan.stage_stats if int(acq.dimension) == 3 and staging else (0, 0)
     variant, wave_chunk, split = plan.variant
     out = torch.empty(plan.out_shape, dtype=torch.complex64, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -228,7 +232,8 @@ def main():
                    "geometry": "on-device, resident per-configuration tables",
                    "plan_build_ms": round(plan_ms, 2),
                    "kernel_variant": variant, "waves_per_split": wave_chunk, "splits": split,
-                   "lds_staging": bool(args.stage) if int(acq.dimension) == 2 else None},
+                   "lds_staging": staging,
+                   "staged_tile_channels": round(staged_pairs[0] / staged_pairs[1], 4) if staged_pairs[1] else None},
         "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": VALU_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / VALU_PEAK_TFLOPS, 4),
                      "traffic": traffic,

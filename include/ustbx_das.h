@@ -295,11 +295,19 @@ int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st);
  * scan lines (the default; it keeps the sample windows that co-resident
  * blocks read in L2). Results do not depend on it. */
 int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st);
-/* Dimension transmit on variant 2 (no reference counterpart): 1 (the
- * default) stages each 64-pixel tile's sample window of every (channel,
- * wave) through LDS with contiguous loads, 0 gathers each term's samples.
- * Results do not depend on it. */
+/* LDS staging of sample windows (no reference counterpart); results do not
+ * depend on it. Dimension both on the receive table (variants 2, 3 only for
+ * linear or cubic data): 1 (the default) runs every (tile, channel) the plan
+ * marked (all 64 pixels' terms in one 64-sample window of each wave's trace,
+ * every weight nonzero) from windows loaded with one contiguous load per
+ * term and read back from LDS, 0 gathers each term's samples. Dimension
+ * transmit on variant 2: 1 stages each tile's window of every (channel, wave)
+ * found at run time, 0 (the default) gathers. Batches take the setting of
+ * each plan when they are created. */
 int ustbx_plan_set_staging(ustbx_plan* plan, int32_t on, ustbx_status* st);
+/* (tile, channel) pairs of a dimension-both receive-table plan that run
+ * staged, and all pairs of the tiles' channel lists (0, 0 otherwise). */
+int ustbx_plan_stage_stats(const ustbx_plan* plan, int64_t* staged, int64_t* listed, ustbx_status* st);
 /* Current variant (or -1 for NULL) and its tile shape. */
 int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split);
 

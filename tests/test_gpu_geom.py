@@ -372,12 +372,102 @@ def test_transmit_staging_equals_gathers(cuda, name, separable, interp):
     plan.close()
 
 
+def _staged_count(ta, ra, td, rd, L, fs, t0, cubic):
+    """(tile, channel) pairs the stage mask should mark, from das.m's FP32
+    matrices: full 64-pixel tiles whose nonzero weights and delays are all
+    finite; per channel with a nonzero weight on the tile, every weight of it
+    and of the tile's waves nonzero on all 64 pixels, every term's index
+    d = ((td + rd) - t0) * fs (FP32) in [1, lim) and every lane's trunc(d)
+    within [that of lane 0, +60] (cubic; +62 linear)."""
+    W, P = ta.shape
+    lim = np.float32(L - 2 if cubic else L - 1)
+    span = 60 if cubic else 62
+    t0, fs = np.float32(t0), np.float32(fs)
+    count = 0
+    for t in range(P // 64):
+        sl = slice(64 * t, 64 * t + 64)
+        a_t, d_t, a_r, d_r = ta[:, sl], td[:, sl], ra[:, sl], rd[:, sl]
+        tx_on, rx_on = a_t != 0, a_r != 0
+        if np.any(tx_on & ~(np.isfinite(d_t) & np.isfinite(a_t))) or \
+                np.any(rx_on & ~(np.isfinite(d_r) & np.isfinite(a_r))):
+            continue
+        waves = np.nonzero(tx_on.any(axis=1))[0]
+        chans = np.nonzero(rx_on.any(axis=1))[0]
+        if waves.size == 0 or chans.size == 0:
+            continue
+        tw_ok = bool(np.all(tx_on[waves]))
+        d = ((d_t[waves][None, :, :] + d_r[chans][:, None, :]) - t0) * fs  # float32 throughout
+        n0 = d.astype(np.int32)
+        ok = np.all(rx_on[chans], axis=1) & tw_ok
+        ok &= np.all((d >= 1) & (d < lim), axis=(1, 2))
+        rel = n0 - n0[:, :, :1]
+        ok &= np.all((rel >= 0) & (rel <= span), axis=(1, 2))
+        count += int(ok.sum())
+    return count
+
+
+def _stage_plan(name, F):
+    """Plans for the dimension-both staging test: the geometry cases, and C2's
+    probe, waves and depths at a quarter of its elements, waves and scan lines
+    (64 lines x 1024 depths, 1024 samples), as IQ (cubic, linear) and as RF
+    (fd = 0: the compact receive table). With C2's depth spacing a 64-pixel
+    tile spans about 42 samples, so most (tile, channel) pairs are staged."""
+    from ustb_amd import configs
+    from ustb_amd.uff import LinearScan
+    if name in CASES:
+        case = CASES[name]
+        interp = Interpolation.cubic if name != "pw_rf_hanning" else Interpolation.linear
+        return _plan(case, dim=3, F=F, separable_phase=True, interpolation=interp), case["L"], case["fs"], 0.0
+    interp = Interpolation.cubic if name == "c2q_cubic" else Interpolation.linear
+    acq = configs.c2(scale=0.25, with_data=False)
+    scan = LinearScan(np.linspace(-20e-3, 20e-3, 64), np.linspace(5e-3, 55e-3, 1024))
+    kw = acq.plan_kwargs(n_samples=1024, n_frames=F, interpolation=interp)
+    if name == "c2q_rf":
+        kw["modulation_frequency"] = 0.0
+    plan = make_plan(acq.probe, scan, acq.sequence, separable_phase=True, **kw)
+    return plan, 1024, acq.fs, acq.t0
+
+
+@pytest.mark.parametrize("name", ["c2q_cubic", "c2q_linear", "c2q_rf", "cpwc_tukey", "dw_sector", "pw_rf_hanning",
+                                  "cpwc_depth0", "matrix3d_dw", "focused_hybrid"])
+def test_both_staging_equals_gathers(cuda, name):
+    """Dimension both: the (tile, channel) pairs the plan marks run from
+    LDS-staged sample windows. The mask marks exactly the pairs das.m's FP32
+    matrices allow (recounted on the host), and the images equal the gather
+    body's bit for bit for every tile shape (full and partial wave rounds,
+    channel groups), NaN-weight tiles included; two frames."""
+    F = 2
+    print(f"case both-staging {name}", flush=True)
+    plan, L, fs, t0 = _stage_plan(name, F)
+    if plan.variant[0] == 4:
+        plan.set_variant(2)
+    assert plan.variant[0] == 2
+    ta, ra, td, rd = [t.cpu().numpy() for t in plan.materialize()]
+    cubic = int(plan.interp) == int(Interpolation.cubic)
+    staged, listed = plan.stage_stats
+    assert staged == _staged_count(ta, ra, td, rd, L, fs, t0, cubic), (staged, listed)
+    if name.startswith("c2q"):
+        assert staged > 0.5 * listed, (staged, listed)
+    g = torch.Generator(device=cuda).manual_seed(41)
+    shape = (F, plan.W, plan.N, L)
+    ch = torch.complex(torch.randn(shape, device=cuda, generator=g), torch.randn(shape, device=cuda, generator=g))
+    for wc, ws in ((4, 4), (5, 5), (8, 4), (16, 4), (6, 8)):
+        plan.set_tiles(wc, ws)
+        plan.set_staging(False)
+        a = plan.run(ch).cpu().numpy()
+        plan.set_staging(True)
+        b = plan.run(ch).cpu().numpy()
+        assert np.array_equal(a, b, equal_nan=True), (wc, ws, np.nanmax(np.abs(a - b)))
+    plan.close()
+
+
 @pytest.mark.parametrize("name", ["cpwc_tukey", "dw_sector", "pw_rf_hanning", "cpwc_depth0", "matrix3d_dw",
                                   "focused_hybrid", "sta_hybrid"])
 @pytest.mark.parametrize("interp", [Interpolation.linear, Interpolation.cubic])
 def test_receive_state_on_the_fly(cuda, name, interp):
     """Variant 4 (receive delays and weights computed in the kernel, no
-    receive table: the FP64 delay das.m rounds, the window in FP32) against
+    receive table: das.m's delay from an FP32 square root refined by one FP64
+    Newton step, the clamps, ratios and window in FP32) against
     variant 2 (the resident table of das.m's FP32 matrices) within 2e-6 of the
     peak with identical NaN positions, and against the oracle's FP64
     restatement on its matrices (the separable phase's reference; das_c's

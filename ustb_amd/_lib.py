@@ -108,6 +108,8 @@ EXPORTS = {
     "ustbx_plan_variant": (c_int32, [c_void_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32)]),
     "ustbx_plan_set_order": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
     "ustbx_plan_set_staging": (ctypes.c_int, [c_void_p, c_int32, ctypes.POINTER(Status)]),
+    "ustbx_plan_stage_stats": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64),
+                                              ctypes.POINTER(Status)]),
     "ustbx_plan_materialize": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                               c_void_p, ctypes.POINTER(Status)]),
     "ustbx_plan_active_terms": (ctypes.c_int, [c_void_p, ctypes.POINTER(c_int64), ctypes.POINTER(Status)]),

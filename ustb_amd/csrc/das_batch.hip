@@ -61,7 +61,12 @@ __global__ void batch_reset_kernel(int* ctr) {
 template <int SM, int WC, int WS>
 __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
   __shared__ float2 red[WS][64];
-  __shared__ float4 rxs[kOtfChunk][64];  // receive entries of on-the-fly acquisitions
+  // receive entries of on-the-fly acquisitions, or the staged channels' windows
+  constexpr int kRxs = (int)sizeof(float4) * kOtfChunk * 64;
+  constexpr int kWin = SM == SM_COEF ? 0 : (int)sizeof(float2) * WS * RB * kStageWin;
+  __shared__ __align__(16) char pool[kRxs > kWin ? kRxs : kWin];
+  float4(*rxs)[64] = reinterpret_cast<float4(*)[64]>(pool);
+  float2* win = reinterpret_cast<float2*>(pool);
   __shared__ int claimed;
   const int home = (int)(blockIdx.x % kQueues);  // blocks reach the XCDs round-robin
   for (;;) {
@@ -93,7 +98,7 @@ __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
     } else if (w[3]) {  // a nonzero weight or its delay not finite on this tile
       rxtab_tile<SM, WC, WS, true>(load_constant(B.acqs + acq), tile, frame, red);
     } else {
-      rxtab_tile<SM, WC, WS, false>(load_constant(B.acqs + acq), tile, frame, red);
+      rxtab_tile<SM, WC, WS, false, RXL_EITHER, true>(load_constant(B.acqs + acq), tile, frame, red, win);
     }
     __syncthreads();  // all threads are past this item before the next claim
   }

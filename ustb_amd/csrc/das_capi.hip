@@ -448,6 +448,8 @@ void rxtab_free(ustbx_plan* pl) {
   G.rx_table = nullptr;
   G.tile_nc = G.tile_ncf = nullptr;
   G.nan_mask = nullptr;
+  G.stage_mask = nullptr;
+  G.stage_words = 0;
   G.tile_chl = G.tile_waves = G.tile_nw = nullptr;
   G.ref_table = nullptr;
   G.tile_order = nullptr;
@@ -480,8 +482,11 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err, bool otf) {
   const size_t b_ref = G.ref_phase ? al(sizeof(float2) * P) : 0;
   const size_t b_ord = al(sizeof(int) * tiles);
   const size_t b_mask = al(sizeof(unsigned long long) * tiles);
-  const size_t b_otf = otf ? al(sizeof(RxOtf)) : 0;
-  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + 2 * b_ord + b_mask + b_otf;
+  const size_t b_otf = otf ? al(sizeof(RxOtf)) + al(sizeof(float4) * (size_t)G.N) : 0;
+  // stage mask: dimension both with the receive table
+  const int stage_words = (!otf && G.dim == 3) ? (G.N + 63) / 64 : 0;
+  const size_t b_smask = al(sizeof(unsigned long long) * tiles * (size_t)stage_words);
+  const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + 2 * b_ord + b_mask + b_otf + b_smask;
   char* m = nullptr;
   err = hipMalloc(&m, total);
   if (err != hipSuccess) return false;
@@ -501,8 +506,14 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err, bool otf) {
   int* ord = (int*)take(b_ord);
   int* nat = (int*)take(b_ord);
   unsigned long long* nmask = (unsigned long long*)take(b_mask);
-  RxOtf* dotf = b_otf ? (RxOtf*)take(b_otf) : nullptr;
-  if (dotf) err = hipMemcpy(dotf, &G.otf, sizeof(RxOtf), hipMemcpyHostToDevice);
+  RxOtf* dotf = b_otf ? (RxOtf*)take(al(sizeof(RxOtf))) : nullptr;
+  float4* e32 = b_otf ? (float4*)take(b_otf - al(sizeof(RxOtf))) : nullptr;
+  unsigned long long* smask = b_smask ? (unsigned long long*)take(b_smask) : nullptr;
+  if (dotf) {
+    G.otf.e32 = e32;
+    err = launch_otf_elements(G, e32, nullptr);
+    if (err == hipSuccess) err = hipMemcpy(dotf, &G.otf, sizeof(RxOtf), hipMemcpyHostToDevice);
+  }
   G.d_otf = dotf;
   if (err == hipSuccess) err = launch_rxtab_tables(G, rx, ref, tnc, tcl, twl, tnw, tbad, nmask, nullptr);
   if (err == hipSuccess) err = hipDeviceSynchronize();
@@ -556,6 +567,17 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err, bool otf) {
   G.ref_table = ref;
   G.tile_order = ord;
   G.tile_good = (int)tiles - n_bad;
+  if (smask) {
+    G.stage_mask = smask;
+    G.stage_words = stage_words;
+    err = launch_stage_mask(G, tbad, smask, nullptr);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err != hipSuccess) {
+      rxtab_free(pl);
+      pl->rxtab_state = -1;
+      return false;
+    }
+  }
   return true;
 }
 
@@ -933,7 +955,10 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   G.wave_chunk = dasc_order ? 1 : (G.separable ? (W >= 16 ? 16 : 8) : 8);
   G.split = dasc_order ? 1 : ((G.dim == 3 || G.dim == 2) ? auto_split(P) : 1);
   G.variant = 0;
-  G.stage = 1;
+  // LDS-staged sample windows: on for dimension both (where the plan's stage
+  // mask marks channels); off for dimension transmit, where its per-(channel,
+  // wave) window search measured slower than the gathers
+  G.stage = G.dim == 3 ? 1 : 0;
   if (!dasc_order && shared_rx_applies(G)) {
     G.variant = 1;
     shared_rx_default_tiles(W, G.wave_chunk, G.split);
@@ -1008,6 +1033,8 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   G.otf.ez = G.ez;
   G.otf.inv_c = 1.0 / G.c;
   G.otf.lens = G.lens;
+  G.otf.inv_c32 = (float)(1.0 / G.c);
+  G.otf.lens32 = (float)G.lens;
   G.otf.wd = G.wd;
   G.otf.compact = G.wd == 0.0f ? 1 : 0;
   {
@@ -1025,6 +1052,14 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     a.oy = G.rx.oy;
     a.oz = G.rx.oz;
     a.win0 = (float)G.rx.win0;
+    G.otf.f0f = (float)a.f0;
+    G.otf.f1f = (float)a.f1;
+    G.otf.min0f = (float)a.min0;
+    G.otf.min1f = (float)a.min1;
+    G.otf.max0f = (float)a.max0;
+    G.otf.max1f = (float)a.max1;
+    G.otf.oxf = (float)a.ox;
+    G.otf.oyf = (float)a.oy;
   }
   if (rxtab_build(pl, e, otf_default(pl))) {
     G.variant = 2;
@@ -1300,6 +1335,26 @@ int ustbx_plan_set_staging(ustbx_plan* plan, int32_t on, ustbx_status* st) {
   if (!plan) return set_status(st, USTBX_E_ARG, "NULL plan");
   if (on != 0 && on != 1) return set_status(st, USTBX_E_ARG, "staging must be 0 or 1");
   plan->G.stage = on;
+  return ok(st);
+}
+
+int ustbx_plan_stage_stats(const ustbx_plan* plan, int64_t* staged, int64_t* listed, ustbx_status* st) {
+  if (!plan || !staged || !listed) return set_status(st, USTBX_E_ARG, "NULL argument");
+  *staged = 0;
+  *listed = 0;
+  const GeomLaunch& G = plan->G;
+  if (plan->rxtab_state <= 0 || !G.tile_nc) return ok(st);
+  const size_t tiles = ((size_t)G.P + 63) / 64;
+  std::vector<int> nc(tiles);
+  std::vector<unsigned long long> m(tiles * (size_t)G.stage_words);
+  hipError_t e = hipSetDevice(plan->device);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(nc.data(), G.tile_nc, sizeof(int) * tiles, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !m.empty())
+    e = hipMemcpy(m.data(), G.stage_mask, sizeof(unsigned long long) * m.size(), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(st, e, "stage statistics");
+  for (size_t t = 0; t < tiles; ++t) *listed += nc[t];
+  for (unsigned long long w : m) *staged += __builtin_popcountll(w);
   return ok(st);
 }
 

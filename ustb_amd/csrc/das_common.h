@@ -514,34 +514,115 @@ struct RxOtf {
   const double* ex;
   const double* ey;
   const double* ez;
+  const float4* e32;  // [N] {ex, ey, ez, 0} rounded to FP32
   double inv_c, lens;
+  float inv_c32, lens32;  // 1/c and the lens delay rounded to FP32
   float wd;     // das_c's wd (the separable receive factor)
   int compact;  // RF (wd == 0): the factor is (ra, 0), as the compact table
   RxOtfApod ap;
+  float f0f, f1f, min0f, min1f, max0f, max1f;  // the aperture parameters rounded to FP32
+  float oxf, oyf;
 };
 
-// {rd, ra, ra*cos(wd rd), ra*sin(wd rd)} of (pixel (px, py, pz), element n):
-// the receive-table entry computed in place, with das.m's delay
-// single(sqrt(xm^2+ym^2+zm^2)/c + lens) in FP64 (the quotient by c taken as
-// a product with 1/c, which can move the FP32 rounding by one ulp in rare
-// ties) and the weight of rx_apod_otf.
-__device__ __forceinline__ float4 rx_otf_entry(const RxOtf& o, double px, double py, double pz, int n) {
+// A pixel as the on-the-fly entries read it: its coordinates rounded to
+// FP32 and, for an origin-centred aperture, its distance from the origin
+// (FP64, rounded).
+struct OtfPixel {
+  double px, py, pz;
+  float x, y, z, zo;
+};
+
+__device__ __forceinline__ OtfPixel otf_pixel(const RxOtf& o, double px, double py, double pz) {
 #pragma clang fp contract(off)
-  const double ex = o.ex[n], ey = o.ey[n], ez = o.ez[n];
-  const double xm = ex - px, ym = ey - py, zm = ez - pz;
-  const float rd = (float)(sqrt((xm * xm + ym * ym) + zm * zm) * o.inv_c + o.lens);
-  double xd, yd, zd;
-  if (o.ap.geo == RXGEO_FLAT) {
-    xd = px - ex;
-    yd = py - ey;
-    zd = pz - ez;
-  } else {
+  OtfPixel q;
+  q.px = px;
+  q.py = py;
+  q.pz = pz;
+  q.x = (float)px;
+  q.y = (float)py;
+  q.z = (float)pz;
+  q.zo = 0.0f;
+  if (o.ap.geo != RXGEO_FLAT) {
     const double dx = px - o.ap.ox, dy = py - o.ap.oy, dz = pz - o.ap.oz;
-    xd = ex - o.ap.ox;
-    yd = ey - o.ap.oy;
-    zd = sqrt((dx * dx + dy * dy) + dz * dz);
+    q.zo = (float)sqrt((dx * dx + dy * dy) + dz * dz);
   }
-  const float ra = rx_apod_otf(o.ap, xd, yd, zd);
+  return q;
+}
+
+// |f * (x / z)| in FP32 with the hardware reciprocal; near is set where the
+// ratio is within 1e-5 of 1/2 for a window that jumps there (boxcar, Hamming),
+// where FP32 rounding could put it on the other side of the edge than das.m's
+// FP64 quotient.
+__device__ __forceinline__ float otf_ratio32(int window, float f, float x, float z, bool& near) {
+  const float r = fabsf(f * (x * __builtin_amdgcn_rcpf(z)));
+  near = near || ((window == WIN_BOXCAR || window == WIN_HAMMING) && fabsf(r - 0.5f) < 1e-5f);
+  return r;
+}
+
+// das.m's receive delay single(sqrt(xm^2+ym^2+zm^2)/c + lens) (das.m:103-106)
+// without an FP64 square root or division: the FP64 sum of squares, its FP32
+// square root, one Newton step in FP64 (relative error about 1e-14 against
+// the correctly rounded root), times 1/c. The FP32 result equals das.m's
+// except where its FP64 value lies within about 1e-14 (relative) of an FP32
+// rounding boundary, where it may differ by one ulp.
+__device__ __forceinline__ float otf_rx_delay(const RxOtf& o, const OtfPixel& q, int n) {
+  const double xm = o.ex[n] - q.px, ym = o.ey[n] - q.py, zm = o.ez[n] - q.pz;
+  const double S = fma(xm, xm, fma(ym, ym, zm * zm));
+  const float s32 = __builtin_amdgcn_sqrtf((float)S);
+  const double s0 = (double)s32;
+  const double r = s32 > 0.0f ? fma(fma(-s0, s0, S), (double)(0.5f * __builtin_amdgcn_rcpf(s32)), s0) : 0.0;
+  return (float)fma(r, o.inv_c, o.lens);
+}
+
+// {rd, ra, ra*cos(wd rd), ra*sin(wd rd)} of (pixel q, element n), the
+// receive-table entry computed in place: the delay of otf_rx_delay and the
+// receive weight of apodization.m:415-467 (no tilt) with FP32 coordinate
+// differences, clamps, ratios and window; where a ratio lies within 1e-5 of a
+// window's jump at 1/2 the weight is formed in FP64 instead (rx_apod_otf), so
+// that it falls on das.m's side of the edge. The phase factor reduces wd*rd
+// in FP64, as the table.
+__device__ __forceinline__ float4 rx_otf_entry(const RxOtf& o, const OtfPixel& q, int n) {
+  const float4 e = o.e32[n];
+  const float xm = e.x - q.x, ym = e.y - q.y, zm = e.z - q.z;
+  const float rd = otf_rx_delay(o, q, n);
+  float ra = 1.0f;
+  if (o.ap.kind != RXAP_ONES) {
+    float xd, yd, zd;
+    if (o.ap.geo == RXGEO_FLAT) {
+      xd = -xm;
+      yd = -ym;
+      zd = -zm;
+    } else {
+      xd = e.x - o.oxf;
+      yd = e.y - o.oyf;
+      zd = q.zo;
+    }
+    bool near = false;
+    const float zx = aperture_clamp_f32(zd, o.min0f, o.max0f);
+    const float zy = aperture_clamp_f32(zd, o.min1f, o.max1f);
+    const float rt = otf_ratio32(o.ap.window, o.f0f, xd, zx, near);
+    float wp = o.ap.win0;
+    if (!(yd == 0.0f && fabsf(zy) > 0.0f && fabsf(zy) < INFINITY))
+      wp = win_eval_f32(o.ap.window, otf_ratio32(o.ap.window, o.f1f, yd, zy, near));
+    ra = win_eval_f32(o.ap.window, rt) * wp;
+    if (near) {
+#pragma clang fp contract(off)
+      const double px = q.px, py = q.py, pz = q.pz;
+      const double ex = o.ex[n], ey = o.ey[n], ez = o.ez[n];
+      double xd64, yd64, zd64;
+      if (o.ap.geo == RXGEO_FLAT) {
+        xd64 = px - ex;
+        yd64 = py - ey;
+        zd64 = pz - ez;
+      } else {
+        const double dx = px - o.ap.ox, dy = py - o.ap.oy, dz = pz - o.ap.oz;
+        xd64 = ex - o.ap.ox;
+        yd64 = ey - o.ap.oy;
+        zd64 = sqrt((dx * dx + dy * dy) + dz * dz);
+      }
+      ra = rx_apod_otf(o.ap, xd64, yd64, zd64);
+    }
+  }
   if (o.compact) return make_float4(rd, ra, ra, ra * 0.0f);
   float sv, cv;
   sincos_wd_hw(o.wd, rd, sv, cv);

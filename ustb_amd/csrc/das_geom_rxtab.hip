@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void geom_ref_table_kernel(GeomLaunch G, fl
 // the tile body will use, so both agree on which weights are nonzero).
 __device__ __forceinline__ float4 rx_at(const GeomLaunch& G, const float4* __restrict__ rxt, int n, int p,
                                         double px, double py, double pz) {
-  if (G.rx_otf) return rx_otf_entry(G.otf, px, py, pz, n);
+  if (G.rx_otf) return rx_otf_entry(G.otf, otf_pixel(G.otf, px, py, pz), n);
   return rx_entry(rxt, G.rx_compact, (size_t)n * G.P + p);
 }
 
@@ -147,6 +147,61 @@ __global__ __launch_bounds__(64) void geom_nan_mask_kernel(GeomLaunch G, const f
   if (threadIdx.x == 0) mask[tile] = m;
 }
 
+// Per tile and channel-list position, whether the channel's terms on the
+// tile can run staged (rxtab_channel_staged): every lane of the tile active;
+// the channel's receive weight and every transmit weight of the tile's waves
+// nonzero on every lane, with finite delays; every term's index in the sample
+// window; and every lane's n0 in [n0 of lane 0, that + kMaxSpan], by the tile
+// body's own index arithmetic (term_index). Tiles with a non-finite nonzero
+// weight or delay (run by the SAFE body) get no bits. One wave64 per tile.
+template <int SM>
+__global__ __launch_bounds__(64) void geom_stage_mask_kernel(GeomLaunch G, const int* __restrict__ tile_bad,
+                                                             unsigned long long* __restrict__ mask) {
+  const int tile = blockIdx.x;
+  const int lane = (int)threadIdx.x;
+  const int p = tile * 64 + lane;
+  unsigned long long* out = mask + (size_t)tile * G.stage_words;
+  const bool run = tile_bad[tile] == 0 && tile * 64 + 63 < G.P;
+  const int nc = run ? G.tile_nc[tile] : 0;
+  const int nw = G.tile_nw[tile];
+  const int* __restrict__ chl = G.tile_chl + (size_t)tile * G.N;
+  const int* __restrict__ wl = G.tile_waves + (size_t)tile * G.W;
+  const size_t PS = (size_t)G.P;
+  const float lim = (float)(SM == SM_LINEAR ? G.L - 1 : G.L - 2);
+  unsigned long long word = 0ull;
+  for (int k = 0; k < G.stage_words * 64; ++k) {
+    if (k < nc) {
+      const float4 rv = rx_entry(G.rx_table, G.rx_compact, (size_t)chl[k] * PS + p);
+      bool ok = rv.y != 0.0f && isfinite(rv.x);
+      for (int i = 0; i < nw && __all(ok); ++i) {
+        const float4 tv = G.tx_table[(size_t)wl[i] * PS + p];
+        const float d = term_index(tv.x, rv.x, G.t0, G.fs);
+        const int n0 = (int)d;
+        const int n00 = __builtin_amdgcn_readfirstlane(n0);
+        ok = ok && tv.y != 0.0f && isfinite(tv.x) && d >= 1.0f && d < lim && n0 >= n00 &&
+             n0 - n00 <= StageWin<SM>::kMaxSpan;
+      }
+      if (__all(ok)) word |= 1ull << (k & 63);
+    }
+    if ((k & 63) == 63) {
+      if (lane == 0) out[k >> 6] = word;
+      word = 0ull;
+    }
+  }
+}
+
+hipError_t launch_stage_mask(const GeomLaunch& G, const int* tile_bad, unsigned long long* stage_mask,
+                             hipStream_t stream) {
+  if (!G.stage_words || !G.rx_table || G.dim != 3 || G.rx_otf) return hipErrorInvalidValue;
+  const unsigned tiles = (unsigned)((G.P + 63) / 64);
+  if (G.interp == INTERP_CUBIC)
+    hipLaunchKernelGGL(geom_stage_mask_kernel<SM_CUBIC>, dim3(tiles), dim3(64), 0, stream, G, tile_bad, stage_mask);
+  else
+    hipLaunchKernelGGL(geom_stage_mask_kernel<SM_LINEAR>, dim3(tiles), dim3(64), 0, stream, G, tile_bad,
+                       stage_mask);
+  return hipGetLastError();
+}
+
 // Catmull-Rom coefficients of the interval [n, n+1] of every trace:
 // p(b) = a0 + a1 b + a2 b^2 + a3 b^3 through samples n-1 .. n+2, the same
 // polynomial as Keys' kernel with a = -1/2. Stored per sample as two float4,
@@ -174,12 +229,20 @@ __global__ __launch_bounds__(kBlock) void spline_coef_kernel(const float2* __res
 }
 
 // dimension.both from a plan's resident tables: one block per (tile, frame).
+// LDS slots of the staged channels: RB windows per split (none for
+// coefficient data, which is never staged).
+template <int SM, int WS>
+struct StageLds {
+  static constexpr int kFloat2 = SM == SM_COEF ? 1 : WS * RB * kStageWin;
+};
+
 template <int SM, int WC, int WS, int RXL>
 __global__ __launch_bounds__(64 * WS) void das_both_rxtab_kernel(RxAcq A, const int* __restrict__ order) {
   __shared__ float2 red[WS][64];
+  __shared__ float2 win[StageLds<SM, WS>::kFloat2];
   int tile = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (order) tile = __builtin_amdgcn_readfirstlane(order[tile]);  // block-uniform
-  rxtab_tile<SM, WC, WS, false, RXL>(A, tile, (int)blockIdx.y, red);
+  rxtab_tile<SM, WC, WS, false, RXL, true>(A, tile, (int)blockIdx.y, red, win);
 }
 
 // The same where some tiles hold a non-finite nonzero weight or delay (the
@@ -191,6 +254,7 @@ template <int SM, int WC, int WS, int RXL>
 __global__ __launch_bounds__(64 * WS) void das_both_rxtab_mixed_kernel(RxAcq A, const int* __restrict__ order,
                                                                        int good, int bad, int bad8) {
   __shared__ float2 red[WS][64];
+  __shared__ float2 win[StageLds<SM, WS>::kFloat2];
   const int b = (int)blockIdx.x;
   if (b < bad8) {
     if (b >= bad) return;  // whole block, before any barrier
@@ -198,7 +262,7 @@ __global__ __launch_bounds__(64 * WS) void das_both_rxtab_mixed_kernel(RxAcq A, 
     rxtab_tile<SM, WC, WS, true, RXL>(A, tile, (int)blockIdx.y, red);
   } else {
     const int tile = __builtin_amdgcn_readfirstlane(order[xcd_tile(b - bad8, good)]);
-    rxtab_tile<SM, WC, WS, false, RXL>(A, tile, (int)blockIdx.y, red);
+    rxtab_tile<SM, WC, WS, false, RXL, true>(A, tile, (int)blockIdx.y, red, win);
   }
 }
 
@@ -299,6 +363,21 @@ hipError_t launch_both_rxtab(const GeomLaunch& G, hipStream_t stream) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_rxtab_ws<SM_COEF>(G, stream);
+}
+
+// The element positions rounded to FP32 for the on-the-fly receive state.
+__global__ __launch_bounds__(kBlock) void otf_elements_kernel(const double* __restrict__ ex,
+                                                              const double* __restrict__ ey,
+                                                              const double* __restrict__ ez, float4* __restrict__ e32,
+                                                              int N) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n < N) e32[n] = make_float4((float)ex[n], (float)ey[n], (float)ez[n], 0.0f);
+}
+
+hipError_t launch_otf_elements(const GeomLaunch& G, float4* e32, hipStream_t stream) {
+  hipLaunchKernelGGL(otf_elements_kernel, dim3((unsigned)((G.N + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
+                     G.ex, G.ey, G.ez, e32, G.N);
+  return hipGetLastError();
 }
 
 hipError_t launch_rxtab_tables(const GeomLaunch& G, float4* rx_table, float2* ref_table, int* tile_nc,

@@ -78,6 +78,11 @@ struct GeomLaunch {
   const int* tile_nc;
   const int* tile_ncf;  // [tiles] leading entries of the channel list with a weight other than NaN
   const unsigned long long* nan_mask;  // [tiles] lanes whose sum the NaN-only channels make NaN
+  // Dimension both (receive table, linear or cubic): per tile, stage_words
+  // 64-bit words; bit k set when the channel at list position k can read
+  // its samples from a 64-sample LDS window (rxtab_channel_staged), or null.
+  const unsigned long long* stage_mask;
+  int stage_words;
   const int* tile_chl;
   const int* tile_waves;
   const int* tile_nw;
@@ -89,8 +94,9 @@ struct GeomLaunch {
   // are all finite; the order is non-null when tile_good < tiles.
   const int* tile_order;
   int tile_good;
-  // Dimension transmit (variant 2): stage each tile's sample windows through
-  // LDS (1) or gather every term's samples (0). Results do not depend on it.
+  // Stage sample windows through LDS (1) or gather every term's samples
+  // (0): dimension both through stage_mask, dimension transmit (variant 2)
+  // per (channel, wave). Results do not depend on it.
   int stage;
 };
 
@@ -109,6 +115,12 @@ size_t tx_cf_workspace_bytes(const GeomLaunch& G);
 hipError_t launch_rxtab_tables(const GeomLaunch& g, float4* rx_table, float2* ref_table, int* tile_nc,
                                int* tile_chl, int* tile_waves, int* tile_nw, int* tile_bad,
                                unsigned long long* nan_mask, hipStream_t stream);
+// Rounds the element positions to FP32 ([N] float4) for the on-the-fly state.
+hipError_t launch_otf_elements(const GeomLaunch& g, float4* e32, hipStream_t stream);
+// Fills stage_mask ([tiles][stage_words]) from the variant-2 tables (tile_bad
+// as launch_rxtab_tables left it; those tiles get no bits).
+hipError_t launch_stage_mask(const GeomLaunch& g, const int* tile_bad, unsigned long long* stage_mask,
+                             hipStream_t stream);
 
 hipError_t launch_das_geom(const GeomLaunch& g, hipStream_t stream);
 

@@ -62,6 +62,8 @@ struct RxAcq {
   const int* tile_nc;      // [tiles] channels with a nonzero receive weight
   const int* tile_ncf;     // [tiles] leading entries of tile_chl with a weight other than NaN
   const unsigned long long* nan_mask;  // [tiles] lanes made NaN by the list's NaN-only tail
+  const unsigned long long* stage_mask;  // [tiles][stage_words] staged list positions, or null
+  int stage_words;
   const int* tile_chl;     // [tiles][N] their indices, ascending
   const int* tile_nw;      // [tiles] waves with a nonzero transmit weight
   const int* tile_waves;   // [tiles][W] their indices, ascending
@@ -95,6 +97,32 @@ __device__ __forceinline__ void keys_weights_horner(float b, float& c0, float& c
   c1 = fmaf(b2, fmaf(1.5f, b, -2.5f), 1.0f);
   c2 = b * fmaf(b, fmaf(-1.5f, b, 2.0f), 0.5f);
   c3 = b2 * fmaf(0.5f, b, -0.5f);
+}
+
+// das_c's sample index of a term (das_c.cpp:372,375), FMA contraction off:
+// the staged tile body and the plan's stage mask form it the same way.
+__device__ __forceinline__ float term_index(float td, float rd, float t0, float fs) {
+#pragma clang fp contract(off)
+  return ((td + rd) - t0) * fs;
+}
+
+// Staged channels: every term of the channel on the tile reads its samples
+// from one window of kStageWin samples of its trace, loaded by the wave in
+// one coalesced 8-byte-per-lane load and read back from LDS. The window
+// starts at the first sample lane 0's term reads (n0 - kBack); the plan marks
+// a channel staged only when every lane's n0 lies in [n0 of lane 0, that +
+// kMaxSpan], so that every sample a term reads is in the window.
+constexpr int kStageWin = 64;
+template <int SM>
+struct StageWin {
+  static constexpr int kBack = SM == SM_CUBIC ? 1 : 0;      // samples before n0 a term reads
+  static constexpr int kMaxSpan = SM == SM_CUBIC ? 60 : 62;  // kStageWin - (samples a term reads)
+};
+
+__device__ __forceinline__ float2 buf_load8(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  typedef unsigned u2v __attribute__((ext_vector_type(2)));
+  const u2v v = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+  return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
 }
 
 // One channel of one split: part = sum_j T_j * sample_j over the split's
@@ -179,6 +207,85 @@ __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const
   return part;
 }
 
+// rxtab_channel for a staged channel (every weight nonzero, every term in the
+// sample window): the same sum, term for term. Per batch of RB terms the wave
+// loads each term's window (lane l: sample base + l), writes the windows to
+// its LDS slots win[i][64], and each lane reads its term's samples from there
+// (the LDS operations of one wave complete in order, so the reads see the
+// writes). Slots past the split's waves (GUARD) load zeros and have T = 0.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) f2v lds_f2;
+typedef const __attribute__((address_space(3))) f4a8 lds_f4a8;
+
+// x * 8 + y in one VALU instruction (y wave-uniform, in an SGPR).
+__device__ __forceinline__ unsigned lshl3_add(unsigned x, unsigned y) {
+  unsigned r;
+  asm("v_lshl_add_u32 %0, %1, 3, %2" : "=v"(r) : "v"(x), "s"(y));
+  return r;
+}
+
+template <int SM, int WC, bool GUARD>
+__device__ __forceinline__ float2 rxtab_channel_staged(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
+                                                       const float2 (&T)[WC], const int (&wb)[WC], int nj,
+                                                       float rd, int nb, float t0, float fs,
+                                                       float2* __restrict__ win, unsigned lane8) {
+  static_assert(SM != SM_COEF, "coefficient data is not staged");
+  using S = StageWin<SM>;
+  const unsigned wbase = (unsigned)(size_t)(__attribute__((address_space(3))) char*)win;  // LDS address
+  float2 part = make_float2(0.0f, 0.0f);
+#pragma unroll
+  for (int j0 = 0; j0 < WC; j0 += RB) {
+    if (GUARD && j0 >= nj) break;
+    float2 q[RB];
+    float bb[RB];
+    unsigned la[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int j = j0 + i;
+      if (j < WC) {
+        const float d = term_index(td[j], rd, t0, fs);
+        const int n0 = (int)d;
+        bb[i] = __builtin_amdgcn_fractf(d);
+        const unsigned n00 = (unsigned)__builtin_amdgcn_readfirstlane(n0);
+        const bool live = !GUARD || j < nj;  // wave-uniform
+        // the window's byte offset in the frame (scalar) plus 8 * lane
+        const unsigned sb = live ? (unsigned)(nb + wb[j]) + (n00 - (unsigned)S::kBack) * 8u : (unsigned)kOOB;
+        q[i] = buf_load8(rs, (int)(sb + lane8));
+        // LDS address of the first sample the term reads, in slot i
+        la[i] = lshl3_add((unsigned)n0, wbase + (unsigned)(i * kStageWin * 8) - n00 * 8u);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      if (j0 + i < WC) *reinterpret_cast<lds_f2*>((size_t)(wbase + lane8) + i * kStageWin * 8) = f2v{q[i].x, q[i].y};
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int j = j0 + i;
+      if (j < WC) {
+        const float b = bb[i];
+        const lds_f4a8* wp = reinterpret_cast<const lds_f4a8*>((size_t)la[i]);
+        float2 sm;
+        if (SM == SM_LINEAR) {
+          const f4a8 x = wp[0];
+          sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
+        } else {
+          const f4a8 lo = wp[0];
+          const f4a8 hi = wp[1];
+          float c0, c1, c2, c3;
+          keys_weights_horner(b, c0, c1, c2, c3);
+          sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
+                           fmaf(c3, hi.w, fmaf(c2, hi.y, fmaf(c1, lo.w, c0 * lo.y))));
+        }
+        cfma(part, T[j], sm);
+      }
+    }
+    asm volatile("" : "+v"(part.x), "+v"(part.y));
+    __builtin_amdgcn_wave_barrier();  // this batch's reads before the next batch's writes
+  }
+  return part;
+}
+
 // One channel's fold: acc += R * (its partial sum over the split's waves).
 template <int SM, int WC, bool GUARD, bool SAFE>
 __device__ __forceinline__ void rxtab_fold(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC], const float2 (&T)[WC],
@@ -198,12 +305,14 @@ __device__ __forceinline__ void rxtab_fold(__amdgpu_buffer_rsrc_t rs, const floa
   }
 }
 
-template <int SM, int WC, bool GUARD, bool SAFE, bool COMPACT>
+template <int SM, int WC, bool GUARD, bool SAFE, bool COMPACT, bool STG>
 __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
                                                const float2 (&T)[WC], const int (&wb)[WC], int nj,
                                                const float4* __restrict__ rxt, const int* __restrict__ chl,
                                                int cstride, int nc, size_t PS, int p, bool active, int lst,
-                                               float t0, float fs, float lim, float2& acc) {
+                                               float t0, float fs, float lim, float2& acc,
+                                               const unsigned long long* __restrict__ smask, int pos0,
+                                               float2* __restrict__ win, unsigned lane8) {
   // channels chl[0], chl[cstride], ... (nc of them); the receive row of the
   // next one in flight during this one, its index one further ahead (a
   // scalar load off the critical path). (Four rows in flight for splits of
@@ -221,6 +330,19 @@ __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const 
       if (active) rn = rx_entry_t<COMPACT>(rxt, (size_t)n * PS + p);
       if (k + 2 < nc) n_next = __builtin_amdgcn_readfirstlane(chl[(k + 2) * cstride]);
     }
+    // a staged channel (list position pos0 + k * cstride) has every weight
+    // nonzero: R is its entry's factor, as in rxtab_fold
+    if constexpr (STG) {
+      // the mask word through the constant address space (a scalar load; the
+      // plan's mask is read-only while kernels run)
+      const int pos = pos0 + k * cstride;
+      if (smask && ((((const __attribute__((address_space(4))) unsigned long long*)smask)[pos >> 6] >> (pos & 63)) &
+                    1ull)) {
+        const float2 part = rxtab_channel_staged<SM, WC, GUARD>(rs, td, T, wb, nj, rv.x, nb, t0, fs, win, lane8);
+        cfma(acc, make_float2(rv.z, rv.w), part);
+        continue;
+      }
+    }
     rxtab_fold<SM, WC, GUARD, SAFE>(rs, td, T, wb, nj, rv, nb, t0, fs, lim, acc);
   }
 }
@@ -230,9 +352,12 @@ __device__ __forceinline__ void rxtab_channels(__amdgpu_buffer_rsrc_t rs, const 
 enum { RXL_FULL = 0, RXL_COMPACT = 1, RXL_EITHER = 2 };
 
 // One tile of one frame; every thread of the block calls it (it ends with a
-// block barrier and the split-ordered reduction through red).
-template <int SM, int WC, int WS, bool SAFE = false, int RXL = RXL_EITHER>
-__device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, float2 (*red)[64]) {
+// block barrier and the split-ordered reduction through red). STG: the
+// channels A.stage_mask marks run staged, split s through the LDS slots
+// win[s * RB * kStageWin ..] (SAFE tiles never).
+template <int SM, int WC, int WS, bool SAFE = false, int RXL = RXL_EITHER, bool STG = false>
+__device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, float2 (*red)[64],
+                                           float2* win = nullptr) {
   constexpr int TP = 64;
   constexpr int ST = SampleMode<SM>::kStride;
   const int lane = threadIdx.x & (TP - 1);
@@ -265,6 +390,14 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
   const int g = s / wsp;
   const int sw = s - g * wsp;           // split within the group
   const int ncg = g < cg ? (nc - g + cg - 1) / cg : 0;
+  constexpr bool kStg = STG && !SAFE && SM != SM_COEF;
+  const unsigned long long* smask =
+      kStg && A.stage_mask ? A.stage_mask + (size_t)tile * (size_t)A.stage_words : nullptr;
+  float2* swin = kStg ? win + s * (RB * kStageWin) : nullptr;
+  // 8 * lane, opaque to the compiler so that window addresses stay one
+  // scalar-plus-vector add
+  unsigned lane8 = (unsigned)lane * 8u;
+  asm volatile("" : "+v"(lane8));
 
   float2 acc = make_float2(0.0f, 0.0f);
   for (int k0 = 0; ncg > 0 && k0 < nw; k0 += wsp * WC) {
@@ -293,18 +426,18 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
     }
     if (nj == WC) {
       if (compact)
-        rxtab_channels<SM, WC, false, SAFE, true>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst,
-                                                  A.t0, A.fs, lim, acc);
+        rxtab_channels<SM, WC, false, SAFE, true, kStg>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active,
+                                                        lst, A.t0, A.fs, lim, acc, smask, g, swin, lane8);
       else
-        rxtab_channels<SM, WC, false, SAFE, false>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst,
-                                                   A.t0, A.fs, lim, acc);
+        rxtab_channels<SM, WC, false, SAFE, false, kStg>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active,
+                                                         lst, A.t0, A.fs, lim, acc, smask, g, swin, lane8);
     } else {
       if (compact)
-        rxtab_channels<SM, WC, true, SAFE, true>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst,
-                                                 A.t0, A.fs, lim, acc);
+        rxtab_channels<SM, WC, true, SAFE, true, kStg>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active,
+                                                       lst, A.t0, A.fs, lim, acc, smask, g, swin, lane8);
       else
-        rxtab_channels<SM, WC, true, SAFE, false>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active, lst,
-                                                  A.t0, A.fs, lim, acc);
+        rxtab_channels<SM, WC, true, SAFE, false, kStg>(rs, td, T, wb, nj, A.rx, chl + g, cg, ncg, PS, p, active,
+                                                        lst, A.t0, A.fs, lim, acc, smask, g, swin, lane8);
     }
   }
   red[s][lane] = acc;
@@ -327,7 +460,8 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
 // group from there, in list order, as rxtab_channels does from the table. All
 // waves run every round and chunk (the barriers); a split with no waves in a
 // round only computes. Given the same entries the sums are rxtab_tile's, term
-// for term.
+// for term. (Two LDS buffers, filling the next chunk while folding this one
+// with one barrier per chunk, measured no faster on the 3-D class.)
 constexpr int kOtfChunk = 16;
 
 template <int SM, int WC, int WS, bool SAFE>
@@ -394,9 +528,10 @@ __device__ __forceinline__ void rxotf_tile(const RxAcq& A, int tile, int f, floa
         asm volatile("" : "+s"(op));
         double px = 0.0, py = 0.0, pz = 0.0, rref = 0.0;
         pixel_xyz(op->scan, active ? p : 0, px, py, pz, rref);
+        const OtfPixel q = otf_pixel(*op, px, py, pz);
         for (int e = s; e < cn; e += WS) {
           const int n = __builtin_amdgcn_readfirstlane(chl[c0 + e]);
-          rxs[e][lane] = active ? rx_otf_entry(*op, px, py, pz, n) : make_float4(0.f, 0.f, 0.f, 0.f);
+          rxs[e][lane] = active ? rx_otf_entry(*op, q, n) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
       }
       __syncthreads();
@@ -436,6 +571,8 @@ inline RxAcq rx_acq(const GeomLaunch& G, int sm) {
   A.tile_nc = G.tile_nc;
   A.tile_ncf = G.tile_ncf;
   A.nan_mask = G.nan_mask;
+  A.stage_mask = G.stage && sm != SM_COEF ? G.stage_mask : nullptr;
+  A.stage_words = G.stage_words;
   A.tile_chl = G.tile_chl;
   A.tile_nw = G.tile_nw;
   A.tile_waves = G.tile_waves;

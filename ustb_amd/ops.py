@@ -272,10 +272,21 @@ class GeometryPlan:
         L.check(L.lib().ustbx_plan_set_order(self.handle, int(order), ctypes.byref(st)), st)
 
     def set_staging(self, on: bool):
-        """Dimension transmit: stage the tiles' sample windows through LDS
-        (default) or gather each term's samples; results are the same."""
+        """Stage sample windows through LDS or gather each term's samples;
+        results are the same. Default: on for dimension both (the channels
+        the plan marked), off for dimension transmit."""
         st = L.Status()
         L.check(L.lib().ustbx_plan_set_staging(self.handle, int(bool(on)), ctypes.byref(st)), st)
+
+    @property
+    def stage_stats(self) -> tuple[int, int]:
+        """(staged, listed) (tile, channel) pairs of a dimension-both
+        receive-table plan."""
+        a, b, st = ctypes.c_int64(), ctypes.c_int64(), L.Status()
+        with torch.cuda.device(self.device):
+            L.check(L.lib().ustbx_plan_stage_stats(self.handle, ctypes.byref(a), ctypes.byref(b),
+                                                   ctypes.byref(st)), st)
+        return a.value, b.value
 
     def set_tiles(self, wave_chunk: int = 0, split: int = 0):
         st = L.Status()
