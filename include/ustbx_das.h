@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define USTBX_ABI_VERSION 2
+#define USTBX_ABI_VERSION 3
 
 /* error codes, with the identifier das_c raises each class with
  * (ustbx_status_id; +mex/source/das_c.cpp:53-252, +mex/source/das_cuda.cu:122).
@@ -210,6 +210,14 @@ typedef struct ustbx_wave {
   double origin[3];     /* wave.origin (wave.m:46) */
 } ustbx_wave;
 
+/* Probe classes whose apodization geometry differs (apodization.m:377-400,
+ * 506-518): flat (linear, matrix, any element list) or curvilinear
+ * (uff.curvilinear_array, uff.curvilinear_matrix_array: the receive aperture
+ * in azimuth around the origin, default (0, 0, -radius); the STA transmit
+ * axis from (0, 0, -radius) to the source). */
+#define USTBX_PROBE_FLAT 0
+#define USTBX_PROBE_CURVILINEAR 1
+
 typedef struct ustbx_apod {
   int32_t window;       /* USTBX_WIN_*; NONE -> ones */
   int32_t _pad;
@@ -254,6 +262,10 @@ typedef struct ustbx_geometry {
   float fs, t0, fd;
   int32_t dim, interp, edge;
   uint32_t flags;
+  int32_t probe_kind;         /* USTBX_PROBE_*: which incidence geometry apodization.m uses */
+  int32_t _pad2;
+  double probe_radius;        /* USTBX_PROBE_CURVILINEAR: radius (uff.curvilinear_array) or
+                                 radius_x (uff.curvilinear_matrix_array) [m] */
 } ustbx_geometry;
 
 typedef struct ustbx_plan ustbx_plan;
@@ -287,7 +299,14 @@ int ustbx_plan_set_tiles(ustbx_plan* plan, int32_t wave_chunk, int32_t split,
  *       data) it keeps das_c's term arithmetic and summation order;
  *   3 = variant 2 (dimension both) over cubic-interpolation coefficients: the plan holds a
  *       buffer of 32 bytes per channel sample that every run rewrites, so
- *       runs of one variant-3 plan on different streams must not overlap.
+ *       runs of one variant-3 plan on different streams must not overlap;
+ *   4 = variant 2 without the receive table (dimension both, a windowed or
+ *       no receive apodization without tilt, flat probes): the blocks compute
+ *       their tile's receive delays (das.m's, to the FP32 rounding) and
+ *       weights (FP32) in place, 16 channels at a time through LDS. The plan
+ *       keeps the transmit table and the tile lists only (the 128 x 128 x 256
+ *       volume of a 32 x 32 array: 1.8 GiB against 65.8 GiB); it is the
+ *       default where the receive table would exceed half the device memory.
  * USTBX_E_UNSUPPORTED where the variant does not apply. */
 int ustbx_plan_set_variant(ustbx_plan* plan, int32_t variant, ustbx_status* st);
 /* Tile visiting order of variants 2 and 3 (no reference counterpart):

@@ -481,14 +481,29 @@ def _clamp(z, min_af, max_af):
 
 
 def rx_apod(px, py, pz, ex, ey, ez, window_kind, f_number=(1.0, 1.0), minimum_aperture=(1e-3, 1e-3),
-            maximum_aperture=(10.0, 10.0), tilt=(0.0, 0.0), origin=None):
-    """Receive apodization (apodization.m:323-468) for flat probes.
-    origin=None: aperture centred on each pixel (linear scan); else centred on
-    the origin (sector scan or a user origin). Returns (N, P) float32."""
+            maximum_aperture=(10.0, 10.0), tilt=(0.0, 0.0), origin=None, curvilinear=False):
+    """Receive apodization (apodization.m:323-468).
+    Flat probes: origin=None centres the aperture on each pixel (linear
+    scan), else on the origin (sector scan or a user origin).
+    curvilinear=True (uff.curvilinear_array / curvilinear_matrix_array,
+    apodization.m:388-398): origin is the arc's centre (0, 0, -radius) unless
+    the user set one; x_dist = origin.z * (pixel azimuth - element azimuth),
+    both azimuths atan2(x - origin.x, z - origin.z); y_dist = origin.y - y;
+    z_dist = hypot(pixel.x - origin.x, pixel.z - origin.z) - origin.z.
+    Returns (N, P) float32."""
     N, P = ex.size, px.size
     if window_kind == WIN_NONE:
         return np.ones((N, P), dtype=np.float32)
-    if origin is None:
+    if curvilinear:
+        x0, y0, z0 = origin
+        el_az = np.arctan2(ex - x0, ez - z0)
+        px_az = np.arctan2(px - x0, pz - z0)
+        dx, dz = px - x0, pz - z0
+        pdist = np.sqrt(dx * dx + dz * dz)
+        xd = z0 * (px_az[:, None] - el_az[None, :])
+        yd = np.broadcast_to(y0 - ey[None, :], (P, N)).copy()
+        zd = np.broadcast_to(pdist[:, None] - z0, (P, N)).copy()
+    elif origin is None:
         xd = px[:, None] - ex[None, :]
         yd = py[:, None] - ey[None, :]
         zd = pz[:, None] - ez[None, :]
@@ -515,9 +530,13 @@ def rx_apod(px, py, pz, ex, ey, ez, window_kind, f_number=(1.0, 1.0), minimum_ap
 
 
 def tx_apod(px, py, pz, waves, window_kind, f_number=(1.0, 1.0), minimum_aperture=(1e-3, 1e-3),
-            maximum_aperture=(10.0, 10.0), tilt=(0.0, 0.0), grating_lobe_angle=(math.pi, math.pi)):
+            maximum_aperture=(10.0, 10.0), tilt=(0.0, 0.0), grating_lobe_angle=(math.pi, math.pi),
+            probe_radius=None):
     """Transmit apodization (apodization.m:249-320, 472-555). waves: dicts with
-    wavefront, src, origin, distance, azimuth, elevation. Returns (W, P) float32."""
+    wavefront, src, origin, distance, azimuth, elevation. probe_radius: a
+    curvilinear probe's radius, whose STA waves (source at the origin) take
+    the axis from (0, 0, -radius) to the source (apodization.m:506-518)
+    instead of [0, 0, 1]. Returns (W, P) float32."""
     W, P = len(waves), px.size
     if window_kind == WIN_NONE:
         return np.ones((W, P), dtype=np.float32)
@@ -534,7 +553,10 @@ def tx_apod(px, py, pz, waves, window_kind, f_number=(1.0, 1.0), minimum_apertur
         OP = (px - o[0], py - o[1], pz - o[2])
         OS = s - o
         if math.sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2]) <= EPS:
-            OS = np.array([0.0, 0.0, 1.0])
+            if probe_radius is not None:
+                OS = np.array([s[0], s[1], s[2] + probe_radius])
+            else:
+                OS = np.array([0.0, 0.0, 1.0])
         ns = math.sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2])
         zu = OS / ns
         e1 = np.array([1.0, 0.0, 0.0])

@@ -6,7 +6,8 @@ import math
 import numpy as np
 
 from ustb_amd.enums import Wavefront, Window
-from ustb_amd.uff import Apodization, LinearArray, LinearScan, MatrixArray, SectorScan, Wave
+from ustb_amd.uff import (Apodization, CurvilinearArray, LinearArray, LinearScan, MatrixArray, SectorScan, Wave,
+                          probe_radius)
 
 
 def matrix_problem(seed=0, P=700, N=12, W=5, F=3, L=96, zero_frac=0.3, edges=True):
@@ -99,6 +100,28 @@ def geom_cases():
     scan = LinearScan(np.linspace(-2e-3, 2e-3, 7), np.linspace(5e-3, 15e-3, 19), np.linspace(-1e-3, 1e-3, 5))
     cases["matrix3d_dw"] = dict(probe=prb, scan=scan, seq=seq, rx=Apodization(Window.tukey75, f_number=(1.5, 1.2)),
                                 tx=Apodization(Window.none), fs=12e6, fd=3e6, L=400, model=3, scan_kind="linear")
+    # curvilinear array (radius 20 mm), diverging waves from behind the arc,
+    # sector scan about the arc's centre: the curvilinear receive aperture
+    # (azimuth around (0, 0, -radius)) and the transmit incidence of finite
+    # sources
+    R = 20e-3
+    prb = CurvilinearArray(32, 0.3e-3, R)
+    ang = np.deg2rad(np.linspace(-20, 20, 4))
+    seq = [Wave.spherical(-(5e-3) * math.sin(a), 0.0, -5e-3 * math.cos(a), sound_speed=c,
+                          origin=(-(5e-3) * math.sin(a), 0.0, -5e-3 * math.cos(a))) for a in ang]
+    scan = SectorScan(np.deg2rad(np.linspace(-25, 25, 19)), np.linspace(R + 2e-3, R + 35e-3, 41), origin=(0.0, 0.0, -R))
+    cases["curv_dw"] = dict(probe=prb, scan=scan, seq=seq, rx=Apodization(Window.hanning, f_number=2.5),
+                            tx=Apodization(Window.tukey50, f_number=1.0), fs=12e6, fd=3e6, L=700, model=3,
+                            scan_kind="sector")
+    # curvilinear STA (origin at each source): the transmit axis from the
+    # arc's centre to the source; boxcar receive with a minimum aperture
+    prb = CurvilinearArray(12, 0.3e-3, 15e-3)
+    seq = [Wave.spherical(x, 0.0, z, sound_speed=c, origin=(x, 0.0, z)) for x, z in zip(prb.x, prb.z)]
+    scan = LinearScan(np.linspace(-3e-3, 3e-3, 17), np.linspace(1e-3, 20e-3, 29))
+    cases["curv_sta"] = dict(probe=prb, scan=scan, seq=seq,
+                             rx=Apodization(Window.boxcar, f_number=6.0, minimum_aperture=2e-3),
+                             tx=Apodization(Window.hanning, f_number=1.5), fs=20e6, fd=0.0, L=700, model=3,
+                             scan_kind="linear")
     return cases
 
 
@@ -110,13 +133,18 @@ def oracle_matrices(case, lens_delay=0.0, pw_margin=1e-3, blending_power=0.5):
     px, py, pz = scan.x, scan.y, scan.z
     waves = [wave_dict(w) for w in seq]
     rx = case["rx"]
+    radius = probe_radius(prb)
     origin = scan.origin if case["scan_kind"] == "sector" else None
+    if rx.origin is not None:
+        origin = rx.origin.xyz
+    elif radius is not None:
+        origin = (0.0, 0.0, -radius)
     ra = O.rx_apod(px, py, pz, prb.x, prb.y, prb.z, int(rx.window), rx.f_number, rx.minimum_aperture,
-                   rx.maximum_aperture, rx.tilt, origin)
+                   rx.maximum_aperture, rx.tilt, origin, curvilinear=radius is not None)
     rd = O.rx_delay(px, py, pz, prb.x, prb.y, prb.z, 1540.0, lens_delay)
     tx = case["tx"]
     ta = O.tx_apod(px, py, pz, waves, int(tx.window), tx.f_number, tx.minimum_aperture, tx.maximum_aperture,
-                   tx.tilt, tx.grating_lobe_angle)
+                   tx.tilt, tx.grating_lobe_angle, probe_radius=radius)
     td = O.tx_delay(px, py, pz, waves, case["scan_kind"], case["model"], pw_margin, blending_power,
                     scan_shape=scan.shape[:2], tx_f_number=tx.f_number)
     return ta, ra, td, rd

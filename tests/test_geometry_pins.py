@@ -10,7 +10,10 @@ tests leave open (CPU):
   (apodization.m:394-409) and the minimum / maximum aperture clamps
   (apodization.m:446-462);
 - the reference phase exp(-1j*2*w0*r/c) (das.m:306-309) in both reference
-  flavours.
+  flavours;
+- curvilinear probes: the element arc (curvilinear_array.m), the receive
+  aperture in azimuth about the arc's centre (apodization.m:388-398) and the
+  STA transmit axis from the arc's centre (apodization.m:506-518).
 
 Every expected value comes from a geometric statement that does not retrace
 the reference's operation sequence: angles measured from a rotated axis, the
@@ -524,3 +527,99 @@ def test_scanline_pins_reject_planted_mistakes(old, new):
 ])
 def test_scanline_pins_reject_planted_mistakes_in_the_layout(old, new):
     assert rejects(check_scanline, mutant(O.scanline_apod, old, new))
+
+
+# --- curvilinear probes (curvilinear_array.m, apodization.m:377-400, 506-518) ---
+
+def test_curvilinear_array_elements_lie_on_the_arc():
+    """Elements of uff.curvilinear_array: all at distance R from the arc's
+    centre (0, 0, -R), neighbours a chord of exactly `pitch` apart, symmetric
+    about x = 0 with the arc's top at the origin; the matrix form repeats the
+    arc along y, y fastest."""
+    from ustb_amd.uff import CurvilinearArray, CurvilinearMatrixArray
+
+    R, pitch = 20e-3, 0.3e-3
+    for N in (31, 32):
+        p = CurvilinearArray(N, pitch, R)
+        assert np.allclose(np.hypot(p.x, p.z + R), R, rtol=0, atol=1e-15)
+        chord = np.hypot(np.diff(p.x), np.diff(p.z))
+        assert np.allclose(chord, pitch, rtol=1e-12, atol=0)
+        assert abs(np.sum(p.x)) < 1e-15 and np.all(p.z <= 0.0)
+        assert np.max(p.z) > -pitch * pitch / R
+        assert np.allclose(p.x, -p.x[::-1], rtol=0, atol=1e-15)
+    m = CurvilinearMatrixArray(5, 3, pitch, 0.4e-3, R)
+    assert m.N_elements == 15
+    assert np.all(m.x[:3] == m.x[0]) and np.allclose(m.y[:3], [-0.4e-3, 0.0, 0.4e-3])
+    assert np.allclose(np.hypot(m.x, m.z + R), R, atol=1e-15)
+
+
+def _curv_ratio_check(fn):
+    """Curvilinear receive aperture: the lateral distance is the arc length
+    R * (angle between the pixel's and the element's radial lines seen from
+    the arc's centre), the axial one the pixel's distance from the centre plus
+    R (z_dist = pixel distance - origin.z with origin.z = -R). A pixel on an
+    element's radial line is at the aperture's centre (window(0)); with a
+    boxcar, pixels whose R * angle / (rho + R) * F# lies just below 1/2 are
+    in the aperture and just above are out; a pixel off the x-z plane keeps
+    the same distances (the pixel distance is measured in x and z)."""
+    R, F = 20e-3, 3.0
+    o = (0.0, 0.0, -R)
+    th = np.array([-0.2, 0.0, 0.15])                  # element azimuths about the centre
+    ex, ey, ez = R * np.sin(th), np.zeros(3), R * np.cos(th) - R
+    rho = 30e-3
+    # pixels on the elements' radial lines
+    px, pz = rho * np.sin(th), rho * np.cos(th) - R
+    a = fn(px, np.zeros(3), pz, ex, ey, ez, O.WIN_HANNING, (F, F), (0.0, 0.0), (np.inf, np.inf),
+           (0.0, 0.0), o, curvilinear=True)
+    assert np.all(np.diag(a) == 1.0), np.diag(a)
+    # boxcar edge: angle where F * R * phi / (rho + R) = 0.5 +- 1 %, from element 1 (theta = 0)
+    phi0 = 0.5 * (rho + R) / (F * R)
+    phis = np.array([0.99, 1.01]) * phi0
+    for y in (0.0, 10e-3):
+        px, pz = rho * np.sin(phis), rho * np.cos(phis) - R
+        w = fn(px, np.full(2, y), pz, ex[1:2], ey[1:2], ez[1:2], O.WIN_BOXCAR, (F, F), (0.0, 0.0),
+               (np.inf, np.inf), (0.0, 0.0), o, curvilinear=True)
+        assert w[0, 0] == 1.0 and w[0, 1] == 0.0, (y, w)
+
+
+def test_curvilinear_receive_aperture_pins():
+    _curv_ratio_check(O.rx_apod)
+    for old, new in (("zd = np.broadcast_to(pdist[:, None] - z0, (P, N)).copy()",
+                      "zd = np.broadcast_to(pdist[:, None] + z0, (P, N)).copy()"),
+                     ("xd = z0 * (px_az[:, None] - el_az[None, :])",
+                      "xd = pdist[:, None] * (px_az[:, None] - el_az[None, :])"),
+                     ("pdist = np.sqrt(dx * dx + dz * dz)",
+                      "pdist = np.sqrt(dx * dx + (py - y0) ** 2 + dz * dz)"),
+                     ("el_az = np.arctan2(ex - x0, ez - z0)", "el_az = np.arctan2(ex - x0, ez)")):
+        assert rejects(_curv_ratio_check, mutant(O.rx_apod, old, new)), old
+
+
+def _curv_sta_check(fn):
+    """Curvilinear STA (source on the wave's origin): the transmit axis runs
+    from the arc's centre to the source, i.e. along the element's normal, so a
+    pixel on that normal is at the centre of the transmit aperture (Hanning
+    window(0) = 1) and a pixel off it by an angle of F#-scaled half width is
+    outside; with the flat fallback axis [0, 0, 1] an off-centre element's
+    normal line would not be the aperture's centre."""
+    R, F = 15e-3, 1.5
+    th = 0.12
+    s = (R * math.sin(th), 0.0, R * math.cos(th) - R)
+    wv = focused(s[0], s[2], origin=s)
+    r = np.array([5e-3, 12e-3])
+    px, pz = s[0] + r * math.sin(th), s[2] + r * math.cos(th)
+    w = fn(px, np.zeros(2), pz, [wv], O.WIN_HANNING, (F, F), (0.0, 0.0), (np.inf, np.inf), probe_radius=R)
+    assert np.all(w == 1.0), w
+    # a pixel at 10 mm along the normal and 4 mm across it: |F * 4 / 10| > 1/2
+    nx, nz = math.sin(th), math.cos(th)
+    qx, qz = s[0] + 10e-3 * nx + 4e-3 * nz, s[2] + 10e-3 * nz - 4e-3 * nx
+    w = fn(np.array([qx]), np.zeros(1), np.array([qz]), [wv], O.WIN_HANNING, (F, F), (0.0, 0.0),
+           (np.inf, np.inf), probe_radius=R)
+    assert w[0, 0] == 0.0, w
+
+
+def test_curvilinear_sta_transmit_axis_pins():
+    _curv_sta_check(O.tx_apod)
+    for old, new in (("OS = np.array([s[0], s[1], s[2] + probe_radius])", "OS = np.array([0.0, 0.0, 1.0])"),
+                     ("OS = np.array([s[0], s[1], s[2] + probe_radius])",
+                      "OS = np.array([s[0], s[1], s[2] - probe_radius])")):
+        assert rejects(_curv_sta_check, mutant(O.tx_apod, old, new)), new

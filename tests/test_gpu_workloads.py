@@ -92,7 +92,7 @@ def test_c4_reduced_batch(cuda):
     plans = [make_plan(a.probe, a.scan, a.sequence, separable_phase=True, **a.plan_kwargs()) for a in wl.configs]
     for a, p in zip(wl.configs, plans):
         if a.name.startswith("C4c"):
-            p.set_variant(4)  # the receive state on the fly, as the full-size 3-D plan has it
+            p.set_variant(4)  # the receive state on the fly: the 3-D class's small-memory plan
             assert p.variant[0] == 4
     batch = Batch([plans[k] for k, _ in wl.acquisitions])
     chs = [torch.from_numpy(wl.datasets[k][d]).to(cuda) for k, d in wl.acquisitions]

@@ -37,7 +37,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_device_count():
-    assert L.lib().ustbx_abi_version() == 2
+    assert L.lib().ustbx_abi_version() == 3
     assert L.lib().ustbx_device_count() >= 0
 
 
@@ -295,6 +295,9 @@ def test_torch_op_fake_kernel_shapes_and_errors():
         torch.ops.ustbx.das(ch, 1.0, 0.0, ta, ra, ta, ra, 0.0, 4)
     with pytest.raises(RuntimeError, match="single precision"):
         torch.ops.ustbx.das(ch.to(torch.complex128), 1.0, 0.0, ta, ra, ta, ra, 0.0, 3)
+    for dim, shape in shapes.items():  # the Python reference's _beamform contract
+        out = torch.ops.ustbx.das_python(ch, 0.0, 1e-7, ta, ra, ta, ra, 0.0, dim)
+        assert tuple(out.shape) == shape and out.dtype == torch.complex64
     c = torch.zeros((1, 1, 2, 8), dtype=torch.complex64)
     a, b = torch.zeros((1, 5)), torch.zeros((2, 5))
     with pytest.raises(NotImplementedError):

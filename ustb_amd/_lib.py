@@ -32,6 +32,7 @@ FLAG_REF_PHASE_SINGLE = 16
 CF_MATLAB, CF_PYTHON = 0, 1
 FIR_CAUSAL, FIR_ZERO_PHASE = 0, 1
 SCAN_LINEAR, SCAN_SECTOR, SCAN_XYZ = 0, 1, 2
+PROBE_FLAT, PROBE_CURVILINEAR = 0, 1
 WAVE_PLANE, WAVE_SPHERICAL, WAVE_PHOTOACOUSTIC = 0, 1, 2
 TXMODEL_SPHERICAL, TXMODEL_UNIFIED, TXMODEL_HYBRID, TXMODEL_BLENDED = 1, 2, 3, 4
 
@@ -76,7 +77,7 @@ class Geometry(ctypes.Structure):
                 ("pw_margin", c_double), ("blending_power", c_double), ("tx_model", c_int32),
                 ("_pad", c_int32), ("tx_delay_matrix", P_float), ("fs", c_float), ("t0", c_float),
                 ("fd", c_float), ("dim", c_int32), ("interp", c_int32), ("edge", c_int32),
-                ("flags", c_uint32)]
+                ("flags", c_uint32), ("probe_kind", c_int32), ("_pad2", c_int32), ("probe_radius", c_double)]
 
 
 class SynthDesc(ctypes.Structure):

@@ -388,10 +388,6 @@ std::vector<int> band_major_order(const DevScan& sc, int tiles, int shares) {
   return order;
 }
 
-// Receive tables above this size are replaced by the on-the-fly receive
-// state by default (dimension both, where it applies).
-constexpr size_t kRxTableOtfBytes = (size_t)2 << 30;
-
 // The on-the-fly receive state covers dimension both with a windowed (or no)
 // receive apodization without tilt; variant 3 (coefficients) keeps the table.
 bool otf_applies(const ustbx_plan* pl) {
@@ -399,7 +395,7 @@ bool otf_applies(const ustbx_plan* pl) {
   if (G.dim != 3) return false;
   const DevRxApod& r = G.rx;
   if (r.kind == RXAP_ONES) return true;
-  if (r.kind != RXAP_WINDOW || r.tilt != 0) return false;
+  if (r.kind != RXAP_WINDOW || r.tilt != 0 || r.geo == RXGEO_CURV) return false;
   switch (r.window) {
     case WIN_BOXCAR: case WIN_HANNING: case WIN_HAMMING: case WIN_TUKEY25: case WIN_TUKEY50: case WIN_TUKEY75:
     case WIN_TRIANGLE:
@@ -427,10 +423,16 @@ bool rxtab_applies(const ustbx_plan* pl, bool otf) {
 }
 
 // Whether a new plan takes the on-the-fly receive state: where it applies
-// and the receive table would exceed kRxTableOtfBytes.
+// and the receive table would not fit (more than kRxTableMaxFraction of the
+// device memory). Where the table fits it is the faster form (the 3-D
+// class: 52 ms against 68 ms per frame), at far more memory (65.8 GiB
+// against 1.8 GiB); ustbx_plan_set_variant(plan, 4) chooses the small plan.
 bool otf_default(const ustbx_plan* pl) {
   const GeomLaunch& G = pl->G;
-  return otf_applies(pl) && (size_t)G.N * (size_t)G.P * sizeof(float4) > kRxTableOtfBytes;
+  if (!otf_applies(pl)) return false;
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+  return (size_t)G.N * (size_t)G.P * sizeof(float4) > (size_t)(kRxTableMaxFraction * (double)total_b);
 }
 
 // Variant 3 adds a coefficient buffer of 32 bytes per channel sample (cubic).
@@ -683,6 +685,10 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     return set_status(st, USTBX_E_ARG, "unknown interpolation");
   if (g->edge != USTBX_EDGE_DASC && g->edge != USTBX_EDGE_PYTHON)
     return set_status(st, USTBX_E_ARG, "unknown edge rule");
+  if (g->probe_kind != USTBX_PROBE_FLAT && g->probe_kind != USTBX_PROBE_CURVILINEAR)
+    return set_status(st, USTBX_E_ARG, "unknown probe kind");
+  if (g->probe_kind == USTBX_PROBE_CURVILINEAR && !(g->probe_radius > 0.0 && isfinite(g->probe_radius)))
+    return set_status(st, USTBX_E_ARG, "a curvilinear probe needs a positive radius");
   int rc = check_apod(g->rx_apod, "receive", st);
   if (rc) return rc;
   rc = check_apod(g->tx_apod, "transmit", st);
@@ -751,7 +757,13 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   } else {
     rx.kind = RXAP_WINDOW;
     rx.window = ra.window;
-    if (sc.kind == USTBX_SCAN_SECTOR || ra.has_origin) {
+    if (g->probe_kind == USTBX_PROBE_CURVILINEAR) {
+      // the origin defaults to the arc's centre (apodization.m:378-383)
+      rx.geo = RXGEO_CURV;
+      rx.ox = ra.has_origin ? ra.origin[0] : 0.0;
+      rx.oy = ra.has_origin ? ra.origin[1] : 0.0;
+      rx.oz = ra.has_origin ? ra.origin[2] : -g->probe_radius;
+    } else if (sc.kind == USTBX_SCAN_SECTOR || ra.has_origin) {
       rx.geo = RXGEO_ORIGIN;
       const double* o = ra.has_origin ? ra.origin : sc.origin;
       rx.ox = o[0]; rx.oy = o[1]; rx.oz = o[2];
@@ -852,8 +864,12 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
     if (!plane_like) {
       double OS[3] = {s.src[0] - s.origin[0], s.src[1] - s.origin[1], s.src[2] - s.origin[2]};
       const double nrm = sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2]);
-      if (nrm <= 2.220446049250313e-16) {  // STA (apodization.m:506-518), flat probes
-        OS[0] = 0.0; OS[1] = 0.0; OS[2] = 1.0;
+      if (nrm <= 2.220446049250313e-16) {  // STA (apodization.m:506-518)
+        if (g->probe_kind == USTBX_PROBE_CURVILINEAR) {  // from the arc's centre to the source
+          OS[0] = s.src[0]; OS[1] = s.src[1]; OS[2] = s.src[2] + g->probe_radius;
+        } else {
+          OS[0] = 0.0; OS[1] = 0.0; OS[2] = 1.0;
+        }
       }
       const double ns = sqrt((OS[0] * OS[0] + OS[1] * OS[1]) + OS[2] * OS[2]);
       const double zu[3] = {OS[0] / ns, OS[1] / ns, OS[2] / ns};
@@ -966,7 +982,7 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
 
   // receive aperture culling: one line of elements sorted by x, a window that
   // is zero above ratio 1/2, no tilt (see GeomLaunch::rx_cull)
-  bool cull = rx.kind == RXAP_WINDOW && rx.tilt == 0 && N >= 2 &&
+  bool cull = rx.kind == RXAP_WINDOW && rx.tilt == 0 && rx.geo != RXGEO_CURV && N >= 2 &&
               (rx.window == WIN_BOXCAR || rx.window == WIN_HANNING || rx.window == WIN_HAMMING ||
                rx.window == WIN_TUKEY25 || rx.window == WIN_TUKEY50 || rx.window == WIN_TUKEY75 ||
                rx.window == WIN_TRIANGLE) &&

@@ -338,12 +338,13 @@ __device__ __forceinline__ void pixel_xyz(const DevScan& s, int p, double& x,
 
 // Receive apodization parameters (one uff.apodization with probe set).
 enum { RXAP_ONES = 0, RXAP_WINDOW = 1, RXAP_VECTOR = 2, RXAP_MATRIX = 3 };
-enum { RXGEO_FLAT = 0, RXGEO_ORIGIN = 1 };
+enum { RXGEO_FLAT = 0, RXGEO_ORIGIN = 1, RXGEO_CURV = 2 };
 
 struct DevRxApod {
   int kind;      // RXAP_*
   int window;    // WIN_*
-  int geo;       // RXGEO_FLAT: x_dist = focus - element; RXGEO_ORIGIN: centred on origin
+  int geo;       // RXGEO_FLAT: x_dist = focus - element; RXGEO_ORIGIN: centred on origin;
+                 // RXGEO_CURV: curvilinear probe, azimuth around the origin
   int tilt;      // rotate_points needed
   double f0, f1;
   double min0, min1;  // minimum_aperture .* f_number
@@ -414,6 +415,17 @@ __device__ __forceinline__ float rx_apod_f32(const DevRxApod& ap, double px,
     xd = px - ex;
     yd = py - ey;
     zd = pz - ez;
+  } else if (ap.geo == RXGEO_CURV) {
+    // curvilinear probes (apodization.m:388-398): the lateral distance is the
+    // azimuth between pixel and element seen from the origin times origin.z,
+    // the axial one the pixel's distance from the origin in (x, z) minus
+    // origin.z
+    const double el_az = atan2(ex - ap.ox, ez - ap.oz);
+    const double px_az = atan2(px - ap.ox, pz - ap.oz);
+    const double dx = px - ap.ox, dz = pz - ap.oz;
+    xd = ap.oz * (px_az - el_az);
+    yd = ap.oy - ey;
+    zd = sqrt(dx * dx + dz * dz) - ap.oz;
   } else {
     const double dx = px - ap.ox, dy = py - ap.oy, dz = pz - ap.oz;
     xd = ex - ap.ox;

@@ -19,9 +19,8 @@ from collections import OrderedDict
 
 import numpy as np
 
-from . import _lib as L
 from .enums import Code, Dimension, Interpolation, SphericalTransmitDelayModel, Window
-from .uff import Apodization, BeamformedData, ChannelData, Scan
+from .uff import Apodization, BeamformedData, ChannelData, Scan, probe_radius
 
 
 class DAS:
@@ -81,7 +80,8 @@ class DAS:
         h.update(repr((n_samples, n_frames, cd.sampling_frequency, cd.initial_time, cd.modulation_frequency,
                        cd.sound_speed, int(self.dimension), int(self.interpolation), self.lens_delay,
                        self.pw_margin, self.blending_power, int(self.spherical_transmit_delay_model),
-                       self.phase, self.reference_phase, type(sc).__name__, int(self.gpu_id))).encode())
+                       self.phase, self.reference_phase, type(sc).__name__, int(self.gpu_id),
+                       probe_radius(cd.probe))).encode())
         return h.hexdigest()
 
     def _needs_hilbert(self) -> bool:
@@ -164,7 +164,7 @@ class DAS:
         0 <= floor(d) < L-1. Returns (P, N', W', F) complex64 (NumPy)."""
         import torch
 
-        from .ops import das
+        from . import torch_ops  # noqa: F401  (registers torch.ops.ustbx.das_python)
 
         dev = torch.device("cuda", 0)
         t = np.asarray(ch_data_time)
@@ -175,6 +175,6 @@ class DAS:
             ch = ch[..., None]
         tch = torch.from_numpy(np.ascontiguousarray(ch.transpose(3, 2, 1, 0).astype(np.complex64))).to(dev)
         T = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float32).T)).to(dev)
-        out = das(tch, dt, t0, T(tx_apodization), T(rx_apodization), T(transmit_delay), T(receive_delay),
-                  float(np.float32(w0)), int(dim), edge=L.EDGE_PYTHON, python_arithmetic=True, dasc_order=True)
+        out = torch.ops.ustbx.das_python(tch, t0, dt, T(tx_apodization), T(rx_apodization), T(transmit_delay),
+                                         T(receive_delay), float(np.float32(w0)), int(dim))
         return out.permute(3, 2, 1, 0).cpu().numpy()

@@ -9,7 +9,7 @@ import numpy as np
 
 from . import _lib as L
 from .enums import Dimension, Interpolation, SphericalTransmitDelayModel, Wavefront, Window
-from .uff import Apodization, LinearScan, Probe, Scan, SectorScan, Wave
+from .uff import Apodization, LinearScan, Probe, Scan, SectorScan, Wave, probe_radius
 
 
 def _dptr(a: np.ndarray):
@@ -131,6 +131,10 @@ def build_geometry(probe: Probe, scan: Scan, sequence: list, *, n_samples: int, 
     g.dim = int(dimension)
     g.interp = int(interpolation)
     g.edge = int(edge)
+    radius = probe_radius(probe)
+    if radius is not None:
+        g.probe_kind = L.PROBE_CURVILINEAR
+        g.probe_radius = radius
     g.flags = ((L.FLAG_SEPARABLE_PHASE if separable_phase else 0)
                | (L.FLAG_REF_PHASE if reference_phase else 0)
                | (L.FLAG_REF_PHASE_SINGLE if reference_phase == "matlab" else 0)

@@ -23,6 +23,16 @@ none / transmit and W' = W for none / receive (das_c.cpp:257-298).
 Errors are raised as das_c raises them (its checks and messages, with its
 "Toolbox:SRP_SRC:*" identifier in front where das_c gives one). There is only a CUDA (HIP)
 kernel: CPU tensors raise, there is no fallback.
+
+    torch.ops.ustbx.das_python(ch, t0, dt, tx_apo, rx_apo, tx_delay, rx_delay,
+                               w0, dim) -> Tensor
+
+is the Python reference's ``DAS._beamform`` (python das.py:203-261) on the
+same layouts: its NumPy arithmetic term by term (FP32 ``(tau - t0) / dt``,
+the exact fraction, FP64 interpolation rounded to complex64, ``apo * v``,
+``exp(1j * fl32(w0 * tau))``), its loop order and its edge rule
+``0 <= floor(d) < L - 1``; t0 and dt are the first time sample and the
+sample period, w0 = 2*pi*fd [rad/s].
 """
 from __future__ import annotations
 
@@ -81,4 +91,21 @@ def das(ch: torch.Tensor, fs: float, t0: float, tx_apo: torch.Tensor, rx_apo: to
 @das.register_fake
 def _das_fake(ch, fs, t0, tx_apo, rx_apo, tx_delay, rx_delay, fd, dim, interp=0):
     P, N, W, F = _check(ch, fs, t0, tx_apo, rx_apo, tx_delay, rx_delay, fd, dim, interp)
+    return ch.new_empty(ops.out_shape(dim, P, N, W, F))
+
+
+@torch.library.custom_op("ustbx::das_python", mutates_args=(), device_types="cuda")
+def das_python(ch: torch.Tensor, t0: float, dt: float, tx_apo: torch.Tensor, rx_apo: torch.Tensor,
+               tx_delay: torch.Tensor, rx_delay: torch.Tensor, w0: float, dim: int) -> torch.Tensor:
+    """DAS._beamform (python das.py:203-261) on the device; see the module
+    docstring."""
+    _check(ch, 1.0 / dt, t0, tx_apo, rx_apo, tx_delay, rx_delay, w0, dim, L.INTERP_LINEAR)
+    return ops.das(ch.contiguous(), float(dt), float(t0), tx_apo.contiguous(), rx_apo.contiguous(),
+                   tx_delay.contiguous(), rx_delay.contiguous(), float(w0), int(dim), edge=L.EDGE_PYTHON,
+                   python_arithmetic=True, dasc_order=True)
+
+
+@das_python.register_fake
+def _das_python_fake(ch, t0, dt, tx_apo, rx_apo, tx_delay, rx_delay, w0, dim):
+    P, N, W, F = _check(ch, 1.0 / dt, t0, tx_apo, rx_apo, tx_delay, rx_delay, w0, dim, L.INTERP_LINEAR)
     return ch.new_empty(ops.out_shape(dim, P, N, W, F))

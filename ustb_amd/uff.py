@@ -127,6 +127,47 @@ class MatrixArray(Probe):
         self.N_x, self.N_y, self.pitch_x, self.pitch_y = N_x, N_y, pitch_x, pitch_y
 
 
+class CurvilinearArray(Probe):
+    """Elements on an arc of radius R, the arc's top at the origin
+    (curvilinear_array.m:update): dtheta = 2*asin(pitch/2/R),
+    theta = (0:N-1)*dtheta - mean, x = R*sin(theta), z = R*cos(theta) - R.
+    Its apodization geometry is curvilinear (apodization.m:377-400, 506-518)."""
+
+    def __init__(self, N: int, pitch: float, radius: float):
+        dtheta = 2.0 * math.asin(pitch / 2.0 / radius)
+        theta = np.arange(N, dtype=np.float64) * dtheta
+        theta = theta - np.sum(theta) / N
+        super().__init__(radius * np.sin(theta), None, radius * np.cos(theta) - radius)
+        self.N, self.pitch, self.radius, self.theta = N, pitch, radius, theta
+
+
+class CurvilinearMatrixArray(Probe):
+    """A curvilinear array in azimuth (radius_x) repeated along y
+    (curvilinear_matrix_array.m:update): x0 = R*sin(theta), y0 = (0:N_y-1)*
+    pitch_y - mean, [X, Y] = meshgrid(x0, y0), Z = R*cos(theta) - R, elements
+    in X(:) order (y fastest)."""
+
+    def __init__(self, N_x: int, N_y: int, pitch_x: float, pitch_y: float, radius_x: float):
+        dtheta = 2.0 * math.asin(pitch_x / 2.0 / radius_x)
+        theta = np.arange(N_x, dtype=np.float64) * dtheta
+        theta = theta - np.sum(theta) / N_x
+        x0 = radius_x * np.sin(theta)
+        y0 = np.arange(N_y, dtype=np.float64) * pitch_y
+        y0 = y0 - np.sum(y0) / N_y
+        X, Y = np.meshgrid(x0, y0)                                   # (N_y, N_x)
+        Z = np.ones((N_y, 1)) * (radius_x * np.cos(theta))[None, :] - radius_x
+        super().__init__(X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F"))
+        self.N_x, self.N_y, self.pitch_x, self.pitch_y, self.radius_x = N_x, N_y, pitch_x, pitch_y, radius_x
+        self.radius = radius_x
+
+
+def probe_radius(probe: Probe) -> float | None:
+    """The radius of a curvilinear probe (its apodization geometry), else None."""
+    if isinstance(probe, (CurvilinearArray, CurvilinearMatrixArray)):
+        return float(probe.radius)
+    return None
+
+
 class Scan:
     kind = "xyz"
 
