@@ -114,6 +114,43 @@ def test_host_entry_matches_device(cuda):
     assert np.array_equal(out, ref)
 
 
+def test_host_entry_chunked_staging_matches_device(cuda):
+    """The host entry stages inputs and outputs through two 8 MiB pinned
+    buffers with a threaded CPU copy and keeps its device buffer across calls:
+    inputs and outputs of several chunks (a 16.8 MB receive delay matrix, a
+    33.6 MB dimension-none output), then a smaller call and a larger one
+    (the buffer is reused, then grown), each equal to the device entry's
+    image bit for bit."""
+    import ctypes
+
+    from ustb_amd import ops
+
+    def problem(seed, P, N, W, F, Ls):
+        rng = np.random.default_rng(seed)
+        ch = (rng.standard_normal((F, W, N, Ls)) + 1j * rng.standard_normal((F, W, N, Ls))).astype(np.complex64)
+        ta = rng.uniform(0, 1, (W, P)).astype(np.float32)
+        ra = rng.uniform(0, 1, (N, P)).astype(np.float32)
+        td = rng.uniform(0, Ls * 0.4, (W, P)).astype(np.float32)
+        rd = rng.uniform(0, Ls * 0.4, (N, P)).astype(np.float32)
+        return ch, ta, ra, td, rd
+
+    for seed, (P, N, W, F, Ls, dim) in enumerate(((65536, 64, 2, 1, 256, 0), (4096, 16, 3, 2, 128, 3),
+                                                    (131072, 48, 1, 1, 512, 1))):
+        print(f"case host entry chunks P={P} N={N} W={W} F={F} L={Ls} dim={dim}", flush=True)
+        ch, ta, ra, td, rd = problem(seed, P, N, W, F, Ls)
+        shape = ops.out_shape(dim, P, N, W, F)
+        out = np.zeros(shape, np.complex64)
+        d = L.DasDesc(L=Ls, N=N, W=W, F=F, P=P, fs=1.0, t0=0.0, fd=0.0, dim=dim, interp=0, edge=0, flags=0,
+                      split=0)
+        st = L.Status()
+        rc = L.lib().ustbx_das_run_host(ctypes.byref(d), ch.ctypes.data, ta.ctypes.data, ra.ctypes.data,
+                                        td.ctypes.data, rd.ctypes.data, out.ctypes.data, 0, ctypes.byref(st))
+        L.check(rc, st)
+        T = lambda a: torch.from_numpy(a).to(cuda)
+        ref = ops.das(T(ch), 1.0, 0.0, T(ta), T(ra), T(td), T(rd), 0.0, dim).cpu().numpy()
+        assert np.array_equal(out, ref)
+
+
 @pytest.mark.parametrize("dim", [0, 3])
 def test_nonfinite_and_huge_delays_and_nan_weights(cuda, dim):
     """das_c converts d = (tau - t0) * fs with (int): NaN, +-Inf and |d| >= 2^31

@@ -7,14 +7,17 @@
 #include <string.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/ustbx_das.h"
 #include "das_launch.h"
 #include "das_plan.h"
 #include "das_status.h"
+#include "host_copy.h"
 
 using namespace ustbx;
 
@@ -89,6 +92,122 @@ void rxtab_default_tiles(int max_nw, int& wc, int& ws) {
   const int need = (max_nw + 7) / 8;
   wc = need <= 4 ? 4 : need <= 6 ? 6 : need <= 8 ? 8 : need <= 10 ? 10 : need <= 12 ? 12 : 16;
 }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host-buffer entry (what a MEX gateway holds): state kept per device across
+// calls, so that a call does no device allocation and no device-wide
+// synchronisation: a device buffer grown to the largest call so far, two
+// pinned staging buffers, a stream of its own (default priority) and an
+// event per staging buffer. Inputs go host memory -> staging (the CPU copy,
+// split over a few threads) -> device (DMA on the stream), chunk by chunk
+// with the two staging buffers alternating, so that the CPU copy of one
+// chunk overlaps the DMA of the previous one; the output comes back the same
+// way. Calls on one device are serialised.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kHostDevices = 64;
+constexpr size_t kStageChunk = (size_t)8 << 20;  // bytes per staging buffer
+
+CopyPool& copy_pool() {
+  static CopyPool pool(4);  // CPU copy threads, the caller's included
+  return pool;
+}
+
+struct HostCopy {
+  void* dst;        // device
+  const void* src;  // host
+  size_t bytes;
+};
+
+struct HostEntry {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  char* dev = nullptr;
+  size_t dev_bytes = 0;
+  char* stage[2] = {nullptr, nullptr};
+  hipEvent_t free_ev[2] = {nullptr, nullptr};  // the staging buffer's last DMA has finished
+  bool used[2] = {false, false};
+
+  hipError_t reserve(size_t bytes) {
+    hipError_t e = hipSuccess;
+    if (!stream && (e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) return e;
+    for (int k = 0; k < 2; ++k) {
+      if (!stage[k] && (e = hipHostMalloc((void**)&stage[k], kStageChunk, hipHostMallocDefault)) != hipSuccess)
+        return e;
+      if (!free_ev[k] && (e = hipEventCreateWithFlags(&free_ev[k], hipEventDisableTiming)) != hipSuccess) return e;
+    }
+    if (bytes > dev_bytes) {
+      if (dev) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(dev);
+        dev = nullptr;
+        dev_bytes = 0;
+      }
+      if ((e = hipMalloc(&dev, bytes)) != hipSuccess) return e;
+      dev_bytes = bytes;
+    }
+    return hipSuccess;
+  }
+
+  // waits until staging buffer k is free again
+  hipError_t acquire(int k) {
+    if (!used[k]) return hipSuccess;
+    used[k] = false;
+    return hipEventSynchronize(free_ev[k]);
+  }
+
+  hipError_t upload(const HostCopy* c, int n) {
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+      for (size_t off = 0; off < c[i].bytes; off += kStageChunk) {
+        const size_t len = std::min(kStageChunk, c[i].bytes - off);
+        hipError_t e = acquire(k);
+        if (e != hipSuccess) return e;
+        copy_pool().copy(stage[k], (const char*)c[i].src + off, len);
+        e = hipMemcpyAsync((char*)c[i].dst + off, stage[k], len, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipEventRecord(free_ev[k], stream);
+        if (e != hipSuccess) return e;
+        used[k] = true;
+        k ^= 1;
+      }
+    }
+    return hipSuccess;
+  }
+
+  // device -> staging (DMA) of chunk c + 1 overlaps staging -> host (CPU) of chunk c
+  hipError_t download(void* dst, const void* src, size_t bytes) {
+    const size_t chunks = (bytes + kStageChunk - 1) / kStageChunk;
+    hipError_t e = hipSuccess;
+    for (size_t c = 0; c < chunks && c < 2; ++c) {
+      const size_t off = c * kStageChunk, len = std::min(kStageChunk, bytes - off);
+      if ((e = acquire((int)c)) != hipSuccess) return e;
+      e = hipMemcpyAsync(stage[c], (const char*)src + off, len, hipMemcpyDeviceToHost, stream);
+      if (e == hipSuccess) e = hipEventRecord(free_ev[c], stream);
+      if (e != hipSuccess) return e;
+      used[c] = true;
+    }
+    for (size_t c = 0; c < chunks; ++c) {
+      const int k = (int)(c & 1);
+      const size_t off = c * kStageChunk, len = std::min(kStageChunk, bytes - off);
+      if ((e = acquire(k)) != hipSuccess) return e;
+      copy_pool().copy((char*)dst + off, stage[k], len);
+      const size_t n2 = c + 2;
+      if (n2 < chunks) {
+        const size_t off2 = n2 * kStageChunk, len2 = std::min(kStageChunk, bytes - off2);
+        e = hipMemcpyAsync(stage[k], (const char*)src + off2, len2, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipEventRecord(free_ev[k], stream);
+        if (e != hipSuccess) return e;
+        used[k] = true;
+      }
+    }
+    return hipStreamSynchronize(stream);
+  }
+};
+
+HostEntry g_host[kHostDevices];
 
 }  // namespace
 
@@ -312,7 +431,7 @@ int ustbx_das_run_host(const ustbx_das_desc* d, const void* ch, const float* tx_
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev == 0) return set_status(st, USTBX_E_GPU, "no HIP device available");
-  if (device < 0 || device >= ndev)
+  if (device < 0 || device >= ndev || device >= kHostDevices)
     return set_status(st, USTBX_E_GPU, "invalid gpu id %d (%d devices)", device, ndev);
   // the caller's current device is restored on return
   int prev = 0;
@@ -324,33 +443,30 @@ int ustbx_das_run_host(const ustbx_das_desc* d, const void* ch, const float* tx_
     int dev;
     ~Restore() { (void)hipSetDevice(dev); }
   } restore{prev};
-  char* buf = nullptr;
+  HostEntry& H = g_host[device];
+  std::lock_guard<std::mutex> lock(H.mu);
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   const size_t total = al(bytes_ch) + 2 * al(bytes_tx) + 2 * al(bytes_rx) + al(bytes_out);
-  e = hipMalloc(&buf, total);
-  if (e != hipSuccess) return hip_fail(st, e, "hipMalloc");
-  char* p = buf;
+  e = H.reserve(total);
+  if (e != hipSuccess) return hip_fail(st, e, "host entry buffers");
+  char* p = H.dev;
   void* dch = p; p += al(bytes_ch);
   float* dta = (float*)p; p += al(bytes_tx);
   float* dtd = (float*)p; p += al(bytes_tx);
   float* dra = (float*)p; p += al(bytes_rx);
   float* drd = (float*)p; p += al(bytes_rx);
   void* dout = p;
-  int result = USTBX_OK;
-  do {
-    if (bytes_ch && (e = hipMemcpy(dch, ch, bytes_ch, hipMemcpyHostToDevice)) != hipSuccess) break;
-    if (bytes_tx && (e = hipMemcpy(dta, tx_apo, bytes_tx, hipMemcpyHostToDevice)) != hipSuccess) break;
-    if (bytes_tx && (e = hipMemcpy(dtd, tx_delay, bytes_tx, hipMemcpyHostToDevice)) != hipSuccess) break;
-    if (bytes_rx && (e = hipMemcpy(dra, rx_apo, bytes_rx, hipMemcpyHostToDevice)) != hipSuccess) break;
-    if (bytes_rx && (e = hipMemcpy(drd, rx_delay, bytes_rx, hipMemcpyHostToDevice)) != hipSuccess) break;
-    result = ustbx_das_run(d, dch, dta, dra, dtd, drd, dout, nullptr, st);
-    if (result) break;
-    if ((e = hipDeviceSynchronize()) != hipSuccess) break;
-    if ((e = hipMemcpy(out, dout, bytes_out, hipMemcpyDeviceToHost)) != hipSuccess) break;
-  } while (0);
-  (void)hipFree(buf);
-  if (result) return result;
-  if (e != hipSuccess) return hip_fail(st, e, "ustbx_das_run_host");
+  const HostCopy in[5] = {{dch, ch, bytes_ch}, {dta, tx_apo, bytes_tx}, {dtd, tx_delay, bytes_tx},
+                          {dra, rx_apo, bytes_rx}, {drd, rx_delay, bytes_rx}};
+  e = H.upload(in, 5);
+  if (e != hipSuccess) return hip_fail(st, e, "host-to-device copies");
+  const int result = ustbx_das_run(d, dch, dta, dra, dtd, drd, dout, H.stream, st);
+  if (result) {
+    (void)hipStreamSynchronize(H.stream);
+    return result;
+  }
+  e = H.download(out, dout, bytes_out);
+  if (e != hipSuccess) return hip_fail(st, e, "device-to-host copies");
   return ok(st);
 }
 

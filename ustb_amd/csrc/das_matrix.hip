@@ -32,6 +32,10 @@ struct MatArgs {
   float fs, t0, wd;
   int edge;
   float dt, w0;  // PY: sample period, angular frequency
+  // PY: per wave / channel, whether its weight column holds a nonzero (NaN
+  // included): the reference skips only all-zero columns (das.py:221-225)
+  const int* __restrict__ tx_any;
+  const int* __restrict__ rx_any;
 };
 
 // The Python reference's term (python das.py:223-250) in NumPy's dtypes:
@@ -129,9 +133,9 @@ __device__ __forceinline__ void sum_channels(const MatArgs& A, int p, int w,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       if (PY) {
-        if (n + u < ne) {
+        if (n + u < ne && A.rx_any[n + u]) {
           const PyTerm t = py_term<IQ>(td, rd[u], ta, ra[u], A.t0, A.dt, A.w0, A.L);
-          if (t.apo != 0.0f) {  // NaN included
+          {  // every pixel of a column with a nonzero weight (0 * NaN reaches the sum)
             const float2* trace = A.ch + (size_t)w * LN + (size_t)(n + u) * A.L;
 #pragma unroll
             for (int f = 0; f < FC; ++f)
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
       if (active) {
         for (int w = 0; w < A.W; ++w) {
           const float ta = A.tx_apo[(size_t)w * PS + p];
-          if (PY || ta != 0.0f) {
+          if (PY ? A.tx_any[w] != 0 : ta != 0.0f) {
             const float td = A.tx_delay[(size_t)w * PS + p];
             sum_channels<INTERP, IQ, PY, FC, U>(A, p, w, ta, td, nb, ne, f0, acc);
           }
@@ -219,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
         for (int f = 0; f < FC; ++f) acc[f] = make_float2(0.0f, 0.0f);
         if (active) {
           const float ta = A.tx_apo[(size_t)w * PS + p];
-          if (PY || ta != 0.0f) {
+          if (PY ? A.tx_any[w] != 0 : ta != 0.0f) {
             const float td = A.tx_delay[(size_t)w * PS + p];
             sum_channels<INTERP, IQ, PY, FC, U>(A, p, w, ta, td, nb, ne, f0, acc);
           }
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
             for (int w = 0; w < A.W; ++w) {
               const PyTerm t = py_term<IQ>(A.tx_delay[(size_t)w * PS + p], rd, A.tx_apo[(size_t)w * PS + p], ra,
                                            A.t0, A.dt, A.w0, A.L);
-              if (t.apo != 0.0f) {
+              if (A.rx_any[n] && A.tx_any[w]) {
 #pragma unroll
                 for (int f = 0; f < FC; ++f)
                   if (f0 + f < A.F)
@@ -293,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void das_matrix_kernel(MatArgs A) {
             const float ra = A.rx_apo[(size_t)n * PS + p];
             if (PY) {
               const PyTerm t = py_term<IQ>(td, A.rx_delay[(size_t)n * PS + p], ta, ra, A.t0, A.dt, A.w0, A.L);
-              if (t.apo != 0.0f) {
+              if (A.rx_any[n] && A.tx_any[w]) {
                 const float2* trace = A.ch + (size_t)w * LN + (size_t)n * A.L;
 #pragma unroll
                 for (int f = 0; f < FC; ++f)
@@ -376,6 +380,29 @@ hipError_t launch_zero_fill(void* p, size_t n_complex, hipStream_t stream) {
   return hipGetLastError();
 }
 
+// PY: flags[c] = 1 where weight column c (W transmit columns, then N
+// receive ones, each P long) holds a nonzero, NaN included (np.any). One
+// block per column.
+__global__ __launch_bounds__(kBlock) void py_columns_kernel(const float* __restrict__ tx_apo,
+                                                            const float* __restrict__ rx_apo, int W, int P,
+                                                            int* __restrict__ flags) {
+  const int c = blockIdx.x;
+  const float* col = c < W ? tx_apo + (size_t)c * P : rx_apo + (size_t)(c - W) * P;
+  bool any = false;
+  for (int p = threadIdx.x; p < P && !any; p += kBlock) any = col[p] != 0.0f;
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) flags[c] = any ? 1 : 0;
+}
+
+static hipError_t dispatch_dim(const MatLaunch& m, MatArgs& A, hipStream_t stream) {
+  switch (m.dim) {
+    case 0: return pick_interp<0>(A, m.interp, m.py, m.split, stream);
+    case 1: return pick_interp<1>(A, m.interp, m.py, m.split, stream);
+    case 2: return pick_interp<2>(A, m.interp, m.py, m.split, stream);
+    default: return pick_interp<3>(A, m.interp, m.py, m.split, stream);
+  }
+}
+
 hipError_t launch_das_matrix(const MatLaunch& m, hipStream_t stream) {
   MatArgs A;
   A.ch = reinterpret_cast<const float2*>(m.ch);
@@ -387,12 +414,22 @@ hipError_t launch_das_matrix(const MatLaunch& m, hipStream_t stream) {
   A.L = m.L; A.N = m.N; A.W = m.W; A.F = m.F; A.P = m.P;
   A.fs = m.fs; A.t0 = m.t0; A.wd = m.wd; A.edge = m.edge;
   A.dt = m.dt; A.w0 = m.w0;
-  switch (m.dim) {
-    case 0: return pick_interp<0>(A, m.interp, m.py, m.split, stream);
-    case 1: return pick_interp<1>(A, m.interp, m.py, m.split, stream);
-    case 2: return pick_interp<2>(A, m.interp, m.py, m.split, stream);
-    default: return pick_interp<3>(A, m.interp, m.py, m.split, stream);
+  A.tx_any = A.rx_any = nullptr;
+  if (!m.py) return dispatch_dim(m, A, stream);
+  // the column flags live in stream-ordered memory for this launch only
+  int* flags = nullptr;
+  hipError_t e = hipMallocAsync((void**)&flags, sizeof(int) * (size_t)(m.W + m.N), stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(py_columns_kernel, dim3((unsigned)(m.W + m.N)), dim3(kBlock), 0, stream, m.tx_apo, m.rx_apo,
+                     m.W, m.P, flags);
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    A.tx_any = flags;
+    A.rx_any = flags + m.W;
+    e = dispatch_dim(m, A, stream);
   }
+  const hipError_t e2 = hipFreeAsync(flags, stream);
+  return e != hipSuccess ? e : e2;
 }
 
 }  // namespace ustbx
