@@ -547,3 +547,32 @@ def test_scanline_wave_count_mismatch_is_refused(cuda):
                   [Wave.plane(0.0)] * 3, n_samples=200, n_frames=1, sampling_frequency=20e6, initial_time=0.0,
                   modulation_frequency=0.0, sound_speed=1540.0,
                   transmit_apodization=Apodization(Window.scanline, MLA=(2, 1)))
+
+
+def test_frame_of_more_than_2_gib(cuda):
+    """A frame of 2.5 GiB (5 waves x 64 channels x 2^20 samples): the plan
+    runs it on variant 0, whose gathers address each (frame, wave) through a
+    buffer resource of its own; RF, dimension both, against das_c on the
+    plan's matrices within 1e-5 (and the receive-table variants refuse it)."""
+    from ustb_amd.enums import Window
+    from ustb_amd.uff import Apodization, LinearArray, LinearScan, Wave
+
+    N, W, Ls, fs = 64, 5, 1 << 20, 20e6
+    probe = LinearArray(N, 0.3e-3)
+    scan = LinearScan(np.linspace(-3e-3, 3e-3, 12), np.linspace(5e-3, 25e-3, 20))
+    seq = [Wave.plane(a) for a in np.deg2rad(np.linspace(-8, 8, W))]
+    print(f"case big frame {W * N * Ls * 8 / 2**30:.2f} GiB", flush=True)
+    plan = make_plan(probe, scan, seq, n_samples=Ls, n_frames=1, sampling_frequency=fs, initial_time=0.0,
+                     modulation_frequency=0.0, sound_speed=1540.0,
+                     receive_apodization=Apodization(Window.hanning, f_number=1.5), dimension=3,
+                     separable_phase=True)
+    assert plan.variant[0] == 0 and not plan.has_variant(2)
+    g = torch.Generator(device=cuda).manual_seed(7)
+    ch = torch.complex(torch.randn((1, W, N, Ls), device=cuda, generator=g),
+                       torch.randn((1, W, N, Ls), device=cuda, generator=g))
+    got = plan.run(ch).cpu().numpy()
+    ta, ra, td, rd = [t.cpu().numpy() for t in plan.materialize()]
+    ref = O.das_c(ch.cpu().numpy(), fs, 0.0, ta, ra, td, rd, 0.0, 3)
+    del ch
+    assert _rel(got, ref) <= TOL
+    plan.close()

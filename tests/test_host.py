@@ -195,9 +195,9 @@ def test_python_arithmetic_flag_argument_checks():
 
 def test_plan_create_rejects_oversized_and_empty_geometry():
     """ustbx_plan_create checks the geometry on the host before any device
-    work: a frame of channel data of 2 GiB or more (the gathers address a
-    frame with 32-bit byte offsets) is E_CAPACITY; an empty scan, F < 1 or
-    L < 2 are E_DIMENSIONS."""
+    work: one wave of channel data (N x L samples) of 2 GiB or more (the
+    gathers address a wave with 32-bit byte offsets; larger frames are
+    fine) is E_CAPACITY; an empty scan, F < 1 or L < 2 are E_DIMENSIONS."""
     import ctypes
 
     from ustb_amd.geometry import build_geometry
@@ -215,8 +215,8 @@ def test_plan_create_rejects_oversized_and_empty_geometry():
         assert not h.value  # nothing created
         return rc, st.message.decode()
 
-    # 128 channels x 2 waves x 2^20 samples x 8 B = 2 GiB
-    rc, msg = create(seq=[Wave.plane(0.0), Wave.plane(0.1)], n_samples=1 << 20)
+    # 128 channels x 2^21 samples x 8 B = 2 GiB in one wave
+    rc, msg = create(n_samples=1 << 21)
     assert rc == L.E_CAPACITY and "2 GiB" in msg
     for kw in (dict(n_samples=1), dict(n_frames=0)):
         rc, _ = create(**kw)

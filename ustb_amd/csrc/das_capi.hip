@@ -41,7 +41,9 @@ int matrix_auto_split(int P) {
 
 // The shared-receive-state kernel covers dimension both with the separable
 // phase on one frame.
-bool shared_rx_applies(const GeomLaunch& G) { return G.dim == 3 && G.separable && G.F == 1 && G.wd != 0.0f; }
+bool shared_rx_applies(const GeomLaunch& G) {
+  return G.dim == 3 && G.separable && G.F == 1 && G.wd != 0.0f && (int64_t)G.L * G.N * G.W * 8 <= 0x7FFFFE00;
+}
 
 // Default tiles of the shared-receive kernel: 8 wave-splits, the smallest
 // instantiated waves-per-split covering W in one round (else 8 per round).
@@ -817,8 +819,10 @@ int ustbx_plan_create(const ustbx_geometry* g, ustbx_plan** plan, ustbx_status* 
   if (!sc.a0 || !sc.a1 || !sc.a2) return set_status(st, USTBX_E_ARG, "NULL scan axis");
   // gathers use 32-bit byte offsets within a frame, and an offset just below
   // 2^31 as the always-out-of-range position of skipped terms
-  if ((int64_t)g->L * g->N * g->W * 8 > 0x7FFFFE00)
-    return set_status(st, USTBX_E_CAPACITY, "one frame of channel data must be below 2 GiB");
+  // the gathers address one wave's channel data with 32-bit byte offsets;
+  // frames of 2 GiB or more run on variant 0 (per-(frame, wave) resources)
+  if ((int64_t)g->L * g->N * 8 > 0x7FFFFE00)
+    return set_status(st, USTBX_E_CAPACITY, "one wave of channel data (N x L samples) must be below 2 GiB");
   const int P = (int)P64, N = g->N, W = g->W;
 
   Arena ar;

@@ -186,20 +186,16 @@ void das_geom_kernel(GeomLaunch G) {
   float2 refp = make_float2(1.0f, 0.0f);
   if (G.ref_phase && active) refp = ref_phase_factor(G.w0_ref, G.wd, G.c32, G.ref_single, rref);
 
-  // byte strides of the channel data within one frame (< 2 GiB, checked when
-  // the plan is created)
+  // byte strides of the channel data within one wave (< 2 GiB, checked when
+  // the plan is created); a frame may be larger: the gathers address each
+  // (frame, wave) through a buffer resource of its own
   const int l8 = G.L * 8;
   const int ln8 = G.L * G.N * 8;
-  const int frame_bytes = (int)(LNW * 8);
 
   for (int f0 = 0; f0 < G.F; f0 += FC) {
     float2 acc[FC];
 #pragma unroll
     for (int f = 0; f < FC; ++f) acc[f] = make_float2(0.0f, 0.0f);
-    __amdgpu_buffer_rsrc_t rs[FC];
-#pragma unroll
-    for (int f = 0; f < FC; ++f)
-      rs[f] = make_rsrc(ch + (size_t)min(f0 + f, G.F - 1) * LNW, frame_bytes);
 
     for (int w0 = 0; w0 < G.W; w0 += WC) {
       // ---- transmit chunk from the resident table ----
@@ -288,18 +284,19 @@ void das_geom_kernel(GeomLaunch G) {
                 }
                 if (valid) {
                   hit = true;
-                  // byte offset of sample n0 of trace (n, w) within a frame
-                  const int off = noff + w * ln8 + (n0 << 3);
+                  // byte offset of sample n0 of trace n within wave w's data
+                  const int off = noff + (n0 << 3);
 #pragma unroll
                   for (int f = 0; f < FC; ++f) {
                     if (f0 + f < G.F) {
+                      const __amdgpu_buffer_rsrc_t rsw = make_rsrc(ch + (size_t)(f0 + f) * LNW + (size_t)w * LN, ln8);
                       float2 sm;
                       if (INTERP == INTERP_LINEAR) {
-                        const float4 x = buf_load16(rs[f], off);
+                        const float4 x = buf_load16(rsw, off);
                         sm = make_float2(fmaf(b, x.z - x.x, x.x), fmaf(b, x.w - x.y, x.y));
                       } else {
-                        const float4 lo = buf_load16(rs[f], off - 8);
-                        const float4 hi = buf_load16(rs[f], off + 8);
+                        const float4 lo = buf_load16(rsw, off - 8);
+                        const float4 hi = buf_load16(rsw, off + 8);
                         float c0, c1, c2, c3;
                         keys_weights(b, c0, c1, c2, c3);
                         sm = make_float2(fmaf(c3, hi.z, fmaf(c2, hi.x, fmaf(c1, lo.z, c0 * lo.x))),
