@@ -129,14 +129,20 @@ __device__ __forceinline__ float2 buf_load8(__amdgpu_buffer_rsrc_t r, int byte_o
 // waves, in batches of RB terms whose gathers are all issued before the
 // first is used. Slots j >= nj are skipped when GUARD (a wave-uniform branch
 // per batch); without it the slots past the tile's waves hold the far delay
-// and read zeros.
-constexpr int RB = 5;
+// and read zeros. RB: 6 terms (12 gathers) in flight for Keys' cubic, 5 for
+// the one-gather modes (the best of 3-6 measured on C2, cubic, and on the
+// 3-D and STA classes, linear).
+template <int SM>
+struct GatherBatch {
+  static constexpr int RB = SM == SM_CUBIC ? 6 : 5;
+};
 
 template <int SM, int WC, bool GUARD, bool SAFE>
 __device__ __forceinline__ float2 rxtab_channel(__amdgpu_buffer_rsrc_t rs, const float (&td)[WC],
                                                 const float2 (&T)[WC], const int (&wb)[WC], int nj, float rd,
                                                 int nb, float t0, float fs, float lim, bool& any) {
   using M = SampleMode<SM>;
+  constexpr int RB = GatherBatch<SM>::RB;
   constexpr float lo_ok = 1.0f;  // das_c's window: 0 < trunc(d)
   float2 part = make_float2(0.0f, 0.0f);
 #pragma unroll
@@ -231,6 +237,7 @@ __device__ __forceinline__ float2 rxtab_channel_staged(__amdgpu_buffer_rsrc_t rs
                                                        float2* __restrict__ win, unsigned lane8) {
   static_assert(SM != SM_COEF, "coefficient data is not staged");
   using S = StageWin<SM>;
+  constexpr int RB = GatherBatch<SM>::RB;
   const unsigned wbase = (unsigned)(size_t)(__attribute__((address_space(3))) char*)win;  // LDS address
   float2 part = make_float2(0.0f, 0.0f);
 #pragma unroll
@@ -354,7 +361,7 @@ enum { RXL_FULL = 0, RXL_COMPACT = 1, RXL_EITHER = 2 };
 // One tile of one frame; every thread of the block calls it (it ends with a
 // block barrier and the split-ordered reduction through red). STG: the
 // channels A.stage_mask marks run staged, split s through the LDS slots
-// win[s * RB * kStageWin ..] (SAFE tiles never).
+// win[s * GatherBatch<SM>::RB * kStageWin ..] (SAFE tiles never).
 template <int SM, int WC, int WS, bool SAFE = false, int RXL = RXL_EITHER, bool STG = false>
 __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, float2 (*red)[64],
                                            float2* win = nullptr) {
@@ -393,7 +400,7 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
   constexpr bool kStg = STG && !SAFE && SM != SM_COEF;
   const unsigned long long* smask =
       kStg && A.stage_mask ? A.stage_mask + (size_t)tile * (size_t)A.stage_words : nullptr;
-  float2* swin = kStg ? win + s * (RB * kStageWin) : nullptr;
+  float2* swin = kStg ? win + s * (GatherBatch<SM>::RB * kStageWin) : nullptr;
   // 8 * lane, opaque to the compiler so that window addresses stay one
   // scalar-plus-vector add
   unsigned lane8 = (unsigned)lane * 8u;
