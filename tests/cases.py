@@ -67,6 +67,13 @@ def geom_cases():
     cases["dw_sector"] = dict(probe=prb, scan=scan, seq=seq, rx=Apodization(Window.hamming, f_number=1.2),
                               tx=Apodization(Window.tukey50, f_number=1.0), fs=10e6, fd=2.5e6, L=600, model=3,
                               scan_kind="sector")
+    # the same on 48 depths x 12 lines: axes that the transmit kernel's
+    # 16-depth x 4-line patch tiles divide (the other cases keep 64-pixel runs)
+    cases["dw_sector_patch"] = dict(cases["dw_sector"], scan=SectorScan(np.deg2rad(np.linspace(-45, 45, 12)),
+                                                                        np.linspace(1e-3, 40e-3, 48)))
+    # RF plane wave on a linear scan of 32 depths x 8 lines (patch tiles)
+    cases["pw_rf_patch"] = dict(cases["pw_rf_hanning"], scan=LinearScan(np.linspace(-4e-3, 4e-3, 8),
+                                                                         np.linspace(5e-3, 25e-3, 32)))
     # STA: sources at elements, wave.delay = |src|/c (TE_ps_sta_rf.m:33-34), hybrid model
     prb = LinearArray(12, 0.3e-3)
     seq = [Wave.spherical(x, 0.0, 0.0, sound_speed=c, delay=abs(x) / c) for x in prb.x]

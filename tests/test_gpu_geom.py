@@ -348,7 +348,8 @@ def test_receive_table_rejected_modes(cuda):
     plan.close()
 
 
-@pytest.mark.parametrize("name", ["dw_sector", "cpwc_tukey", "pw_rf_hanning", "focused_hybrid", "cpwc_depth0"])
+@pytest.mark.parametrize("name", ["dw_sector", "dw_sector_patch", "cpwc_tukey", "pw_rf_hanning", "focused_hybrid",
+                                  "cpwc_depth0"])
 @pytest.mark.parametrize("separable", [True, False])
 @pytest.mark.parametrize("interp", [Interpolation.linear, Interpolation.cubic])
 def test_transmit_staging_equals_gathers(cuda, name, separable, interp):

@@ -336,6 +336,20 @@ __device__ __forceinline__ void pixel_xyz(const DevScan& s, int p, double& x,
   if (s.ref_kind == 0) rref = z;
 }
 
+// Pixel of lane `lane` (0..63) of 64-pixel tile `tile`. ps = 0: pixels
+// tile*64 .. tile*64+63 in the scan order. ps > 0 (a structured scan whose
+// fastest axis length n0 is a multiple of 2^ps and whose line count
+// n1*n2 is a multiple of 64 >> ps, n1 itself too): the tile is a patch of
+// 2^ps consecutive depths by 64 >> ps consecutive scan lines, lanes
+// depth-fastest; tiles run down the depth first (n0 >> ps of them per
+// group of lines), then across the line groups.
+__device__ __forceinline__ int tile_pixel(int ps, int n0, int tile, int lane) {
+  if (ps == 0) return tile * 64 + lane;
+  const int ntz = n0 >> ps;
+  const int grp = tile / ntz, tz = tile - grp * ntz;
+  return ((tz << ps) | (lane & ((1 << ps) - 1))) + n0 * ((grp << (6 - ps)) + (lane >> ps));
+}
+
 // Receive apodization parameters (one uff.apodization with probe set).
 enum { RXAP_ONES = 0, RXAP_WINDOW = 1, RXAP_VECTOR = 2, RXAP_MATRIX = 3 };
 enum { RXGEO_FLAT = 0, RXGEO_ORIGIN = 1, RXGEO_CURV = 2 };

@@ -63,7 +63,7 @@ __global__ __launch_bounds__(64) void geom_tile_scan_kernel(GeomLaunch G, const 
                                                             int* __restrict__ tile_waves, int* __restrict__ tile_nw,
                                                             int* __restrict__ tile_bad) {
   const int tile = blockIdx.x;
-  const int p = tile * 64 + (int)threadIdx.x;
+  const int p = tile_pixel(G.tile_ps, G.scan.n0, tile, (int)threadIdx.x);
   const bool active = p < G.P;
   const size_t PS = (size_t)G.P;
   double px = 0.0, py = 0.0, pz = 0.0, rref = 0.0;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(64) void geom_nan_mask_kernel(GeomLaunch G, const f
                                                            const int* __restrict__ tile_nw,
                                                            unsigned long long* __restrict__ mask) {
   const int tile = blockIdx.x;
-  const int p = tile * 64 + (int)threadIdx.x;
+  const int p = tile_pixel(G.tile_ps, G.scan.n0, tile, (int)threadIdx.x);
   const bool active = p < G.P;
   const size_t PS = (size_t)G.P;
   double px = 0.0, py = 0.0, pz = 0.0, rref = 0.0;

@@ -56,7 +56,7 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
   using M = SampleMode<SM>;
   constexpr int TP = 64;
   constexpr int ST = M::kStride;
-  const int p = tile * TP + lane;
+  const int p = tile_pixel(A.tile_ps, A.tile_n0, tile, lane);
   const bool active = p < A.P;
   const size_t PS = (size_t)A.P;
   const int nw = __builtin_amdgcn_readfirstlane(A.tile_nw[tile]);
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, f
   const int nf = min(FC, F - f0);
   float2 v[FC];
   tx_channel<SM, FC, PH, SAFE, STAGE>(A, wd, tile, n, f0, nf, lane, v, win + (STAGE ? s * FC * kWin : 0));
-  const int p = tile * TP + lane;
+  const int p = tile_pixel(A.tile_ps, A.tile_n0, tile, lane);
   if (p < A.P) {
     const size_t PS = (size_t)A.P;
 #pragma unroll
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_cf_part_kernel(RxAcq A, int F,
 #pragma unroll
     for (int f = 0; f < FC; ++f) v[f] = make_float2(0.0f, 0.0f);
   }
-  const int p = tile * TP + lane;
+  const int p = tile_pixel(A.tile_ps, A.tile_n0, tile, lane);
   const size_t PS = (size_t)A.P;
 #pragma unroll
   for (int f = 0; f < FC; ++f) {

@@ -94,6 +94,10 @@ struct GeomLaunch {
   // are all finite; the order is non-null when tile_good < tiles.
   const int* tile_order;
   int tile_good;
+  // Pixels of the variant-2 tiles (tile_pixel): 0 = 64 consecutive pixels;
+  // 4 = patches of 16 depths x 4 scan lines (dimension transmit on a linear
+  // or sector scan whose axes divide evenly). Fixed when the plan is made.
+  int tile_ps;
   // Stage sample windows through LDS (1) or gather every term's samples
   // (0): dimension both through stage_mask, dimension transmit (variant 2)
   // per (channel, wave). Results do not depend on it.

@@ -69,6 +69,7 @@ struct RxAcq {
   const int* tile_waves;   // [tiles][W] their indices, ascending
   const float2* ref;       // [P] reference phase, or null
   int P, N, W, L;
+  int tile_ps, tile_n0;    // the tiles' pixels: tile_pixel(tile_ps, tile_n0, tile, lane)
   float t0, fs;
   int frame_bytes;         // W * N * L * stride (< 0x7FFFFE00)
   int rx_compact;          // RF plans (wd = 0): the receive table holds float2 {rd, ra}
@@ -588,6 +589,8 @@ inline RxAcq rx_acq(const GeomLaunch& G, int sm) {
   A.N = G.N;
   A.W = G.W;
   A.L = G.L;
+  A.tile_ps = G.tile_ps;
+  A.tile_n0 = G.scan.n0;
   A.t0 = G.t0;
   A.fs = G.fs;
   A.frame_bytes = G.L * G.N * G.W * (sm == SM_COEF ? 32 : 8);
