@@ -534,6 +534,8 @@ def run_cf(args, torch, dev):
                      **acq.plan_kwargs())
     if args.wave_chunk or args.split:
         plan.set_tiles(args.wave_chunk, args.split)
+    if args.stage >= 0:
+        plan.set_staging(bool(args.stage))
     M = plan.active_elements(L.DIM_RX)
     per_channel = torch.empty(plan.out_shape, dtype=torch.complex64, device=dev)
     image = torch.empty((plan.F, plan.P), dtype=torch.complex64, device=dev)

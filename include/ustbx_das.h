@@ -320,12 +320,14 @@ int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st);
  * marked (all 64 pixels' terms in one 64-sample window of each wave's trace,
  * every weight nonzero) from windows loaded with one contiguous load per
  * term and read back from LDS, 0 gathers each term's samples. Dimension
- * transmit on variant 2: 1 stages each tile's window of every (channel, wave)
- * found at run time, 0 (the default) gathers. Batches take the setting of
- * each plan when they are created. */
+ * transmit on variant 2: 1 runs every (tile, channel) the plan marked (each
+ * wave's terms on the tile within one 64-sample window per frame) from LDS,
+ * 0 (the default) gathers. Batches take the setting of each plan when they
+ * are created. */
 int ustbx_plan_set_staging(ustbx_plan* plan, int32_t on, ustbx_status* st);
-/* (tile, channel) pairs of a dimension-both receive-table plan that run
- * staged, and all pairs of the tiles' channel lists (0, 0 otherwise). */
+/* (tile, channel) pairs of a receive-table plan (dimension both or
+ * transmit) that the plan marked for staging, and all pairs of the tiles'
+ * channel lists (0, 0 otherwise). */
 int ustbx_plan_stage_stats(const ustbx_plan* plan, int64_t* staged, int64_t* listed, ustbx_status* st);
 /* Current variant (or -1 for NULL) and its tile shape. */
 int32_t ustbx_plan_variant(const ustbx_plan* plan, int32_t* wave_chunk, int32_t* split);

@@ -67,10 +67,12 @@ def geom_cases():
     cases["dw_sector"] = dict(probe=prb, scan=scan, seq=seq, rx=Apodization(Window.hamming, f_number=1.2),
                               tx=Apodization(Window.tukey50, f_number=1.0), fs=10e6, fd=2.5e6, L=600, model=3,
                               scan_kind="sector")
-    # the same on 48 depths x 12 lines: axes that the transmit kernel's
-    # 16-depth x 4-line patch tiles divide (the other cases keep 64-pixel runs)
-    cases["dw_sector_patch"] = dict(cases["dw_sector"], scan=SectorScan(np.deg2rad(np.linspace(-45, 45, 12)),
-                                                                        np.linspace(1e-3, 40e-3, 48)))
+    # the same on 48 depths x 12 lines of a narrow sector: axes that the
+    # transmit kernel's 16-depth x 4-line patch tiles divide (the other cases
+    # keep 64-pixel runs), sampled finely enough (1.6 samples per depth step)
+    # that a tile's sample windows fit the staged form's 64-sample rows
+    cases["dw_sector_patch"] = dict(cases["dw_sector"], scan=SectorScan(np.deg2rad(np.linspace(-5, 5, 12)),
+                                                                        np.linspace(20e-3, 26e-3, 48)))
     # RF plane wave on a linear scan of 32 depths x 8 lines (patch tiles)
     cases["pw_rf_patch"] = dict(cases["pw_rf_hanning"], scan=LinearScan(np.linspace(-4e-3, 4e-3, 8),
                                                                          np.linspace(5e-3, 25e-3, 32)))

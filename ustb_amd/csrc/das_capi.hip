@@ -611,8 +611,9 @@ bool rxtab_build(ustbx_plan* pl, hipError_t& err, bool otf) {
   const size_t b_ord = al(sizeof(int) * tiles);
   const size_t b_mask = al(sizeof(unsigned long long) * tiles);
   const size_t b_otf = otf ? al(sizeof(RxOtf)) + al(sizeof(float4) * (size_t)G.N) : 0;
-  // stage mask: dimension both with the receive table
-  const int stage_words = (!otf && G.dim == 3) ? (G.N + 63) / 64 : 0;
+  // stage mask: the receive table, dimension both (bits by channel-list
+  // position) or transmit (bits by channel)
+  const int stage_words = (!otf && (G.dim == 3 || G.dim == 2)) ? (G.N + 63) / 64 : 0;
   const size_t b_smask = al(sizeof(unsigned long long) * tiles * (size_t)stage_words);
   const size_t total = b_rx + b_nc + b_cl + b_wl + b_nw + b_bad + b_ref + 2 * b_ord + b_mask + b_otf + b_smask;
   char* m = nullptr;

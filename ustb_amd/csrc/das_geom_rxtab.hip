@@ -190,10 +190,65 @@ __global__ __launch_bounds__(64) void geom_stage_mask_kernel(GeomLaunch G, const
   }
 }
 
+// Dimension transmit: per tile, bit n (channel n, as the transmit kernel
+// indexes them) when channel n's terms on the tile can run staged
+// (tx_channel): for every wave of the tile's list, with the transmit kernel's
+// own term arithmetic (a zero weight's delay replaced by kFar, term_index,
+// ok = the index in the sample window), every ok lane's n0 lies in [nref -
+// kTxSlack, nref + kMaxSpan - kTxSlack], nref being the n0 of the first ok
+// lane. Tiles with a non-finite nonzero weight or delay get no bits. One
+// wave64 per tile.
+template <int SM>
+__global__ __launch_bounds__(64) void geom_tx_stage_mask_kernel(GeomLaunch G, const int* __restrict__ tile_bad,
+                                                                unsigned long long* __restrict__ mask) {
+  const int tile = blockIdx.x;
+  const int lane = (int)threadIdx.x;
+  const int p = tile_pixel(G.tile_ps, G.scan.n0, tile, lane);
+  const bool active = p < G.P;
+  unsigned long long* out = mask + (size_t)tile * G.stage_words;
+  const int nw = G.tile_nw[tile];
+  const bool run = tile_bad[tile] == 0;
+  const int* __restrict__ wl = G.tile_waves + (size_t)tile * G.W;
+  const size_t PS = (size_t)G.P;
+  const float lim = (float)(SM == SM_LINEAR ? G.L - 1 : G.L - 2);
+  unsigned long long word = 0ull;
+  for (int n = 0; n < G.stage_words * 64; ++n) {
+    if (run && n < G.N) {
+      const float4 rv = active ? rx_entry(G.rx_table, G.rx_compact, (size_t)n * PS + p) : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float rd = rv.y != 0.0f ? rv.x : kFar;
+      bool good = true;
+      for (int i = 0; i < nw && good; ++i) {
+        const float4 tv = active ? G.tx_table[(size_t)wl[i] * PS + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float d = term_index(tv.y != 0.0f ? tv.x : kFar, rd, G.t0, G.fs);
+        const bool ok = d >= 1.0f && d < lim;
+        const int n0 = ok ? (int)d : 0;
+        const unsigned long long okm = __ballot(ok);
+        const unsigned nref = okm ? (unsigned)__shfl(n0, (int)__builtin_ctzll(okm)) : 0u;
+        const unsigned rel = (unsigned)n0 - nref + (unsigned)kTxSlack;  // in [0, kMaxSpan] where staged
+        good = __all(!ok || rel <= (unsigned)StageWin<SM>::kMaxSpan);
+      }
+      if (good) word |= 1ull << (n & 63);
+    }
+    if ((n & 63) == 63) {
+      if (lane == 0) out[n >> 6] = word;
+      word = 0ull;
+    }
+  }
+}
+
 hipError_t launch_stage_mask(const GeomLaunch& G, const int* tile_bad, unsigned long long* stage_mask,
                              hipStream_t stream) {
-  if (!G.stage_words || !G.rx_table || G.dim != 3 || G.rx_otf) return hipErrorInvalidValue;
+  if (!G.stage_words || !G.rx_table || !(G.dim == 3 || G.dim == 2) || G.rx_otf) return hipErrorInvalidValue;
   const unsigned tiles = (unsigned)((G.P + 63) / 64);
+  if (G.dim == 2) {
+    if (G.interp == INTERP_CUBIC)
+      hipLaunchKernelGGL(geom_tx_stage_mask_kernel<SM_CUBIC>, dim3(tiles), dim3(64), 0, stream, G, tile_bad,
+                         stage_mask);
+    else
+      hipLaunchKernelGGL(geom_tx_stage_mask_kernel<SM_LINEAR>, dim3(tiles), dim3(64), 0, stream, G, tile_bad,
+                         stage_mask);
+    return hipGetLastError();
+  }
   if (G.interp == INTERP_CUBIC)
     hipLaunchKernelGGL(geom_stage_mask_kernel<SM_CUBIC>, dim3(tiles), dim3(64), 0, stream, G, tile_bad, stage_mask);
   else

@@ -38,18 +38,20 @@ namespace ustbx {
 // nonzero weight or its delay is not finite): a term enters the sum only when
 // it is in the window, and the separable receive factor multiplies the sum
 // only when some term did, as das_c skips the others before multiplying.
-// Samples per frame of a wave's LDS window (STAGE): 64 lanes x 2.
-constexpr int kWin = 128;
+// STAGE (branch-free tiles only): the channels that the plan's stage mask
+// marks for the tile (bit n of its words) read their samples from LDS. Per
+// wave of the tile's list, the reference index nref is the n0 of the first
+// lane whose term is in the window (ok); the mask guarantees that every ok
+// lane's n0 lies in [nref - kTxSlack, nref + kMaxSpan - kTxSlack]. One
+// contiguous 8-byte-per-lane load per frame fills the wave's row win[f][0,
+// 64) with samples nref - kBack - kTxSlack .. + 63 of the trace; ok lanes
+// read their term's samples back from there, the others from the row's
+// zero slots [64, 68) (the gather path reads zeros for them through an
+// out-of-range offset). The LDS operations of one wave complete in order, so
+// the reads see the writes. The arithmetic is the gather path's, so the two
+// agree bit for bit.
+constexpr int kTxRow = kStageWin + 4;  // float2 slots per frame row: window + zeros
 
-// STAGE (branch-free tiles only): per (channel, wave) the tile's samples of
-// each frame are staged through LDS. The wave-uniform window [s0, s0 + span)
-// covers every in-window term's samples (the lanes' sample indices reduced
-// to their minimum and maximum); lane j loads samples s0 + 2j and s0 + 2j + 1
-// of each frame with one 16-byte load (the loads of a wave are contiguous),
-// writes them to the wave's LDS region win [FC][kWin], and every lane then
-// reads its samples from LDS. A window wider than kWin samples falls back to
-// one gather per term and frame. The arithmetic is the gather path's, so the
-// two give the same result bit for bit.
 template <int SM, int FC, int PH, bool SAFE, bool STAGE>
 __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, int n, int f0, int nf, int lane,
                                            float2 (&v)[FC], float2* __restrict__ win) {
@@ -109,6 +111,19 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
   int w = nw > 0 ? __builtin_amdgcn_readfirstlane(wl[0]) : 0;
   int w_next = nw > 1 ? __builtin_amdgcn_readfirstlane(wl[1]) : 0;
   bool any = false;  // SAFE: some term of the channel was in the window
+  // staged: the plan's stage mask marks channel n on this tile (a scalar
+  // load through the constant address space; the mask is read-only here)
+  bool stg = false;
+  unsigned wbase = 0u, lane8 = 0u;
+  if (STAGE && !SAFE && A.stage_mask) {
+    const unsigned long long word =
+        ((const __attribute__((address_space(4))) unsigned long long*)A.stage_mask)[(size_t)tile * A.stage_words +
+                                                                                    (n >> 6)];
+    stg = (word >> (n & 63)) & 1ull;
+    wbase = (unsigned)(size_t)(__attribute__((address_space(3))) char*)win;  // LDS address
+    lane8 = (unsigned)lane * 8u;
+    asm volatile("" : "+v"(lane8));  // keeps window addresses one scalar-plus-vector add
+  }
   float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
   if (active && nw > 0) vn = A.tx[(size_t)w * PS + p];
   for (int i = 0; i < nw; ++i) {
@@ -150,40 +165,35 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
     // the first gather's offset folded into the wave-uniform part (n0 >= 1
     // wherever ok holds, so the sum stays non-negative)
     float4 q[FC][M::kLoads];
-    bool staged = false;
-    if (STAGE) {
-      const unsigned lo = wave_reduce_u32<false>(ok ? (unsigned)n0 : 0xFFFFFFFFu);
-      if (lo == 0xFFFFFFFFu) continue;  // no term of this wave in the window: nothing to add
-      const unsigned hi = wave_reduce_u32<true>(ok ? (unsigned)n0 : 0u);
-      const int s0 = (int)lo + (SM == SM_CUBIC ? -1 : 0);  // first sample of the window (>= 0)
-      const int span = (int)(hi - lo) + (SM == SM_CUBIC ? 4 : 2);
-      if (span <= kWin) {
-        staged = true;
-        if (lane < ((span + 1) >> 1)) {
-          const unsigned go = (unsigned)((s0 + 2 * lane) * 8) + nb + (unsigned)(wc * wst);
-          float4 st[FC];
+    if (STAGE && stg) {
+      using S = StageWin<SM>;
+      const unsigned long long okm = __builtin_amdgcn_ballot_w64(ok);
+      const unsigned nref =
+          okm ? (unsigned)__builtin_amdgcn_readlane(n0, (int)__builtin_ctzll(okm)) : 0u;  // wave-uniform
+      // byte offset of the window's first sample (wraps below sample 0: those
+      // slots hold zeros or the previous trace, and no lane reads them)
+      const unsigned sb = nb + (unsigned)(wc * wst) + (nref - (unsigned)(S::kBack + kTxSlack)) * 8u;
+      float2 st[FC];
 #pragma unroll
-          for (int f = 0; f < FC; ++f) st[f] = buf_load16(rs, (int)(go + (unsigned)(f * fb)));
+      for (int f = 0; f < FC; ++f) st[f] = buf_load8(rs, (int)(sb + (unsigned)(f * fb) + lane8));
 #pragma unroll
-          for (int f = 0; f < FC; ++f) reinterpret_cast<float4*>(win + f * kWin)[lane] = st[f];
-        }
-        // this lane's first sample in the window (n0 - 1 for cubic), kept
-        // inside the wave's region for the lanes outside the window
-        const int li = min(max(ok ? n0 - (int)lo : 0, 0), kWin - (SM == SM_CUBIC ? 4 : 2));
-        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int f = 0; f < FC; ++f)
+        *reinterpret_cast<lds_f2*>((size_t)(wbase + lane8) + f * kTxRow * 8) = f2v{st[f].x, st[f].y};
+      __builtin_amdgcn_wave_barrier();
+      // LDS address of the first sample the term reads (n0 - kBack), or of
+      // the zero slots
+      const unsigned la =
+          ok ? lshl3_add((unsigned)n0, wbase + ((unsigned)kTxSlack - nref) * 8u) : wbase + kStageWin * 8u;
 #pragma unroll
-        for (int f = 0; f < FC; ++f) {
-          const float2* w = win + f * kWin + li;
-          const float2 a = w[0], b1 = w[1];
-          q[f][0] = ok ? make_float4(a.x, a.y, b1.x, b1.y) : z4;
-          if (M::kLoads == 2) {
-            const float2 c = w[2], d2 = w[3];
-            q[f][M::kLoads - 1] = ok ? make_float4(c.x, c.y, d2.x, d2.y) : z4;
-          }
+      for (int f = 0; f < FC; ++f) {
+        const f4a8 lo = *reinterpret_cast<const lds_f4a8*>((size_t)la + f * kTxRow * 8);
+        q[f][0] = make_float4(lo.x, lo.y, lo.z, lo.w);
+        if (M::kLoads == 2) {
+          const f4a8 hi = *reinterpret_cast<const lds_f4a8*>((size_t)la + f * kTxRow * 8 + 16);
+          q[f][M::kLoads - 1] = make_float4(hi.x, hi.y, hi.z, hi.w);
         }
       }
-    }
-    if (!staged) {
+    } else {
       const unsigned off =
           ok ? (unsigned)(n0 * ST) + (nb + (unsigned)(wc * wst) + (unsigned)M::kLo) : (unsigned)kOOB;
 #pragma unroll
@@ -225,6 +235,7 @@ __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, i
       }
       part[f] = (!SAFE || ok) ? np : part[f];
     }
+    if (STAGE && stg) __builtin_amdgcn_wave_barrier();  // this wave's reads before the next wave's writes
   }
   const float2 refp = (active && A.ref) ? A.ref[p] : make_float2(1.0f, 0.0f);
 #pragma unroll
@@ -241,7 +252,7 @@ template <int SM, int FC, int CS, int PH, bool SAFE, bool STAGE>
 __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, float wd,
                                                                const int* __restrict__ order, int tiles) {
   constexpr int TP = 64;
-  __shared__ __align__(16) float2 win[STAGE ? CS * FC * kWin : 1];
+  __shared__ __align__(16) float2 win[STAGE ? CS * FC * kTxRow : 1];
   const int lane = threadIdx.x & (TP - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
   const int chunks = (F + FC - 1) / FC;
@@ -254,8 +265,12 @@ __global__ __launch_bounds__(64 * CS) void das_tx_rxtab_kernel(RxAcq A, int F, f
   const int tile = order ? __builtin_amdgcn_readfirstlane(order[rank]) : rank;
   const int f0 = (slab - group * chunks) * FC;
   const int nf = min(FC, F - f0);
+  if (STAGE && lane < FC * 4) {  // this wave's zero slots, never overwritten
+    win[s * FC * kTxRow + (lane >> 2) * kTxRow + kStageWin + (lane & 3)] = make_float2(0.0f, 0.0f);
+    __builtin_amdgcn_wave_barrier();
+  }
   float2 v[FC];
-  tx_channel<SM, FC, PH, SAFE, STAGE>(A, wd, tile, n, f0, nf, lane, v, win + (STAGE ? s * FC * kWin : 0));
+  tx_channel<SM, FC, PH, SAFE, STAGE>(A, wd, tile, n, f0, nf, lane, v, win + (STAGE ? s * FC * kTxRow : 0));
   const int p = tile_pixel(A.tile_ps, A.tile_n0, tile, lane);
   if (p < A.P) {
     const size_t PS = (size_t)A.P;
@@ -278,7 +293,7 @@ __global__ __launch_bounds__(64 * CS) void das_tx_cf_part_kernel(RxAcq A, int F,
                                                                  float4* __restrict__ part) {
   constexpr int TP = 64;
   __shared__ float2 red[CS][TP];
-  __shared__ __align__(16) float2 win[STAGE ? CS * FC * kWin : 1];
+  __shared__ __align__(16) float2 win[STAGE ? CS * FC * kTxRow : 1];
   const int lane = threadIdx.x & (TP - 1);
   const int s = __builtin_amdgcn_readfirstlane((int)threadIdx.x / TP);
   const int chunks = (F + FC - 1) / FC;
@@ -290,9 +305,13 @@ __global__ __launch_bounds__(64 * CS) void das_tx_cf_part_kernel(RxAcq A, int F,
   const int tile = order ? __builtin_amdgcn_readfirstlane(order[rank]) : rank;
   const int f0 = (slab - group * chunks) * FC;
   const int nf = min(FC, F - f0);
+  if (STAGE && lane < FC * 4) {  // this wave's zero slots, never overwritten
+    win[s * FC * kTxRow + (lane >> 2) * kTxRow + kStageWin + (lane & 3)] = make_float2(0.0f, 0.0f);
+    __builtin_amdgcn_wave_barrier();
+  }
   float2 v[FC];
   if (n < A.N) {
-    tx_channel<SM, FC, PH, SAFE, STAGE>(A, wd, tile, n, f0, nf, lane, v, win + (STAGE ? s * FC * kWin : 0));
+    tx_channel<SM, FC, PH, SAFE, STAGE>(A, wd, tile, n, f0, nf, lane, v, win + (STAGE ? s * FC * kTxRow : 0));
   } else {
 #pragma unroll
     for (int f = 0; f < FC; ++f) v[f] = make_float2(0.0f, 0.0f);

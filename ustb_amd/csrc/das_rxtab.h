@@ -120,6 +120,11 @@ struct StageWin {
   static constexpr int kMaxSpan = SM == SM_CUBIC ? 60 : 62;  // kStageWin - (samples a term reads)
 };
 
+// The transmit kernel's windows start kTxSlack samples before lane 0's
+// first sample, since on tiles of several scan lines lane 0 need not have
+// the tile's smallest index.
+constexpr int kTxSlack = 4;
+
 __device__ __forceinline__ float2 buf_load8(__amdgpu_buffer_rsrc_t r, int byte_off) {
   typedef unsigned u2v __attribute__((ext_vector_type(2)));
   const u2v v = __builtin_bit_cast(u2v, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));

@@ -280,8 +280,8 @@ class GeometryPlan:
 
     @property
     def stage_stats(self) -> tuple[int, int]:
-        """(staged, listed) (tile, channel) pairs of a dimension-both
-        receive-table plan."""
+        """(staged, listed) (tile, channel) pairs of a receive-table plan
+        (dimension both or transmit)."""
         a, b, st = ctypes.c_int64(), ctypes.c_int64(), L.Status()
         with torch.cuda.device(self.device):
             L.check(L.lib().ustbx_plan_stage_stats(self.handle, ctypes.byref(a), ctypes.byref(b),
