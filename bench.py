@@ -670,11 +670,24 @@ def run_matrix(args, torch, dev):
         pix = _sample_pixels(acq.P, 2048, seed=5)
         sel = lambda t: np.ascontiguousarray(t.cpu().numpy()[:, pix])
         ref = O.das_c(acq.data, acq.fs, acq.t0, sel(ta), sel(ra), sel(td), sel(rd), acq.fd, dim)
+        if interp == 1:
+            # das_c is linear: its parity from a linear call on the same
+            # matrices; the cubic image against the FP64 beamformer with das_c's
+            # FP32 index
+            got_c = out.cpu().numpy()[..., pix]
+            ref64 = O.beamform_fp64(acq.data, acq.fs, acq.t0, sel(ta), sel(ra), sel(td), sel(rd), acq.fd, dim,
+                                    interp="cubic", w0=O.dasc_wd(acq.fd), index="fp32")
+            ops.das(ch, acq.fs, acq.t0, ta, ra, td, rd, acq.fd, dim, interp=0, split=args.split, out=out)
+            torch.cuda.synchronize()
         got = out.cpu().numpy()[..., pix]
         peak = float(np.nanmax(np.abs(ref)))
         result["errors"] = {"pixels_sampled": int(pix.size),
-                            "vs_dasc": float(np.nanmax(np.abs(got - ref)) / peak),
-                            "bitwise_vs_dasc": bool(np.array_equal(got, ref, equal_nan=True))}
+                            "vs_dasc" if interp == 0 else "linear_vs_dasc": float(np.nanmax(np.abs(got - ref)) / peak),
+                            "bitwise_vs_dasc" if interp == 0 else "linear_bitwise_vs_dasc":
+                                bool(np.array_equal(got, ref, equal_nan=True))}
+        if interp == 1:
+            result["errors"]["cubic_vs_fp64_dasc_index"] = float(np.nanmax(np.abs(got_c - ref64)) /
+                                                                 float(np.nanmax(np.abs(ref64))))
         log(f"errors {result['errors']}")
     print(json.dumps(result))
 

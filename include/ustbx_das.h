@@ -113,8 +113,7 @@ typedef struct ustbx_das_desc {
   int32_t interp;     /* USTBX_INTERP_*            */
   int32_t edge;       /* USTBX_EDGE_*              */
   uint32_t flags;     /* USTBX_FLAG_*              */
-  int32_t split;      /* channel groups per pixel tile: 1, 2, 4, 8 (0 = auto: enough waves
-                         to fill the device, P/64 x split >= 8192) */
+  int32_t split;      /* channel groups per pixel tile: 1, 2, 4, 8 (0 = auto: 8) */
 } ustbx_das_desc;
 
 /* All pointers are device (HBM) pointers; `out` is fully written (no need to

@@ -31,13 +31,13 @@ float dasc_wd(float fd) {
 
 int auto_split(int P) { return P >= (1 << 18) ? 1 : 4; }
 
-// Channel groups of the matrix kernel: enough waves to fill the chip, P/64 x
-// split >= 8192 (32 per CU): 8 at 64 K pixels, 1 from 512 K.
-int matrix_auto_split(int P) {
-  int s = 1;
-  while (s < 8 && (int64_t)P * s < (int64_t)8192 * 64) s *= 2;
-  return s;
-}
+// Channel groups of the matrix kernel: 8, the fastest of 1, 2, 4, 8 on C1
+// (65 K pixels), C2 (512 K pixels, dimension both: 8.74 ms against 9.98 ms
+// with 1 group) and C3 (262 K pixels, transmit: 21.5 ms against 30.2 ms with
+// 2); the groups' shorter channel loops keep more blocks' sample windows in
+// L2 at once. Dimension transmit and none sum nothing across groups, so
+// there the choice does not change the result.
+int matrix_auto_split(int /*P*/) { return 8; }
 
 // The shared-receive-state kernel covers dimension both with the separable
 // phase on one frame.
