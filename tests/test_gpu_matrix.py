@@ -151,7 +151,7 @@ def test_host_entry_chunked_staging_matches_device(cuda):
         assert np.array_equal(out, ref)
 
 
-@pytest.mark.parametrize("dim", [0, 3])
+@pytest.mark.parametrize("dim", [0, 2, 3])
 def test_nonfinite_and_huge_delays_and_nan_weights(cuda, dim):
     """das_c converts d = (tau - t0) * fs with (int): NaN, +-Inf and |d| >= 2^31
     fail its 0 < n0 < L-1 test and the term is skipped; a NaN weight is nonzero
