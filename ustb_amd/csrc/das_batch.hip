@@ -63,7 +63,7 @@ __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
   __shared__ float2 red[WS][64];
   // receive entries of on-the-fly acquisitions, or the staged channels' windows
   constexpr int kRxs = (int)sizeof(float4) * kOtfChunk * 64;
-  constexpr int kWin = SM == SM_COEF ? 0 : (int)sizeof(float2) * WS * GatherBatch<SM>::RB * kStageWin;
+  constexpr int kWin = SM == SM_COEF ? 0 : (int)sizeof(float2) * WS * StageBatch<SM>::RB * kStageWin;
   __shared__ __align__(16) char pool[kRxs > kWin ? kRxs : kWin];
   float4(*rxs)[64] = reinterpret_cast<float4(*)[64]>(pool);
   float2* win = reinterpret_cast<float2*>(pool);

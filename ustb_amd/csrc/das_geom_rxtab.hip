@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void spline_coef_kernel(const float2* __res
 // coefficient data, which is never staged).
 template <int SM, int WS>
 struct StageLds {
-  static constexpr int kFloat2 = SM == SM_COEF ? 1 : WS * GatherBatch<SM>::RB * kStageWin;
+  static constexpr int kFloat2 = SM == SM_COEF ? 1 : WS * StageBatch<SM>::RB * kStageWin;
 };
 
 template <int SM, int WC, int WS, int RXL>

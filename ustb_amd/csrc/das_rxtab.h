@@ -135,11 +135,21 @@ __device__ __forceinline__ float2 buf_load8(__amdgpu_buffer_rsrc_t r, int byte_o
 // waves, in batches of RB terms whose gathers are all issued before the
 // first is used. Slots j >= nj are skipped when GUARD (a wave-uniform branch
 // per batch); without it the slots past the tile's waves hold the far delay
-// and read zeros. RB: 6 terms (12 gathers) in flight for Keys' cubic, 5 for
-// the one-gather modes (the best of 3-6 measured on C2, cubic, and on the
-// 3-D and STA classes, linear).
+// and read zeros. RB: 3 terms (6 gathers) in flight for Keys' cubic, 5 for
+// the one-gather modes. For cubic the batch sets the kernel's registers: with
+// 6 terms the C2 kernel needs 115 VGPRs and holds 4 waves per SIMD, with 3 it
+// fits in 96 and holds 5 (C2 2.49 -> 2.38 ms; the staged channels keep their
+// own batch of 6, StageBatch). Linear: the best of 3-6 on the 3-D and STA
+// classes.
 template <int SM>
 struct GatherBatch {
+  static constexpr int RB = SM == SM_CUBIC ? 3 : 5;
+};
+// Terms per batch of the staged channels (rxtab_channel_staged): one 8-byte
+// window load per term, so batches are cheaper in registers; 6 and 8 measured
+// alike on C2, 10 slower.
+template <int SM>
+struct StageBatch {
   static constexpr int RB = SM == SM_CUBIC ? 6 : 5;
 };
 
@@ -243,7 +253,7 @@ __device__ __forceinline__ float2 rxtab_channel_staged(__amdgpu_buffer_rsrc_t rs
                                                        float2* __restrict__ win, unsigned lane8) {
   static_assert(SM != SM_COEF, "coefficient data is not staged");
   using S = StageWin<SM>;
-  constexpr int RB = GatherBatch<SM>::RB;
+  constexpr int RB = StageBatch<SM>::RB;
   const unsigned wbase = (unsigned)(size_t)(__attribute__((address_space(3))) char*)win;  // LDS address
   float2 part = make_float2(0.0f, 0.0f);
 #pragma unroll
@@ -367,7 +377,7 @@ enum { RXL_FULL = 0, RXL_COMPACT = 1, RXL_EITHER = 2 };
 // One tile of one frame; every thread of the block calls it (it ends with a
 // block barrier and the split-ordered reduction through red). STG: the
 // channels A.stage_mask marks run staged, split s through the LDS slots
-// win[s * GatherBatch<SM>::RB * kStageWin ..] (SAFE tiles never).
+// win[s * StageBatch<SM>::RB * kStageWin ..] (SAFE tiles never).
 template <int SM, int WC, int WS, bool SAFE = false, int RXL = RXL_EITHER, bool STG = false>
 __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, float2 (*red)[64],
                                            float2* win = nullptr) {
@@ -406,7 +416,7 @@ __device__ __forceinline__ void rxtab_tile(const RxAcq& A, int tile, int f, floa
   constexpr bool kStg = STG && !SAFE && SM != SM_COEF;
   const unsigned long long* smask =
       kStg && A.stage_mask ? A.stage_mask + (size_t)tile * (size_t)A.stage_words : nullptr;
-  float2* swin = kStg ? win + s * (GatherBatch<SM>::RB * kStageWin) : nullptr;
+  float2* swin = kStg ? win + s * (StageBatch<SM>::RB * kStageWin) : nullptr;
   // 8 * lane, opaque to the compiler so that window addresses stay one
   // scalar-plus-vector add
   unsigned lane8 = (unsigned)lane * 8u;
