@@ -900,6 +900,9 @@ def run_serve(args, torch, dev):
     t = time.time()
     cfgs, datasets = configs.c5()
     plans = [make_plan(c.probe, c.scan, c.sequence, separable_phase=True, **c.plan_kwargs()) for c in cfgs]
+    if args.stage >= 0:
+        for p in plans:
+            p.set_staging(bool(args.stage))
     torch.cuda.synchronize()
     log(f"C5: {len(cfgs)} configurations, data and plans in {time.time() - t:.1f}s")
     srv = Server(plans, datasets, hbm_budget=64 << 30, slice_ms=args.slice_ms, tiles=(8, 4),

@@ -321,8 +321,8 @@ int ustbx_plan_set_order(ustbx_plan* plan, int32_t order, ustbx_status* st);
  * term and read back from LDS, 0 gathers each term's samples. Dimension
  * transmit on variant 2: 1 runs every (tile, channel) the plan marked (each
  * wave's terms on the tile within one 64-sample window per frame) from LDS,
- * 0 (the default) gathers. Batches take the setting of each plan when they
- * are created. */
+ * 0 (the default) gathers. Batches and batchers gather every term whatever
+ * the setting (their persistent kernel holds fewer registers so). */
 int ustbx_plan_set_staging(ustbx_plan* plan, int32_t on, ustbx_status* st);
 /* (tile, channel) pairs of a receive-table plan (dimension both or
  * transmit) that the plan marked for staging, and all pairs of the tiles'

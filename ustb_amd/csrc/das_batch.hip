@@ -58,15 +58,18 @@ __global__ void batch_reset_kernel(int* ctr) {
   if (threadIdx.x < kQueues) ctr[threadIdx.x] = 0;
 }
 
-template <int SM, int WC, int WS>
+// OTF: the form for batches with an on-the-fly acquisition (its tile body,
+// rxotf_tile, and LDS for its receive entries). The batch kernels gather
+// every term (no LDS-staged windows): the bodies a persistent kernel holds
+// set its registers for all of them, and without the staged and on-the-fly
+// bodies the kernel keeps 78 VGPRs and 6 waves per SIMD (97 and 4 with them:
+// C5 served 51 frames/s against 61).
+template <int SM, int WC, int WS, bool OTF>
 __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
   __shared__ float2 red[WS][64];
-  // receive entries of on-the-fly acquisitions, or the staged channels' windows
-  constexpr int kRxs = (int)sizeof(float4) * kOtfChunk * 64;
-  constexpr int kWin = SM == SM_COEF ? 0 : (int)sizeof(float2) * WS * StageBatch<SM>::RB * kStageWin;
-  __shared__ __align__(16) char pool[kRxs > kWin ? kRxs : kWin];
+  constexpr int kRxs = OTF ? (int)sizeof(float4) * kOtfChunk * 64 : 16;  // on-the-fly receive entries
+  __shared__ __align__(16) char pool[kRxs];
   float4(*rxs)[64] = reinterpret_cast<float4(*)[64]>(pool);
-  float2* win = reinterpret_cast<float2*>(pool);
   __shared__ int claimed;
   const int home = (int)(blockIdx.x % kQueues);  // blocks reach the XCDs round-robin
   for (;;) {
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
     const __attribute__((address_space(4))) int* w = (const __attribute__((address_space(4))) int*)B.items + 4 * it;
     const int acq = w[0], tile = w[1], frame = w[2];
     const __attribute__((address_space(4))) RxAcq* ad = (const __attribute__((address_space(4))) RxAcq*)(B.acqs + acq);
-    if (ad->rx_otf) {  // the receive state computed in the block
+    if (OTF && ad->rx_otf) {  // the receive state computed in the block
       if (w[3])
         rxotf_tile<SM, WC, WS, true>(load_constant(B.acqs + acq), tile, frame, red, rxs);
       else
@@ -98,57 +101,60 @@ __global__ __launch_bounds__(64 * WS) void das_batch_kernel(BatchLaunch B) {
     } else if (w[3]) {  // a nonzero weight or its delay not finite on this tile
       rxtab_tile<SM, WC, WS, true>(load_constant(B.acqs + acq), tile, frame, red);
     } else {
-      rxtab_tile<SM, WC, WS, false, RXL_EITHER, true>(load_constant(B.acqs + acq), tile, frame, red, win);
+      rxtab_tile<SM, WC, WS, false>(load_constant(B.acqs + acq), tile, frame, red);
     }
     __syncthreads();  // all threads are past this item before the next claim
   }
 }
 
-template <int SM, int WC, int WS>
+template <int SM, int WC, int WS, bool OTF>
 hipError_t batch_launch_t(const BatchLaunch& B, int nblocks, hipStream_t stream) {
-  hipLaunchKernelGGL((das_batch_kernel<SM, WC, WS>), dim3(nblocks), dim3(64 * WS), 0, stream, B);
+  hipLaunchKernelGGL((das_batch_kernel<SM, WC, WS, OTF>), dim3(nblocks), dim3(64 * WS), 0, stream, B);
   return hipGetLastError();
 }
 
-template <int SM, int WC, int WS>
+template <int SM, int WC, int WS, bool OTF>
 int batch_occupancy_t() {
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, das_batch_kernel<SM, WC, WS>, 64 * WS, 0) != hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, das_batch_kernel<SM, WC, WS, OTF>, 64 * WS, 0) !=
+      hipSuccess)
     return 0;
   return occ;
 }
 
-// Dispatch over the instantiated (sampling mode, waves per split, splits):
-// f is a functor with a template <int SM, int WC, int WS> call operator.
-template <int SM, int WS, typename F>
+// Dispatch over the instantiated (sampling mode, waves per split, splits,
+// on-the-fly form): f is a functor with a template <int SM, int WC, int WS,
+// bool OTF> call operator.
+template <int SM, int WS, bool OTF, typename F>
 auto dispatch_wc(int wc, const F& f) {
   switch (wc) {
-    case 4: return f.template operator()<SM, 4, WS>();
-    case 5: return f.template operator()<SM, 5, WS>();
-    case 6: return f.template operator()<SM, 6, WS>();
-    case 8: return f.template operator()<SM, 8, WS>();
-    case 10: return f.template operator()<SM, 10, WS>();
-    case 12: return f.template operator()<SM, 12, WS>();
-    default: return f.template operator()<SM, 16, WS>();
+    case 4: return f.template operator()<SM, 4, WS, OTF>();
+    case 5: return f.template operator()<SM, 5, WS, OTF>();
+    case 6: return f.template operator()<SM, 6, WS, OTF>();
+    case 8: return f.template operator()<SM, 8, WS, OTF>();
+    case 10: return f.template operator()<SM, 10, WS, OTF>();
+    case 12: return f.template operator()<SM, 12, WS, OTF>();
+    default: return f.template operator()<SM, 16, WS, OTF>();
   }
 }
 
-template <int SM, typename F>
+template <int SM, bool OTF, typename F>
 auto dispatch_ws(int wc, int ws, const F& f) {
-  if (ws == 4) return dispatch_wc<SM, 4>(wc, f);
-  if (ws == 5) return dispatch_wc<SM, 5>(wc, f);
-  return dispatch_wc<SM, 8>(wc, f);
+  if (ws == 4) return dispatch_wc<SM, 4, OTF>(wc, f);
+  if (ws == 5) return dispatch_wc<SM, 5, OTF>(wc, f);
+  return dispatch_wc<SM, 8, OTF>(wc, f);
 }
 
 template <typename F>
-auto dispatch(int sm, int wc, int ws, const F& f) {
-  return sm == SM_LINEAR ? dispatch_ws<SM_LINEAR>(wc, ws, f) : dispatch_ws<SM_CUBIC>(wc, ws, f);
+auto dispatch(int sm, int wc, int ws, bool otf, const F& f) {
+  if (otf) return sm == SM_LINEAR ? dispatch_ws<SM_LINEAR, true>(wc, ws, f) : dispatch_ws<SM_CUBIC, true>(wc, ws, f);
+  return sm == SM_LINEAR ? dispatch_ws<SM_LINEAR, false>(wc, ws, f) : dispatch_ws<SM_CUBIC, false>(wc, ws, f);
 }
 
 struct OccupancyFn {
-  template <int SM, int WC, int WS>
+  template <int SM, int WC, int WS, bool OTF>
   int operator()() const {
-    return batch_occupancy_t<SM, WC, WS>();
+    return batch_occupancy_t<SM, WC, WS, OTF>();
   }
 };
 
@@ -156,9 +162,9 @@ struct LaunchFn {
   const BatchLaunch& B;
   int nblocks;
   hipStream_t stream;
-  template <int SM, int WC, int WS>
+  template <int SM, int WC, int WS, bool OTF>
   hipError_t operator()() const {
-    return batch_launch_t<SM, WC, WS>(B, nblocks, stream);
+    return batch_launch_t<SM, WC, WS, OTF>(B, nblocks, stream);
   }
 };
 
@@ -170,6 +176,7 @@ struct ustbx_batch {
   int device;
   int count;
   int sm;
+  bool otf;  // some plan computes its receive state on the fly
   int wc, ws;
   int nblocks;
   int64_t n_items;
@@ -193,7 +200,7 @@ int set_shape(ustbx_batch* b, int wc, int ws, ustbx_status* st) {
   int cus = 0;
   hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
   if (e != hipSuccess) return hip_fail(st, e, "hipDeviceGetAttribute");
-  const int occ = dispatch(b->sm, wc, ws, OccupancyFn{});
+  const int occ = dispatch(b->sm, wc, ws, b->otf, OccupancyFn{});
   if (occ <= 0) return set_status(st, USTBX_E_GPU, "the batch kernel does not fit on a compute unit");
   b->wc = wc;
   b->ws = ws;
@@ -214,7 +221,7 @@ hipError_t launch(const ustbx_batch* b, hipStream_t stream) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   BatchLaunch B{b->d_acqs, b->d_items, b->d_qoff, b->d_ctr};
-  return dispatch(b->sm, b->wc, b->ws, LaunchFn{B, b->nblocks, stream});
+  return dispatch(b->sm, b->wc, b->ws, b->otf, LaunchFn{B, b->nblocks, stream});
 }
 
 }  // namespace
@@ -248,6 +255,8 @@ int ustbx_batch_create(const ustbx_plan* const* plans, int32_t count, ustbx_batc
   b->device = device;
   b->count = count;
   b->sm = interp == USTBX_INTERP_CUBIC ? SM_CUBIC : SM_LINEAR;
+  b->otf = false;
+  for (int i = 0; i < count; ++i) b->otf = b->otf || plans[i]->G.rx_otf != 0;
   b->plans.assign(plans, plans + count);
 
   // queues: per acquisition (largest work first) the XCD shares of its tiles
@@ -420,7 +429,7 @@ struct ustbx_batcher {
   int device;
   int sm;
   int wc, ws;
-  int nblocks;
+  int nblocks[2];  // the kernel without and with the on-the-fly form
   int max_acqs;
   int64_t max_items;
   char* dev;         // queue offsets | descriptors | items | counters
@@ -460,8 +469,10 @@ int ustbx_batcher_create(int32_t interp, int32_t max_acqs, int64_t max_items, in
   b->max_items = max_items;
   int cus = 0;
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device);
-  const int occ = dispatch(b->sm, b->wc, b->ws, OccupancyFn{});
-  b->nblocks = ((cus * std::max(occ, 1) + kQueues - 1) / kQueues) * kQueues;
+  for (int o = 0; o < 2; ++o) {
+    const int occ = dispatch(b->sm, b->wc, b->ws, o == 1, OccupancyFn{});
+    b->nblocks[o] = ((cus * std::max(occ, 1) + kQueues - 1) / kQueues) * kQueues;
+  }
   auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
   b->b_qoff = al(sizeof(int) * (kQueues + 1));
   b->b_acq = al(sizeof(RxAcq) * (size_t)max_acqs);
@@ -569,7 +580,9 @@ int ustbx_batcher_submit(ustbx_batcher* b, int32_t count, const ustbx_plan* cons
     BatchLaunch B{reinterpret_cast<const RxAcq*>(b->dev + b->b_qoff),
                   reinterpret_cast<const int4*>(b->dev + b->b_qoff + b->b_acq), reinterpret_cast<const int*>(b->dev),
                   d_ctr};
-    e = dispatch(b->sm, b->wc, b->ws, LaunchFn{B, b->nblocks, s});
+    bool otf = false;
+    for (int i = 0; i < count; ++i) otf = otf || plans[i]->G.rx_otf != 0;
+    e = dispatch(b->sm, b->wc, b->ws, otf, LaunchFn{B, b->nblocks[otf ? 1 : 0], s});
   }
   if (e == hipSuccess) e = hipEventRecord(b->launched, s);
   if (e != hipSuccess) return hip_fail(st, e, "das_batch_kernel launch");
