@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; C4: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 3; C4: 1)")
-    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4", "C5", "RTB", "RF2IQ", "UFF"],
+    ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4", "C4C", "C5", "RTB", "RF2IQ", "UFF"],
                     help="RTB: focused transmits (retrospective transmit beamforming), the focused-wave "
                          "delay models; RF2IQ: the RF front end (fast_demodulation and hilbert) on STA RF "
                          "of 128 waves x 128 channels x 2048 samples; UFF: C3's frames streamed from a UFF file through the reader "

@@ -349,4 +349,14 @@ def rtb(seed: int = 6, scale: float = 1.0, with_data: bool = True, n_speckle: in
     return acq
 
 
-BUILDERS = {"C1": c1, "C2": c2, "C3": c3, "RTB": rtb}
+def c4c(seed: int = 4, scale: float = 1.0, with_data: bool = True, n_speckle: int = 2000) -> Acquisition:
+    """C4's 3-D class on its own: one frame of the 32 x 32 matrix array, 25
+    diverging waves, 128 x 128 x 256 voxels (the first dataset of C4's 3-D
+    kind)."""
+    acq = _c4_3d(scale)
+    if with_data:
+        acq.data = _c4_data(acq, np.random.default_rng(seed * 100 + 20), max(50, int(round(n_speckle * scale))))
+    return acq
+
+
+BUILDERS = {"C1": c1, "C2": c2, "C3": c3, "C4C": c4c, "RTB": rtb}
