@@ -56,7 +56,6 @@ template <int SM, int FC, int PH, bool SAFE, bool STAGE>
 __device__ __forceinline__ void tx_channel(const RxAcq& A, float wd, int tile, int n, int f0, int nf, int lane,
                                            float2 (&v)[FC], float2* __restrict__ win) {
   using M = SampleMode<SM>;
-  constexpr int TP = 64;
   constexpr int ST = M::kStride;
   const int p = tile_pixel(A.tile_ps, A.tile_n0, tile, lane);
   const bool active = p < A.P;
