@@ -73,6 +73,10 @@ def geom_cases():
     # that a tile's sample windows fit the staged form's 64-sample rows
     cases["dw_sector_patch"] = dict(cases["dw_sector"], scan=SectorScan(np.deg2rad(np.linspace(-5, 5, 12)),
                                                                         np.linspace(20e-3, 26e-3, 48)))
+    # CPWC from depth 0 (NaN receive weights on the first row) on 48 depths x
+    # 24 lines: patch tiles that take the SAFE body in dimension transmit
+    cases["cpwc_depth0_patch"] = dict(cases["cpwc_depth0"], scan=LinearScan(np.linspace(-5e-3, 5e-3, 24),
+                                                                              np.linspace(0.0, 30e-3, 48)))
     # RF plane wave on a linear scan of 32 depths x 8 lines (patch tiles)
     cases["pw_rf_patch"] = dict(cases["pw_rf_hanning"], scan=LinearScan(np.linspace(-4e-3, 4e-3, 8),
                                                                          np.linspace(5e-3, 25e-3, 32)))

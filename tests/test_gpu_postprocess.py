@@ -104,7 +104,7 @@ def test_coherence_factor_matches_oracle(cuda, dim, flavour):
 
 
 @pytest.mark.parametrize("name", ["dw_sector", "dw_sector_patch", "cpwc_tukey", "pw_rf_hanning", "pw_rf_patch",
-                                  "cpwc_depth0", "focused_hybrid"])
+                                  "cpwc_depth0", "cpwc_depth0_patch", "focused_hybrid"])
 @pytest.mark.parametrize("separable", [True, False])
 def test_fused_coherence_factor_equals_unfused(cuda, name, separable):
     """ustbx_plan_run_cf against ustbx_plan_run + ustbx_coherence_factor
