@@ -38,8 +38,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 20; C4: 5)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 3; C4: 1)")
     ap.add_argument("--config", default="C2", choices=["C1", "C2", "C3", "C4", "C4C", "C5", "RTB", "RF2IQ", "UFF"],
-                    help="RTB: focused transmits (retrospective transmit beamforming), the focused-wave "
-                         "delay models; RF2IQ: the RF front end (fast_demodulation and hilbert) on STA RF "
+                    help="C4C: C4's 3-D class alone (one frame of the 32 x 32 matrix array, 25 waves, "
+                         "128 x 128 x 256 voxels); RTB: focused transmits (retrospective transmit beamforming), the "
+                         "focused-wave delay models; RF2IQ: the RF front end (fast_demodulation and hilbert) on STA RF "
                          "of 128 waves x 128 channels x 2048 samples; UFF: C3's frames streamed from a UFF file through the reader "
                          "into a one-frame plan")
     ap.add_argument("--tx-model", default="hybrid", choices=["spherical", "unified", "hybrid", "blended"],
