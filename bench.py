@@ -906,7 +906,8 @@ def run_serve(args, torch, dev):
             p.set_staging(bool(args.stage))
     torch.cuda.synchronize()
     log(f"C5: {len(cfgs)} configurations, data and plans in {time.time() - t:.1f}s")
-    srv = Server(plans, datasets, hbm_budget=64 << 30, slice_ms=args.slice_ms, tiles=(8, 4),
+    srv = Server(plans, datasets, hbm_budget=64 << 30, slice_ms=args.slice_ms,
+                 tiles=(args.wave_chunk or 16, args.split or 4),
                  long_slices=args.long_slices, depth=args.depth, **idle_settings(args.idle))
     log(f"calibrated ms per position {[round(x, 4) for x in srv.ms_per_pos]}, positions {srv.positions}")
     names = [c.name for c in cfgs]

@@ -162,7 +162,7 @@ class ServeResult:
 class Server:
     def __init__(self, plans: list[GeometryPlan], datasets: list[list[np.ndarray]], *,
                  hbm_budget: int = 64 << 30, slice_ms: float = 2.0, depth: int = 2,
-                 tiles: tuple[int, int] = (8, 4), max_acqs: int = 4096, copy_window: int = 512 << 20,
+                 tiles: tuple[int, int] = (16, 4), max_acqs: int = 4096, copy_window: int = 512 << 20,
                  long_slices: int = 8, idle_slice_ms: float | None = 1.0, idle_long_slices: int | None = 4,
                  idle_depth: int | None = 1, idle_below_ms: float = 250.0):
         """slice_ms / long_slices / depth: iteration length (short while a
