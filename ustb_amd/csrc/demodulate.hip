@@ -17,13 +17,14 @@
 //
 // The mixing factors depend on the sample index only: they are computed once
 // per (device, L, fs, t0, fmod) into a small device table that later calls
-// reuse. One block per (trace, run of up to kR * kDB outputs) stages the
+// reuse. One block per (trace, run of up to R * kDB outputs) stages the
 // mixed samples of its input window in LDS, split by decimation phase
-// (phase r holds window samples r, r + down, ...; one pad slot every 4
-// entries, so the lanes' reads, kR entries apart, hit distinct banks). Each
-// lane forms kR consecutive outputs: per phase, a window of kR samples slides
-// along the phase's taps in registers, so every staged sample a lane needs is
-// read once. The sum runs phase by phase (within a phase in tap order).
+// (phase r holds window samples r, r + down, ...; one pad slot every R
+// entries, so the lanes' reads, R entries apart, hit distinct banks). Each
+// lane forms R consecutive outputs (R = 8 without decimation, 4 with): per
+// phase, a window of R samples slides along the phase's taps in registers, so
+// every staged sample a lane needs is read once. The sum runs phase by phase
+// (within a phase in tap order), whatever R is.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -39,11 +40,10 @@ namespace ustbx {
 namespace {
 
 constexpr int kDB = 256;         // threads per block
-constexpr int kR = 4;            // consecutive outputs per lane
 // window samples a lane loads per staging round: 9 x 256 covers the window of
-// 1024 outputs at decimation 2 with up to 242 taps (and at decimation 1 with
-// any) in one round, so a block waits for memory once (8: two rounds, 12 %
-// slower on 128 x 128 traces of 2048 samples, 31 taps)
+// 1024 outputs at decimation 2, or 2048 at decimation 1, with up to 241 taps
+// in one round, so a block waits for memory once (8: two rounds, 12 % slower
+// on 128 x 128 traces of 2048 samples, 31 taps, decimation 2)
 constexpr int kSB = 9;
 constexpr int kMaxTaps = 767;    // passed by value (kernel arguments stay under 4 KB); zero-phase: K <= 384
 constexpr size_t kLdsMax = 64 * 1024;
@@ -60,14 +60,14 @@ struct DemodArgs {
 
 // entries of one phase for m window samples (one pad slot per 4) and the
 // dynamic LDS of a block of OB outputs
-inline int phase_slots(int mr) { return mr + (mr >> 2) + 1; }
-inline int phase_len(int OB, int down, int T) {
+inline int phase_slots(int mr, int R) { return mr + mr / R + 1; }
+inline int phase_len(int OB, int down, int T, int R) {
   const int count = (OB - 1) * down + T;
-  return (count + down - 1) / down + 2 * kR;
+  return (count + down - 1) / down + 2 * R;
 }
-inline size_t demod_lds(int OB, int down, int T) {
-  return sizeof(float2) * (size_t)down * phase_slots(phase_len(OB, down, T)) +
-         sizeof(float) * (size_t)(T + (kR + 1) * down);
+inline size_t demod_lds(int OB, int down, int T, int R) {
+  return sizeof(float2) * (size_t)down * phase_slots(phase_len(OB, down, T, R), R) +
+         sizeof(float) * (size_t)(T + (R + 1) * down);
 }
 
 __global__ __launch_bounds__(kDB) void mix_table_kernel(float2* __restrict__ tab, int L, double fs, double t0,
@@ -89,8 +89,10 @@ __device__ __forceinline__ float2 mixed(const float* __restrict__ x, int m, cons
   return make_float2(v * e.x, v * e.y);
 }
 
-__device__ __forceinline__ int slot(int m) { return m + (m >> 2); }
+template <int kR>
+__device__ __forceinline__ int slot(int m) { return m + m / kR; }
 
+template <int kR>
 __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
   extern __shared__ float2 lds[];
   const int down = A.down, T = A.ntaps, PS = A.PS;
@@ -140,7 +142,7 @@ __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
         }
         if (i >= count) v = make_float2(0.0f, 0.0f);
         const int q = i / down;
-        lds[(size_t)(i - q * down) * PS + slot(q)] = v;
+        lds[(size_t)(i - q * down) * PS + slot<kR>(q)] = v;
       }
     }
   }
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
     // w[s] holds phase sample jt + q0 + s at the start of each block of kR taps
     float2 w[kR];
 #pragma unroll
-    for (int k = 0; k < kR; ++k) w[k] = ph[slot(jt + k)];
+    for (int k = 0; k < kR; ++k) w[k] = ph[slot<kR>(jt + k)];
     for (int q0 = 0; q0 < Q; q0 += kR) {
 #pragma unroll
       for (int u = 0; u < kR; ++u) {
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(kDB) void demodulate_kernel(DemodArgs A) {
           acc[k].x = fmaf(t, s.x, acc[k].x);
           acc[k].y = fmaf(t, s.y, acc[k].y);
         }
-        w[u] = ph[slot(jt + q0 + u + kR)];
+        w[u] = ph[slot<kR>(jt + q0 + u + kR)];
       }
     }
   }
@@ -264,10 +266,15 @@ int ustbx_demodulate(const float* rf, void* iq, int32_t L, int64_t traces, doubl
   if (L == 0 || traces == 0) return ok(st);
   if (traces > 0x7FFFFFFF) return set_status(st, USTBX_E_ARG, "too many traces");
   if (!rf || !iq) return set_status(st, USTBX_E_ARG, "NULL buffer");
-  // outputs per block: kR per lane, fewer where the window would not fit in LDS
-  int OB = kR * kDB;
-  while (OB > kR && demod_lds(OB, down, T) > kLdsMax) OB /= 2;
-  if (demod_lds(OB, down, T) > kLdsMax)
+  // outputs per lane: 8 halves the LDS reads per FMA (the zero-phase form,
+  // decimation 1, is VALU-bound: 14 % faster than 4); with decimation the
+  // traces have fewer outputs (1024 of 2048 samples at 2), which 4 per lane
+  // spreads over every lane of a block
+  const int R = down == 1 ? 8 : 4;
+  // outputs per block: R per lane, fewer where the window would not fit in LDS
+  int OB = R * kDB;
+  while (OB > R && demod_lds(OB, down, T, R) > kLdsMax) OB /= 2;
+  if (demod_lds(OB, down, T, R) > kLdsMax)
     return set_status(st, USTBX_E_ARG, "decimation factor too large for the filter");
   // taps in FP32 (MATLAB filter runs in the data's single precision); the
   // zero-phase form's autocorrelation is formed in FP64 first
@@ -297,12 +304,16 @@ int ustbx_demodulate(const float* rf, void* iq, int32_t L, int64_t traces, doubl
   A.off0 = -(K - 1);
   A.OB = OB;
   A.mode = mode;
-  A.PS = phase_slots(phase_len(OB, down, T));
-  A.TP = T + (kR + 1) * down;
+  A.PS = phase_slots(phase_len(OB, down, T, R), R);
+  A.TP = T + (R + 1) * down;
   const int64_t gy = (A.Lout + A.OB - 1) / A.OB;
   if (gy > 65535) return set_status(st, USTBX_E_ARG, "trace too long for the decimation factor");
-  hipLaunchKernelGGL(demodulate_kernel, dim3((unsigned)traces, (unsigned)gy), dim3(kDB), demod_lds(OB, down, T), s,
-                     A);
+  if (R == 8)
+    hipLaunchKernelGGL(demodulate_kernel<8>, dim3((unsigned)traces, (unsigned)gy), dim3(kDB),
+                       demod_lds(OB, down, T, R), s, A);
+  else
+    hipLaunchKernelGGL(demodulate_kernel<4>, dim3((unsigned)traces, (unsigned)gy), dim3(kDB),
+                       demod_lds(OB, down, T, R), s, A);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(st, e, "demodulate_kernel launch");
   return ok(st);
