@@ -298,6 +298,10 @@ def _timed(torch, stream, fn, warmup, steps):
     e1.record(stream)
     torch.cuda.synchronize()
     w_ms = (time.perf_counter() - t0) * 1e3 / steps
+    host_paced = e0.elapsed_time(e1) / steps
+    if not hasattr(torch.cuda, "_sleep"):  # no spin kernel in this torch build: the host-paced events
+        log("torch.cuda._sleep is missing: kernel times are host-paced")
+        return host_paced, w_ms
     # spin long enough for the host to enqueue every step (twice the wall time)
     cal = 1 << 20
     with torch.cuda.stream(stream):
